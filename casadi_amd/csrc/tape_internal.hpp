@@ -1,0 +1,75 @@
+// Handle types behind the cxh_* C ABI (host side, shared by the .hip units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sx_ops.h"
+
+namespace sxh {
+
+void hip_check(hipError_t e, const char* what);
+int device_lds_bytes();
+void allow_lds(const void* kernel, size_t bytes);
+int64_t round_up(int64_t a, int64_t b);
+
+// Owning device allocation that only grows.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  ~DeviceBuffer();
+  void reserve(size_t nbytes);
+  void upload(const void* src, size_t nbytes);
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+// One input or output block: nonzero j of instance k at p[j*s_nz + k*s_inst].
+struct IOArg {
+  const double* p;
+  int64_t s_nz;
+  int64_t s_inst;
+};
+
+// At most this many input / output blocks per tape; the tables travel by
+// value in the kernel arguments.
+#define CXH_MAX_IO 16
+struct IOTable {
+  IOArg a[CXH_MAX_IO];
+};
+IOTable make_io(const double* const* p, const std::vector<int64_t>& nnz, int32_t layout, int64_t ld);
+
+void validate_tape(const std::vector<Instr>& a, int64_t ws, const std::vector<int64_t>& nnz_in,
+                   const std::vector<int64_t>& nnz_out);
+
+}  // namespace sxh
+
+struct cxh_tape_s {
+  std::vector<sxh::Instr> alg;
+  int64_t ws = 1;
+  std::vector<int64_t> nnz_in, nnz_out;
+  std::vector<int2> unused_in;
+  sxh::DeviceBuffer d_tape, d_unused;
+  sxh::DeviceBuffer scratch, trail, ckpt;
+  int block = 64;
+  int w_lds = 1;
+};
+
+struct cxh_jac_s {
+  cxh_tape_s* tape = nullptr;
+  int ndir = 0;
+  int64_t nnz_jac = 0;
+  std::vector<int> in_off, out_off;
+  sxh::DeviceBuffer d_seed, d_scatter, d_in_off, d_out_off;
+};

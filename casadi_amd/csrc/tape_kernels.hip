@@ -1,0 +1,782 @@
+// gfx950 kernels that evaluate CasADi SX tapes over batches of instances,
+// and the C ABI entry points that launch them.
+//
+// One lane evaluates one instance.  Every lane of a wavefront walks the same
+// instruction stream, so instruction fetch, the op dispatch and all slot
+// indices are wave-uniform (scalar loads and branches); only the values differ
+// per lane.  The work vector w[worksize] of every lane lives in LDS laid out
+// [slot][lane] (a wave reads 64 consecutive doubles per slot: conflict-free),
+// or, when it does not fit, in HBM scratch laid out [slot][global lane]
+// (coalesced 512 B per wave access).  Inputs and outputs are per-nonzero
+// structure-of-arrays [nz][instance] or CasADi's instance-major Map layout.
+//
+// Semantics restated:
+//   forward  SXFunction::eval          casadi/core/sx_function.cpp:72-127
+//   adjoint  SXFunction::ad_reverse    casadi/core/sx_function.cpp:1368-1569
+//   tangent  SXFunction::ad_forward    casadi/core/sx_function.cpp:1194-1366
+// evaluated numerically instead of symbolically.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/casadi_hip.h"
+#include "capi_common.hpp"
+#include "sx_ops.h"
+#include "tape_internal.hpp"
+
+using namespace sxh;
+
+namespace sxh {
+
+thread_local std::string g_last_error;
+int cxh_fail(const std::string& msg) {
+  g_last_error = msg;
+  return -1;
+}
+void cxh_clear_error() { g_last_error.clear(); }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DeviceBuffer::~DeviceBuffer() {
+  if (ptr) (void)hipFree(ptr);
+}
+void DeviceBuffer::reserve(size_t nbytes) {
+  if (nbytes <= bytes) return;
+  if (ptr) hip_check(hipFree(ptr), "hipFree");
+  ptr = nullptr;
+  bytes = 0;
+  hip_check(hipMalloc(&ptr, nbytes), "hipMalloc");
+  bytes = nbytes;
+}
+void DeviceBuffer::upload(const void* src, size_t nbytes) {
+  reserve(std::max<size_t>(nbytes, 16));
+  if (nbytes) hip_check(hipMemcpy(ptr, src, nbytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+// Largest dynamic LDS allocation a block may opt into (160 KB on gfx950).
+int device_lds_bytes() {
+  static int lds = [] {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0)
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        v = 64 * 1024;
+    return v;
+  }();
+  return lds;
+}
+
+// Raise a kernel's dynamic-LDS limit before launching it with `bytes`.
+void allow_lds(const void* kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    hip_check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)),
+              "hipFuncSetAttribute");
+}
+
+}  // namespace sxh
+
+extern "C" const char* cxh_last_error(void) { return sxh::g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Work-vector views: [slot][lane] in LDS or [slot][global lane] in HBM.
+// ---------------------------------------------------------------------------
+struct WLds {
+  double* base;  // LDS, already offset by the lane
+  int stride;
+  __device__ double& operator[](int s) const { return base[s * stride]; }
+};
+struct WGlb {
+  double* base;  // HBM scratch, already offset by the global lane
+  int64_t stride;
+  __device__ double& operator[](int s) const { return base[s * stride]; }
+};
+
+// Per-instance address of nonzero j: p + j*s_nz + k*s_inst
+__device__ __forceinline__ const double* io_addr(const IOArg& a, int j, int64_t k) {
+  return a.p + j * a.s_nz + k * a.s_inst;
+}
+
+__device__ __forceinline__ double instr_const(const Instr& e) { return e.d; }
+
+// Load one instruction with a single 16-byte wave-uniform load.
+__device__ __forceinline__ Instr load_instr(const Instr* __restrict__ tape, int t) {
+  Instr e;
+  const int4 v = reinterpret_cast<const int4*>(tape)[t];
+  e.op = v.x;
+  e.i0 = v.y;
+  e.i1 = v.z;
+  e.i2 = v.w;
+  return e;
+}
+
+// Execute instruction e of the primal sweep on work vector W.
+template <class W>
+__device__ __forceinline__ void primal_step(const Instr& e, W& w, const IOTable& arg, const IOTable& res,
+                                            int64_t k, bool active) {
+  switch (e.op) {
+    case OP_CONST: w[e.i0] = instr_const(e); break;
+    case OP_INPUT: {
+      const IOArg a = arg.a[e.i1];
+      w[e.i0] = a.p ? *io_addr(a, e.i2, k) : 0.0;
+    } break;
+    case OP_OUTPUT: {
+      const IOArg r = res.a[e.i0];
+      if (r.p && active) *const_cast<double*>(io_addr(r, e.i2, k)) = w[e.i1];
+    } break;
+    default: {
+      const double x = w[e.i1], y = w[e.i2];
+      w[e.i0] = op_fun(e.op, x, y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward evaluation
+// ---------------------------------------------------------------------------
+template <bool LDS>
+__global__ void __launch_bounds__(256)
+tape_fwd_kernel(const Instr* __restrict__ tape, int n_instr, int ws, const IOTable arg, const IOTable res,
+                int64_t n, double* __restrict__ wglob, int64_t wstride) {
+  extern __shared__ double lds[];
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = k < n;
+  const int64_t kk = active ? k : n - 1;  // inactive lanes re-read a valid instance
+  if constexpr (LDS) {
+    WLds w{lds + threadIdx.x, static_cast<int>(blockDim.x)};
+    for (int t = 0; t < n_instr; ++t) primal_step(load_instr(tape, t), w, arg, res, kk, active);
+  } else {
+    WGlb w{wglob + k, wstride};
+    for (int t = 0; t < n_instr; ++t) primal_step(load_instr(tape, t), w, arg, res, kk, active);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward + adjoint with checkpointed segments
+//
+// Forward: run the tape; at the start of every segment but the first, save
+// the work vector (checkpoint).  Reverse: for each segment, last first,
+// restore its checkpoint and re-run it forward recording, for every
+// instruction, the value its destination slot held before (the trail); then
+// walk the segment backwards: read f = w[i0], restore w[i0] from the trail,
+// read the operands x = w[i1], y = w[i2], form the partials and push the
+// adjoint seed of slot i0 into slots i1 / i2.  The trail of the last segment
+// is recorded during the forward pass already.
+// ---------------------------------------------------------------------------
+struct AdjArgs {
+  const Instr* tape;
+  int n_instr, ws, seg_len, n_seg;
+  IOTable arg;
+  IOTable res;           // primal outputs (forward pass)
+  IOTable aseed;         // adjoint seeds, one per output
+  IOTable asens;         // adjoint sensitivities, one per input
+  const int2* unused_in; // (input, nz) pairs read by no INPUT instruction
+  int n_unused;
+  int64_t n;
+  double* trail;         // [seg_len][lanes]
+  double* ckpt;          // [n_seg][ws][lanes]
+  double* wglob;         // [2*ws][lanes] when the work vectors live in HBM
+  int64_t lanes;         // stride of the HBM buffers
+  int mode;              // 0 fwd+adj, 1 fwd + all checkpoints, 2 adj from checkpoints
+};
+
+// lane: index into the per-lane scratch (trail); inst: instance for I/O
+// (equal to lane for active lanes, clamped to a valid instance otherwise).
+template <class W>
+__device__ __forceinline__ void fwd_segment(const AdjArgs& A, W& w, int t0, int t1, int64_t lane,
+                                            int64_t inst, bool active, bool record, bool write_res) {
+  for (int t = t0; t < t1; ++t) {
+    const Instr e = load_instr(A.tape, t);
+    if (record && e.op != OP_OUTPUT) A.trail[static_cast<int64_t>(t - t0) * A.lanes + lane] = w[e.i0];
+    if (e.op == OP_OUTPUT) {
+      if (write_res) {
+        const IOArg r = A.res.a[e.i0];
+        if (r.p && active) *const_cast<double*>(io_addr(r, e.i2, inst)) = w[e.i1];
+      }
+    } else {
+      primal_step(e, w, A.arg, A.res, inst, active);
+    }
+  }
+}
+
+template <class W>
+__device__ __forceinline__ void rev_segment(const AdjArgs& A, W& w, W& aw, int t0, int t1, int64_t lane,
+                                            int64_t k, bool active) {
+  for (int t = t1 - 1; t >= t0; --t) {
+    const Instr e = load_instr(A.tape, t);
+    if (e.op == OP_OUTPUT) {
+      const IOArg s = A.aseed.a[e.i0];
+      if (s.p) aw[e.i1] += *io_addr(s, e.i2, k);
+      continue;
+    }
+    const double told = A.trail[static_cast<int64_t>(t - t0) * A.lanes + lane];
+    if (e.op == OP_INPUT) {
+      const IOArg s = A.asens.a[e.i1];
+      if (s.p && active) *const_cast<double*>(io_addr(s, e.i2, k)) = aw[e.i0];
+      aw[e.i0] = 0.0;
+      w[e.i0] = told;
+      continue;
+    }
+    if (e.op == OP_CONST) {
+      aw[e.i0] = 0.0;
+      w[e.i0] = told;
+      continue;
+    }
+    const double f = w[e.i0];
+    w[e.i0] = told;
+    const double x = w[e.i1], y = w[e.i2];
+    const double seed = aw[e.i0];
+    aw[e.i0] = 0.0;
+    double d[2];
+    op_der(e.op, x, y, f, d);
+    if (e.op == OP_IF_ELSE_ZERO) {
+      aw[e.i2] += sxh_if_else_zero(d[1], seed);
+    } else if (op_is_binary(e.op)) {
+      aw[e.i1] += d[0] * seed;
+      aw[e.i2] += d[1] * seed;
+    } else {
+      aw[e.i1] += d[0] * seed;
+    }
+  }
+}
+
+template <class W>
+__device__ void adj_body(const AdjArgs& A, W& w, W& aw, int64_t k, int64_t kk, bool active) {
+  const int ws = A.ws;
+  for (int s = 0; s < ws; ++s) {
+    w[s] = 0.0;
+    aw[s] = 0.0;
+  }
+  const int last = A.n_seg - 1;
+  if (A.mode != 2) {
+    // forward pass
+    for (int sg = 0; sg < A.n_seg; ++sg) {
+      const int t0 = sg * A.seg_len, t1 = min(A.n_instr, t0 + A.seg_len);
+      if (sg > 0 || A.mode == 1) {
+        double* c = A.ckpt + static_cast<int64_t>(sg) * ws * A.lanes + k;
+        for (int s = 0; s < ws; ++s) c[static_cast<int64_t>(s) * A.lanes] = w[s];
+      }
+      fwd_segment(A, w, t0, t1, k, kk, active, A.mode == 0 && sg == last, true);
+    }
+    if (A.mode == 1) return;
+  }
+  // reverse pass
+  for (int sg = last; sg >= 0; --sg) {
+    const int t0 = sg * A.seg_len, t1 = min(A.n_instr, t0 + A.seg_len);
+    if (A.mode == 2 || sg != last) {
+      const double* c = A.ckpt + static_cast<int64_t>(sg) * ws * A.lanes + k;
+      if (sg > 0 || A.mode == 2)
+        for (int s = 0; s < ws; ++s) w[s] = c[static_cast<int64_t>(s) * A.lanes];
+      fwd_segment(A, w, t0, t1, k, kk, active, true, false);
+    }
+    rev_segment(A, w, aw, t0, t1, k, kk, active);
+  }
+  if (active) {
+    for (int u = 0; u < A.n_unused; ++u) {
+      const int2 p = A.unused_in[u];
+      const IOArg s = A.asens.a[p.x];
+      if (s.p) *const_cast<double*>(io_addr(s, p.y, k)) = 0.0;
+    }
+  }
+}
+
+template <bool LDS>
+__global__ void __launch_bounds__(256) tape_adj_kernel(AdjArgs A) {
+  extern __shared__ double lds[];
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = k < A.n;
+  const int64_t kk = active ? k : A.n - 1;
+  if constexpr (LDS) {
+    WLds w{lds + threadIdx.x, static_cast<int>(blockDim.x)};
+    WLds aw{lds + static_cast<int64_t>(A.ws) * blockDim.x + threadIdx.x, static_cast<int>(blockDim.x)};
+    adj_body(A, w, aw, k, kk, active);
+  } else {
+    WGlb w{A.wglob + k, A.lanes};
+    WGlb aw{A.wglob + static_cast<int64_t>(A.ws) * A.lanes + k, A.lanes};
+    adj_body(A, w, aw, k, kk, active);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused primal + multi-direction tangent sweep + Jacobian scatter
+// Layout of the tangents: direction d of slot s at tw[s*ndir + d].
+// ---------------------------------------------------------------------------
+struct JacArgs {
+  const Instr* tape;
+  int n_instr, ws, ndir;
+  IOTable arg;
+  IOTable res;
+  const int* in_off;      // flat offset of input i's nonzeros
+  const int* out_off;     // flat offset of output i's nonzeros
+  const double* seedval;  // [flat_in_nz][ndir]
+  const int2* scatter;    // [flat_out_nz][ndir] -> up to two Jacobian nonzeros
+  double* jac;            // Jacobian nonzeros
+  int64_t jac_s_nz, jac_s_inst;
+  int64_t n;
+  double* wglob;          // [(1+ndir)*ws][lanes] when not in LDS
+  int64_t lanes;
+};
+
+template <class W>
+__device__ void jac_body(const JacArgs& A, W& w, W& tw, int64_t k, int64_t kk, bool active) {
+  const int nd = A.ndir;
+  for (int t = 0; t < A.n_instr; ++t) {
+    const Instr e = load_instr(A.tape, t);
+    switch (e.op) {
+      case OP_CONST:
+        w[e.i0] = instr_const(e);
+        for (int d = 0; d < nd; ++d) tw[e.i0 * nd + d] = 0.0;
+        break;
+      case OP_INPUT: {
+        const IOArg a = A.arg.a[e.i1];
+        w[e.i0] = a.p ? *io_addr(a, e.i2, kk) : 0.0;
+        const double* sv = A.seedval + static_cast<int64_t>(A.in_off[e.i1] + e.i2) * nd;
+        for (int d = 0; d < nd; ++d) tw[e.i0 * nd + d] = sv[d];
+      } break;
+      case OP_OUTPUT: {
+        const IOArg r = A.res.a[e.i0];
+        if (r.p && active) *const_cast<double*>(io_addr(r, e.i2, k)) = w[e.i1];
+        const int2* sc = A.scatter + static_cast<int64_t>(A.out_off[e.i0] + e.i2) * nd;
+        if (active) {
+          for (int d = 0; d < nd; ++d) {
+            const int2 tgt = sc[d];
+            const double v = tw[e.i1 * nd + d];
+            if (tgt.x >= 0) A.jac[tgt.x * A.jac_s_nz + k * A.jac_s_inst] = v;
+            if (tgt.y >= 0) A.jac[tgt.y * A.jac_s_nz + k * A.jac_s_inst] = v;
+          }
+        }
+      } break;
+      default: {
+        const double x = w[e.i1], y = w[e.i2];
+        const double f = op_fun(e.op, x, y);
+        double dd[2];
+        op_der(e.op, x, y, f, dd);
+        if (e.op == OP_IF_ELSE_ZERO) {
+          for (int d = 0; d < nd; ++d) tw[e.i0 * nd + d] = sxh_if_else_zero(dd[1], tw[e.i2 * nd + d]);
+        } else if (op_is_binary(e.op)) {
+          for (int d = 0; d < nd; ++d)
+            tw[e.i0 * nd + d] = dd[0] * tw[e.i1 * nd + d] + dd[1] * tw[e.i2 * nd + d];
+        } else {
+          for (int d = 0; d < nd; ++d) tw[e.i0 * nd + d] = dd[0] * tw[e.i1 * nd + d];
+        }
+        w[e.i0] = f;
+      }
+    }
+  }
+}
+
+template <bool LDS>
+__global__ void __launch_bounds__(256) tape_jac_kernel(JacArgs A) {
+  extern __shared__ double lds[];
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = k < A.n;
+  const int64_t kk = active ? k : A.n - 1;
+  if constexpr (LDS) {
+    WLds w{lds + threadIdx.x, static_cast<int>(blockDim.x)};
+    WLds tw{lds + static_cast<int64_t>(A.ws) * blockDim.x + threadIdx.x, static_cast<int>(blockDim.x)};
+    jac_body(A, w, tw, k, kk, active);
+  } else {
+    WGlb w{A.wglob + k, A.lanes};
+    WGlb tw{A.wglob + static_cast<int64_t>(A.ws) * A.lanes + k, A.lanes};
+    jac_body(A, w, tw, k, kk, active);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace sxh {
+
+// Validate a tape so that no kernel can index outside its buffers.
+void validate_tape(const std::vector<Instr>& a, int64_t ws, const std::vector<int64_t>& nnz_in,
+                   const std::vector<int64_t>& nnz_out) {
+  auto slot_ok = [&](int s) { return s >= 0 && s < ws; };
+  for (size_t t = 0; t < a.size(); ++t) {
+    const Instr& e = a[t];
+    const std::string where = "tape instruction " + std::to_string(t) + ": ";
+    switch (e.op) {
+      case OP_CONST:
+        if (!slot_ok(e.i0)) throw std::runtime_error(where + "slot out of range");
+        break;
+      case OP_INPUT:
+        if (!slot_ok(e.i0) || e.i1 < 0 || e.i1 >= static_cast<int>(nnz_in.size()) || e.i2 < 0 ||
+            e.i2 >= nnz_in[e.i1])
+          throw std::runtime_error(where + "input index out of range");
+        break;
+      case OP_OUTPUT:
+        if (!slot_ok(e.i1) || e.i0 < 0 || e.i0 >= static_cast<int>(nnz_out.size()) || e.i2 < 0 ||
+            e.i2 >= nnz_out[e.i0])
+          throw std::runtime_error(where + "output index out of range");
+        break;
+      case OP_PARAMETER:
+        throw std::runtime_error(where + "free parameter: the Function has free variables");
+      case OP_CALL:
+        throw std::runtime_error(where + "OP_CALL is not supported on the GPU tape");
+      default:
+        if (!op_is_arith(e.op)) throw std::runtime_error(where + "unsupported operation " + std::to_string(e.op));
+        if (!slot_ok(e.i0) || !slot_ok(e.i1) || !slot_ok(e.i2))
+          throw std::runtime_error(where + "slot out of range");
+    }
+  }
+}
+
+// Kernel-argument table of the I/O blocks (passed by value, no staging copy).
+IOTable make_io(const double* const* p, const std::vector<int64_t>& nnz, int32_t layout, int64_t ld) {
+  IOTable v{};
+  if (nnz.size() > CXH_MAX_IO) throw std::runtime_error("too many input/output blocks");
+  for (size_t i = 0; i < nnz.size(); ++i) {
+    v.a[i].p = p ? p[i] : nullptr;
+    if (layout == CXH_SOA) {
+      v.a[i].s_nz = ld;
+      v.a[i].s_inst = 1;
+    } else {
+      v.a[i].s_nz = 1;
+      v.a[i].s_inst = nnz[i];
+    }
+  }
+  return v;
+}
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace sxh
+
+extern "C" {
+
+int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize, int32_t n_in,
+                    const int64_t* nnz_in, int32_t n_out, const int64_t* nnz_out, cxh_tape* out) {
+  CXH_TRY
+  if (!out) return cxh_fail("cxh_tape_create: null output handle");
+  if (n_instr < 0 || worksize < 0 || n_in < 0 || n_out < 0) return cxh_fail("cxh_tape_create: negative size");
+  if (n_instr > (1 << 30) || worksize > (1 << 24)) return cxh_fail("cxh_tape_create: tape too large");
+  if (n_in > CXH_MAX_IO || n_out > CXH_MAX_IO)
+    return cxh_fail("cxh_tape_create: at most " + std::to_string(CXH_MAX_IO) + " input and output blocks");
+  auto t = std::make_unique<cxh_tape_s>();
+  t->alg.resize(n_instr);
+  if (n_instr) std::memcpy(t->alg.data(), instr, n_instr * sizeof(Instr));
+  t->ws = std::max<int64_t>(worksize, 1);
+  t->nnz_in.assign(nnz_in, nnz_in + n_in);
+  t->nnz_out.assign(nnz_out, nnz_out + n_out);
+  validate_tape(t->alg, t->ws, t->nnz_in, t->nnz_out);
+  // input nonzeros read by no INPUT instruction (their adjoint is zero)
+  std::vector<std::vector<char>> used(n_in);
+  for (int i = 0; i < n_in; ++i) used[i].assign(t->nnz_in[i], 0);
+  for (const Instr& e : t->alg)
+    if (e.op == OP_INPUT) used[e.i1][e.i2] = 1;
+  for (int i = 0; i < n_in; ++i)
+    for (int64_t j = 0; j < t->nnz_in[i]; ++j)
+      if (!used[i][j]) t->unused_in.push_back({i, static_cast<int>(j)});
+  t->d_tape.upload(t->alg.data(), t->alg.size() * sizeof(Instr));
+  t->d_unused.upload(t->unused_in.data(), t->unused_in.size() * sizeof(int2));
+  *out = t.release();
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_tape_destroy(cxh_tape t) {
+  delete t;
+  return 0;
+}
+
+int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds) {
+  if (!t) return cxh_fail("null tape");
+  if (block != -1) {
+    if (block != 64 && block != 128 && block != 256) return cxh_fail("block must be 64, 128 or 256");
+    t->block = block;
+  }
+  if (w_in_lds != -1) t->w_lds = w_in_lds;
+  return 0;
+}
+
+int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const* res, int32_t layout,
+                 int64_t ld, void* stream) {
+  CXH_TRY
+  if (!t) return cxh_fail("null tape");
+  if (n < 0) return cxh_fail("negative n");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  if (n == 0 || t->alg.empty()) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const IOTable d_arg = make_io(arg, t->nnz_in, layout, ld);
+  const IOTable d_res = make_io(res, t->nnz_out, layout, ld);
+  const int bs = t->block;
+  const int64_t grid = (n + bs - 1) / bs;
+  const size_t lds = static_cast<size_t>(t->ws) * bs * sizeof(double);
+  const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
+  if (use_lds) {
+    allow_lds(reinterpret_cast<const void*>(tape_fwd_kernel<true>), lds);
+    hipLaunchKernelGGL(tape_fwd_kernel<true>, dim3(grid), dim3(bs), lds, s, t->d_tape.as<Instr>(),
+                       static_cast<int>(t->alg.size()), static_cast<int>(t->ws), d_arg, d_res, n,
+                       nullptr, int64_t(0));
+  } else {
+    const int64_t lanes = grid * bs;
+    t->scratch.reserve(static_cast<size_t>(t->ws) * lanes * sizeof(double));
+    hipLaunchKernelGGL(tape_fwd_kernel<false>, dim3(grid), dim3(bs), 0, s, t->d_tape.as<Instr>(),
+                       static_cast<int>(t->alg.size()), static_cast<int>(t->ws), d_arg, d_res, n,
+                       t->scratch.as<double>(), lanes);
+  }
+  hip_check(hipGetLastError(), "tape_fwd_kernel launch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_map_eval_host(cxh_tape t, int64_t n, const double* const* arg, double* const* res) {
+  CXH_TRY
+  if (!t) return cxh_fail("null tape");
+  if (n <= 0) return 0;
+  const size_t ni = t->nnz_in.size(), no = t->nnz_out.size();
+  std::vector<DeviceBuffer> din(ni), dout(no);
+  std::vector<const double*> pa(ni, nullptr);
+  std::vector<double*> pr(no, nullptr);
+  for (size_t i = 0; i < ni; ++i) {
+    if (arg[i] && t->nnz_in[i] > 0) {
+      din[i].upload(arg[i], static_cast<size_t>(n) * t->nnz_in[i] * sizeof(double));
+      pa[i] = din[i].as<double>();
+    }
+  }
+  for (size_t i = 0; i < no; ++i) {
+    if (res[i] && t->nnz_out[i] > 0) {
+      dout[i].reserve(static_cast<size_t>(n) * t->nnz_out[i] * sizeof(double));
+      pr[i] = dout[i].as<double>();
+    }
+  }
+  int rc = cxh_map_eval(t, n, pa.data(), pr.data(), CXH_AOS, n, nullptr);
+  if (rc) return rc;
+  for (size_t i = 0; i < no; ++i)
+    if (pr[i])
+      hip_check(hipMemcpy(res[i], pr[i], static_cast<size_t>(n) * t->nnz_out[i] * sizeof(double),
+                          hipMemcpyDeviceToHost),
+                "hipMemcpy D2H");
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return 0;
+  CXH_CATCH
+}
+
+}  // extern "C"
+
+namespace sxh {
+
+static void segments(const cxh_tape_s* t, int64_t every, int& seg_len, int& n_seg) {
+  const int64_t ni = static_cast<int64_t>(t->alg.size());
+  if (every <= 0 || every >= ni) {
+    seg_len = static_cast<int>(std::max<int64_t>(ni, 1));
+    n_seg = 1;
+  } else {
+    seg_len = static_cast<int>(every);
+    n_seg = static_cast<int>((ni + every - 1) / every);
+  }
+}
+
+static int launch_adj(cxh_tape t, int mode, int64_t n, const double* const* arg, const double* const* aseed,
+                      double* const* res, double* const* asens, int32_t layout, int64_t ld,
+                      int64_t every, double* ckpt_ext, hipStream_t s) {
+  if (!t) return cxh_fail("null tape");
+  if (n < 0) return cxh_fail("negative n");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  if (n == 0 || t->alg.empty()) return 0;
+  AdjArgs A{};
+  A.tape = t->d_tape.as<Instr>();
+  A.n_instr = static_cast<int>(t->alg.size());
+  A.ws = static_cast<int>(t->ws);
+  segments(t, every, A.seg_len, A.n_seg);
+  A.arg = make_io(arg, t->nnz_in, layout, ld);
+  A.res = make_io(res, t->nnz_out, layout, ld);
+  A.aseed = make_io(aseed, t->nnz_out, layout, ld);
+  A.asens = make_io(asens, t->nnz_in, layout, ld);
+  A.unused_in = t->d_unused.as<int2>();
+  A.n_unused = static_cast<int>(t->unused_in.size());
+  A.n = n;
+  A.mode = mode;
+  const int bs = t->block;
+  const int64_t grid = (n + bs - 1) / bs;
+  A.lanes = grid * bs;
+  if (mode != 1) t->trail.reserve(static_cast<size_t>(A.seg_len) * A.lanes * sizeof(double));
+  A.trail = t->trail.as<double>();
+  if (ckpt_ext) {
+    A.ckpt = ckpt_ext;
+  } else if (A.n_seg > 1) {
+    t->ckpt.reserve(static_cast<size_t>(A.n_seg) * A.ws * A.lanes * sizeof(double));
+    A.ckpt = t->ckpt.as<double>();
+  }
+  const size_t lds = 2 * static_cast<size_t>(A.ws) * bs * sizeof(double);
+  const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
+  if (use_lds) {
+    allow_lds(reinterpret_cast<const void*>(tape_adj_kernel<true>), lds);
+    hipLaunchKernelGGL(tape_adj_kernel<true>, dim3(grid), dim3(bs), lds, s, A);
+  } else {
+    t->scratch.reserve(2 * static_cast<size_t>(A.ws) * A.lanes * sizeof(double));
+    A.wglob = t->scratch.as<double>();
+    hipLaunchKernelGGL(tape_adj_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
+  }
+  hip_check(hipGetLastError(), "tape_adj_kernel launch");
+  return 0;
+}
+
+}  // namespace sxh
+
+extern "C" {
+
+int cxh_map_fwd_adj(cxh_tape t, int64_t n, const double* const* arg, const double* const* adj_out,
+                    double* const* res, double* const* adj_in, int32_t layout, int64_t ld,
+                    int64_t checkpoint_every, void* stream) {
+  CXH_TRY
+  return launch_adj(t, 0, n, arg, adj_out, res, adj_in, layout, ld, checkpoint_every, nullptr,
+                    static_cast<hipStream_t>(stream));
+  CXH_CATCH
+}
+
+int cxh_ckpt_size(cxh_tape t, int64_t n, int64_t checkpoint_every, int32_t block, int64_t* n_doubles) {
+  CXH_TRY
+  if (!t || !n_doubles) return cxh_fail("null argument");
+  int seg_len, n_seg;
+  segments(t, checkpoint_every, seg_len, n_seg);
+  const int bs = block > 0 ? block : t->block;
+  const int64_t lanes = (n + bs - 1) / bs * bs;
+  *n_doubles = static_cast<int64_t>(n_seg) * t->ws * lanes;
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_map_fwd_ckpt(cxh_tape t, int64_t n, const double* const* arg, double* const* res, int32_t layout,
+                     int64_t ld, int64_t checkpoint_every, double* ckpt, void* stream) {
+  CXH_TRY
+  if (!ckpt) return cxh_fail("null checkpoint buffer");
+  return launch_adj(t, 1, n, arg, nullptr, res, nullptr, layout, ld, checkpoint_every, ckpt,
+                    static_cast<hipStream_t>(stream));
+  CXH_CATCH
+}
+
+int cxh_map_rev_ckpt(cxh_tape t, int64_t n, const double* const* arg, const double* const* adj_out,
+                     double* const* adj_in, int32_t layout, int64_t ld, int64_t checkpoint_every,
+                     const double* ckpt, void* stream) {
+  CXH_TRY
+  if (!ckpt) return cxh_fail("null checkpoint buffer");
+  return launch_adj(t, 2, n, arg, adj_out, nullptr, adj_in, layout, ld, checkpoint_every,
+                    const_cast<double*>(ckpt), static_cast<hipStream_t>(stream));
+  CXH_CATCH
+}
+
+int cxh_jac_create(cxh_tape t, int32_t forward, int64_t ndir, const int64_t* seed_ptr, const int64_t* seed_idx,
+                   int64_t nnz_jac, const int64_t* src_dir, const int64_t* src_nz, cxh_jac* out) {
+  CXH_TRY
+  if (!t || !out) return cxh_fail("null argument");
+  if (!forward) return cxh_fail("cxh_jac_create: only forward (D1) plans run on the tangent kernel");
+  if (ndir < 0 || ndir > 4096) return cxh_fail("cxh_jac_create: bad number of directions");
+  auto j = std::make_unique<cxh_jac_s>();
+  j->tape = t;
+  j->ndir = static_cast<int>(ndir);
+  j->nnz_jac = nnz_jac;
+  int64_t nin = 0, nout = 0;
+  for (int64_t v : t->nnz_in) {
+    j->in_off.push_back(static_cast<int>(nin));
+    nin += v;
+  }
+  for (int64_t v : t->nnz_out) {
+    j->out_off.push_back(static_cast<int>(nout));
+    nout += v;
+  }
+  std::vector<double> seedval(static_cast<size_t>(std::max<int64_t>(nin, 1)) * std::max<int64_t>(ndir, 1), 0.0);
+  for (int64_t d = 0; d < ndir; ++d)
+    for (int64_t q = seed_ptr[d]; q < seed_ptr[d + 1]; ++q) {
+      int64_t c = seed_idx[q];
+      if (c < 0 || c >= nin) throw std::runtime_error("cxh_jac_create: seed index out of range");
+      seedval[c * ndir + d] = 1.0;
+    }
+  std::vector<int2> scatter(static_cast<size_t>(std::max<int64_t>(nout, 1)) * std::max<int64_t>(ndir, 1),
+                            int2{-1, -1});
+  for (int64_t k = 0; k < nnz_jac; ++k) {
+    if (src_dir[k] < 0) continue;
+    if (src_dir[k] >= ndir || src_nz[k] < 0 || src_nz[k] >= nout)
+      throw std::runtime_error("cxh_jac_create: scatter source out of range");
+    int2& tg = scatter[src_nz[k] * ndir + src_dir[k]];
+    if (tg.x < 0) tg.x = static_cast<int>(k);
+    else if (tg.y < 0) tg.y = static_cast<int>(k);
+    else throw std::runtime_error("cxh_jac_create: more than two targets per sensitivity");
+  }
+  j->d_seed.upload(seedval.data(), seedval.size() * sizeof(double));
+  j->d_scatter.upload(scatter.data(), scatter.size() * sizeof(int2));
+  j->d_in_off.upload(j->in_off.data(), j->in_off.size() * sizeof(int));
+  j->d_out_off.upload(j->out_off.data(), j->out_off.size() * sizeof(int));
+  *out = j.release();
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_jac_destroy(cxh_jac j) {
+  delete j;
+  return 0;
+}
+
+int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* res, double* jac,
+                int32_t layout, int64_t ld, void* stream) {
+  CXH_TRY
+  if (!j) return cxh_fail("null jac plan");
+  cxh_tape t = j->tape;
+  if (n < 0) return cxh_fail("negative n");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  if (n == 0 || t->alg.empty()) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  JacArgs A{};
+  A.tape = t->d_tape.as<Instr>();
+  A.n_instr = static_cast<int>(t->alg.size());
+  A.ws = static_cast<int>(t->ws);
+  A.ndir = j->ndir;
+  A.arg = make_io(arg, t->nnz_in, layout, ld);
+  A.res = make_io(res, t->nnz_out, layout, ld);
+  A.in_off = j->d_in_off.as<int>();
+  A.out_off = j->d_out_off.as<int>();
+  A.seedval = j->d_seed.as<double>();
+  A.scatter = j->d_scatter.as<int2>();
+  A.jac = jac;
+  if (layout == CXH_SOA) {
+    A.jac_s_nz = ld;
+    A.jac_s_inst = 1;
+  } else {
+    A.jac_s_nz = 1;
+    A.jac_s_inst = j->nnz_jac;
+  }
+  A.n = n;
+  const int bs = t->block;
+  const int64_t grid = (n + bs - 1) / bs;
+  A.lanes = grid * bs;
+  const size_t lds = static_cast<size_t>(1 + A.ndir) * A.ws * bs * sizeof(double);
+  const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
+  if (use_lds) {
+    allow_lds(reinterpret_cast<const void*>(tape_jac_kernel<true>), lds);
+    hipLaunchKernelGGL(tape_jac_kernel<true>, dim3(grid), dim3(bs), lds, s, A);
+  } else {
+    t->scratch.reserve(static_cast<size_t>(1 + A.ndir) * A.ws * A.lanes * sizeof(double));
+    A.wglob = t->scratch.as<double>();
+    hipLaunchKernelGGL(tape_jac_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
+  }
+  hip_check(hipGetLastError(), "tape_jac_kernel launch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_device_info(int32_t* n_cu, int32_t* lds_per_cu_kb, char* name, int32_t name_len) {
+  CXH_TRY
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t p;
+  hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+  if (n_cu) *n_cu = p.multiProcessorCount;
+  if (lds_per_cu_kb) *lds_per_cu_kb = static_cast<int32_t>(p.sharedMemPerMultiprocessor / 1024);
+  if (name && name_len > 0) {
+    std::strncpy(name, p.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return 0;
+  CXH_CATCH
+}
+
+}  // extern "C"

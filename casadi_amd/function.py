@@ -1,0 +1,299 @@
+"""Function and Map: CasADi's callable-graph API, Python side.
+
+``Function(name, [x, u], [expr], ...)`` compiles the SX graph into a tape
+(SXFunction::init, casadi/core/sx_function.cpp:476-841, restated natively in
+casadi_amd/csrc/sx_function.cpp).  ``f.map(n, parallelization)`` batches it
+(Map::create, casadi/core/map.cpp:39-51) with ``"serial"``, ``"openmp"`` or
+the new ``"hip"`` parallelization, which runs on the GPU through the C ABI.
+"""
+import ctypes as C
+import numbers
+
+import numpy as np
+
+from . import _lib as L
+from .sx import SX, Sparsity
+
+INSTR_DTYPE = np.dtype([("op", "<i4"), ("i0", "<i4"), ("i1", "<i4"), ("i2", "<i4")])
+
+PARALLELIZATIONS = ("serial", "openmp", "hip", "unroll", "inline", "thread")
+
+
+class Function:
+    """An SX Function (casadi::Function backed by an SXFunction)."""
+
+    def __init__(self, name, ex_in=None, ex_out=None, name_in=None, name_out=None, opts=None,
+                 _handle=None):
+        if _handle is not None:
+            self._h = _handle
+        else:
+            opts = dict(opts or {})
+            ex_in = [x if isinstance(x, SX) else SX.from_value(x) for x in ex_in]
+            ex_out = [x if isinstance(x, SX) else SX.from_value(x) for x in ex_out]
+            sps_in = [x.sparsity.compressed() for x in ex_in]
+            sps_out = [x.sparsity.compressed() for x in ex_out]
+            nzs_in = [x.nz for x in ex_in]
+            nzs_out = [x.nz for x in ex_out]
+            keep = [sps_in, sps_out, nzs_in, nzs_out]
+            p_sp_in = L.ptr_array([a.ctypes.data for a in sps_in])
+            p_sp_out = L.ptr_array([a.ctypes.data for a in sps_out])
+            p_nz_in = L.ptr_array([a.ctypes.data if a.size else None for a in nzs_in])
+            p_nz_out = L.ptr_array([a.ctypes.data if a.size else None for a in nzs_out])
+            names_in = [n.encode() for n in (name_in or [f"i{i}" for i in range(len(ex_in))])]
+            names_out = [n.encode() for n in (name_out or [f"o{i}" for i in range(len(ex_out))])]
+            c_in = (C.c_char_p * max(len(names_in), 1))(*names_in)
+            c_out = (C.c_char_p * max(len(names_out), 1))(*names_out)
+            h = L.P()
+            L.check(L.cxs_function_new(name.encode(), len(ex_in), p_sp_in, p_nz_in,
+                                       C.cast(c_in, L.PP), len(ex_out), p_sp_out, p_nz_out,
+                                       C.cast(c_out, L.PP), int(bool(opts.get("allow_free", False))),
+                                       C.byref(h)), f"Function {name}")
+            del keep
+            self._h = h
+        self._cache = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L is not None and L.cxs_function_free is not None:
+            L.cxs_function_free(h)
+            self._h = None
+
+    # -- introspection -------------------------------------------------------
+    def _info(self):
+        if "info" not in self._cache:
+            ni, no, nin, ws, nf = L.I32(), L.I32(), L.I64(), L.I64(), L.I64()
+            L.check(L.cxs_function_info(self._h, C.byref(ni), C.byref(no), C.byref(nin), C.byref(ws),
+                                        C.byref(nf)))
+            self._cache["info"] = (ni.value, no.value, nin.value, ws.value, nf.value)
+        return self._cache["info"]
+
+    def name(self):
+        return L.cxs_function_name(self._h).decode()
+
+    def n_in(self):
+        return self._info()[0]
+
+    def n_out(self):
+        return self._info()[1]
+
+    def n_instructions(self):
+        return self._info()[2]
+
+    def sz_w(self):
+        return self._info()[3]
+
+    def has_free(self):
+        return self._info()[4] > 0
+
+    def name_in(self, i=None):
+        if i is None:
+            return [self.name_in(k) for k in range(self.n_in())]
+        return L.cxs_function_name_in(self._h, i).decode()
+
+    def name_out(self, i=None):
+        if i is None:
+            return [self.name_out(k) for k in range(self.n_out())]
+        return L.cxs_function_name_out(self._h, i).decode()
+
+    def index_in(self, name):
+        return self.name_in().index(name)
+
+    def index_out(self, name):
+        return self.name_out().index(name)
+
+    def sparsity_in(self, i):
+        i = self.index_in(i) if isinstance(i, str) else i
+        key = ("spi", i)
+        if key not in self._cache:
+            h = L.P()
+            L.check(L.cxs_function_sparsity_in(self._h, i, C.byref(h)))
+            self._cache[key] = Sparsity.from_compressed(L.take_buf(h))
+        return self._cache[key]
+
+    def sparsity_out(self, i):
+        i = self.index_out(i) if isinstance(i, str) else i
+        key = ("spo", i)
+        if key not in self._cache:
+            h = L.P()
+            L.check(L.cxs_function_sparsity_out(self._h, i, C.byref(h)))
+            self._cache[key] = Sparsity.from_compressed(L.take_buf(h))
+        return self._cache[key]
+
+    def nnz_in(self, i=None):
+        if i is None:
+            return sum(self.nnz_in(k) for k in range(self.n_in()))
+        return self.sparsity_in(i).nnz()
+
+    def nnz_out(self, i=None):
+        if i is None:
+            return sum(self.nnz_out(k) for k in range(self.n_out()))
+        return self.sparsity_out(i).nnz()
+
+    def size_in(self, i):
+        return self.sparsity_in(i).shape
+
+    def size_out(self, i):
+        return self.sparsity_out(i).shape
+
+    def sx_out(self, i):
+        h = L.P()
+        L.check(L.cxs_function_nz_out(self._h, i, C.byref(h)))
+        return SX(self.sparsity_out(i), L.take_buf(h, "uint64"))
+
+    def sx_in(self, i):
+        h = L.P()
+        L.check(L.cxs_function_nz_in(self._h, i, C.byref(h)))
+        return SX(self.sparsity_in(i), L.take_buf(h, "uint64"))
+
+    def instructions(self):
+        """The compiled tape as a structured array (op, i0, i1, i2); constants of
+        OP_CONST records are the float64 overlaid on (i1, i2)."""
+        if "tape" not in self._cache:
+            a = np.zeros(self.n_instructions(), INSTR_DTYPE)
+            if a.size:
+                L.check(L.cxs_function_tape(self._h, a.ctypes.data))
+            self._cache["tape"] = a
+        return self._cache["tape"]
+
+    def instruction_constants(self):
+        a = self.instructions()
+        return a.view(np.uint8).reshape(-1, 16)[:, 8:].copy().view("<f8").reshape(-1)
+
+    # -- evaluation -----------------------------------------------------------
+    def _numeric_args(self, args):
+        out = []
+        for i, a in enumerate(args):
+            sp = self.sparsity_in(i)
+            if a is None:
+                out.append(None)
+                continue
+            v = np.asarray(a, dtype=float)
+            if v.ndim == 0 or v.size == 1:
+                dense = np.full(sp.shape, float(v.reshape(-1)[0]) if v.size else 0.0)
+            else:
+                if v.ndim == 1:
+                    v = v.reshape(-1, 1) if sp.shape[1] == 1 else v.reshape(1, -1)
+                if v.shape != sp.shape:
+                    raise ValueError(f"{self.name()}: input {i} has shape {v.shape}, expected {sp.shape}")
+                dense = v
+            out.append(np.ascontiguousarray(dense[sp.row, sp.get_col()]))
+        return out
+
+    def _dense_out(self, i, nz):
+        sp = self.sparsity_out(i)
+        d = np.zeros(sp.shape)
+        d[sp.row, sp.get_col()] = nz
+        return d
+
+    def __call__(self, *args, **kwargs):
+        if kwargs:
+            if args:
+                raise ValueError("mix of positional and keyword arguments")
+            res = self.call({k: v for k, v in kwargs.items()})
+            return res
+        res = self.call(list(args))
+        return res[0] if len(res) == 1 else tuple(res)
+
+    def call(self, args):
+        named = isinstance(args, dict)
+        if named:
+            lst = [None] * self.n_in()
+            for k, v in args.items():
+                lst[self.index_in(k)] = v
+            args = lst
+        if len(args) != self.n_in():
+            raise ValueError(f"{self.name()}: expected {self.n_in()} inputs, got {len(args)}")
+        symbolic = any(isinstance(a, SX) for a in args)
+        res = self._call_sx(args) if symbolic else self._call_num(args)
+        if named:
+            return {self.name_out(i): r for i, r in enumerate(res)}
+        return res
+
+    def _call_num(self, args):
+        arg = self._numeric_args(args)
+        res = [np.zeros(self.nnz_out(i)) for i in range(self.n_out())]
+        pa = L.ptr_array([a.ctypes.data if a is not None and a.size else None for a in arg])
+        pr = L.ptr_array([r.ctypes.data if r.size else None for r in res])
+        L.check(L.cxs_function_eval(self._h, pa, pr), self.name())
+        return [self._dense_out(i, r) for i, r in enumerate(res)]
+
+    def _call_sx(self, args):
+        arrs = []
+        for i, a in enumerate(args):
+            sp = self.sparsity_in(i)
+            if a is None:
+                x = SX.zeros(sp)
+            else:
+                x = a if isinstance(a, SX) else SX.from_value(a)
+                if x.is_scalar() and sp.shape != (1, 1):
+                    x = SX.from_dense_nodes(np.full(sp.shape, int(x.nz[0]), np.uint64))
+                if x.shape != sp.shape:
+                    if x.numel() == sp.numel():
+                        x = SX.from_dense_nodes(x.dense_nodes().T.reshape(sp.shape[::-1]).T)
+                    else:
+                        raise ValueError(f"{self.name()}: input {i} has shape {x.shape}, expected {sp.shape}")
+                d = x.dense_nodes()
+                x = SX(sp, d[sp.row, sp.get_col()])
+            arrs.append(np.ascontiguousarray(x.nz))
+        res = [np.zeros(self.nnz_out(i), np.uint64) for i in range(self.n_out())]
+        pa = L.ptr_array([a.ctypes.data if a.size else None for a in arrs])
+        pr = L.ptr_array([r.ctypes.data if r.size else None for r in res])
+        L.check(L.cxs_function_call(self._h, pa, pr), self.name())
+        return [SX(self.sparsity_out(i), r) for i, r in enumerate(res)]
+
+    # -- derivatives (casadi/core/function_internal.cpp:2220-2440) -------------
+    def _derived(self, fn, *extra):
+        h = L.P()
+        L.check(fn(self._h, *extra, C.byref(h)), self.name())
+        return Function(None, _handle=h)
+
+    def jacobian(self):
+        if "jac" not in self._cache:
+            self._cache["jac"] = self._derived(L.cxs_function_jacobian)
+        return self._cache["jac"]
+
+    def forward(self, nfwd):
+        key = ("fwd", nfwd)
+        if key not in self._cache:
+            self._cache[key] = self._derived(L.cxs_function_forward, nfwd)
+        return self._cache[key]
+
+    def reverse(self, nadj):
+        key = ("adj", nadj)
+        if key not in self._cache:
+            self._cache[key] = self._derived(L.cxs_function_reverse, nadj)
+        return self._cache[key]
+
+    def jac_sparsity(self, oind=0, iind=0, compact=True, symmetric=False):
+        h = L.P()
+        L.check(L.cxs_function_jac_sparsity(self._h, oind, iind, int(compact), int(symmetric), C.byref(h)))
+        return Sparsity.from_compressed(L.take_buf(h))
+
+    def jac_plan(self, symmetric=False):
+        """Colouring + scatter plan of the compressed Jacobian of output 0 w.r.t.
+        input 0 (XFunction::jac bookkeeping): dict with jsp, forward, seeds
+        (CSR), src_dir, src_nz."""
+        key = ("plan", bool(symmetric))
+        if key not in self._cache:
+            fwd = L.I32()
+            hs = [L.P() for _ in range(5)]
+            L.check(L.cxs_function_jac_plan(self._h, int(symmetric), C.byref(fwd), *[C.byref(h) for h in hs]))
+            jsp, sptr, sidx, sdir, snz = [L.take_buf(h) for h in hs]
+            self._cache[key] = dict(jsp=Sparsity.from_compressed(jsp), forward=bool(fwd.value),
+                                    seed_ptr=sptr, seed_idx=sidx, src_dir=sdir, src_nz=snz,
+                                    ndir=len(sptr) - 1)
+        return self._cache[key]
+
+    # -- batching ---------------------------------------------------------------
+    def map(self, n, parallelization="serial", max_num_threads=None, opts=None):
+        """Map::create (casadi/core/map.cpp:39-51)."""
+        if parallelization not in PARALLELIZATIONS:
+            raise ValueError(f"Unknown parallelization '{parallelization}' for map: "
+                             f"choose one of {PARALLELIZATIONS}")
+        from .map import MapFunction
+        return MapFunction(self, int(n), parallelization, max_num_threads, opts or {})
+
+    def __repr__(self):
+        ins = ", ".join(f"{self.name_in(i)}[{self.size_in(i)[0]}x{self.size_in(i)[1]}]" for i in range(self.n_in()))
+        outs = ", ".join(f"{self.name_out(i)}[{self.size_out(i)[0]}x{self.size_out(i)[1]}]" for i in range(self.n_out()))
+        return f"{self.name()}:({ins})->({outs}) SXFunction"

@@ -1,0 +1,197 @@
+"""GPU execution of SX tapes through the C ABI (``cxh_*``).
+
+Per-instance data is structure-of-arrays: input ``i`` of a batch of ``n``
+instances is a float64 CUDA tensor of shape ``(nnz_in(i), n)`` (nonzero-major,
+row stride ``ld``), which is exactly CasADi's map argument ``repmat(x, 1, n)``
+stored row-major.  PyTorch provides device memory and streams only; every
+evaluation runs the hand-written kernels in ``libcasadi_hip.so``.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+CXH_SOA, CXH_AOS = 0, 1
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _ptrs(tensors):
+    return L.ptr_array([t.data_ptr() if t is not None else None for t in tensors])
+
+
+def _check_soa(tensors, nnz, n, what):
+    torch = _torch()
+    ld = None
+    for i, t in enumerate(tensors):
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+            raise TypeError(f"{what}[{i}] must be a float64 CUDA tensor")
+        if t.dim() != 2 or t.shape[0] != nnz[i] or t.shape[1] < n:
+            raise ValueError(f"{what}[{i}] has shape {tuple(t.shape)}, expected ({nnz[i]}, {n})")
+        if nnz[i] > 1 and t.stride(1) != 1:
+            raise ValueError(f"{what}[{i}] must have unit stride along instances")
+        s = t.stride(0) if nnz[i] > 1 else n
+        if ld is None:
+            ld = s
+        elif nnz[i] > 1 and s != ld:
+            raise ValueError(f"{what}: all blocks must share the leading dimension")
+    return ld if ld is not None else n
+
+
+class HipTape:
+    """A Function's tape resident on the GPU (cxh_tape)."""
+
+    def __init__(self, f, block=64, w_in_lds=True):
+        self.f = f
+        self.nnz_in = [f.nnz_in(i) for i in range(f.n_in())]
+        self.nnz_out = [f.nnz_out(i) for i in range(f.n_out())]
+        instr = f.instructions()
+        ni = np.array(self.nnz_in, np.int64)
+        no = np.array(self.nnz_out, np.int64)
+        h = L.P()
+        L.check(L.cxh_tape_create(instr.ctypes.data if instr.size else None, instr.size, f.sz_w(),
+                                  len(ni), ni.ctypes.data_as(L.PI64), len(no), no.ctypes.data_as(L.PI64),
+                                  C.byref(h)), f"cxh_tape_create({f.name()})")
+        self._h = h
+        self.config(block, w_in_lds)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L.cxh_tape_destroy is not None:
+            L.cxh_tape_destroy(h)
+            self._h = None
+
+    def config(self, block=-1, w_in_lds=-1):
+        L.check(L.cxh_tape_config(self._h, int(block), int(w_in_lds) if w_in_lds != -1 else -1))
+        if block != -1:
+            self.block = int(block)
+
+    def _alloc(self, nnz, n, like):
+        torch = _torch()
+        return [torch.empty((k, n), dtype=torch.float64, device=like.device) if k else None for k in nnz]
+
+    def _like(self, args):
+        for a in args:
+            if a is not None:
+                return a
+        torch = _torch()
+        return torch.empty(0, dtype=torch.float64, device="cuda")
+
+    # -- forward ---------------------------------------------------------------
+    def eval(self, args, n=None, out=None, stream=None):
+        """Outputs of n instances (list of (nnz_out(i), n) tensors)."""
+        like = self._like(args)
+        n = n if n is not None else like.shape[1]
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        out = out if out is not None else self._alloc(self.nnz_out, n, like)
+        ld_o = _check_soa(out, self.nnz_out, n, "res")
+        if ld_o != ld and any(k > 1 for k in self.nnz_out):
+            raise ValueError("inputs and outputs must share the leading dimension")
+        L.check(L.cxh_map_eval(self._h, n, _ptrs(args), _ptrs(out), CXH_SOA, ld, _stream_ptr(stream)),
+                "cxh_map_eval")
+        return out
+
+    def eval_host(self, args_aos, n):
+        """Map layout on the host: instance k of input i at args_aos[i][k*nnz:(k+1)*nnz]."""
+        arg = [np.ascontiguousarray(a, dtype=float) if a is not None else None for a in args_aos]
+        res = [np.zeros(n * k) for k in self.nnz_out]
+        pa = L.ptr_array([a.ctypes.data if a is not None and a.size else None for a in arg])
+        pr = L.ptr_array([r.ctypes.data if r.size else None for r in res])
+        L.check(L.cxh_map_eval_host(self._h, n, pa, pr), "cxh_map_eval_host")
+        return res
+
+    # -- adjoint ---------------------------------------------------------------
+    def fwd_adj(self, args, seeds, n=None, checkpoint_every=0, res=True, stream=None):
+        """Primal outputs and adj_in = J^T seeds for every instance, one launch."""
+        like = self._like(args)
+        n = n if n is not None else like.shape[1]
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        _check_soa(seeds, self.nnz_out, n, "adj_out")
+        outs = self._alloc(self.nnz_out, n, like) if res else [None] * len(self.nnz_out)
+        adj = self._alloc(self.nnz_in, n, like)
+        L.check(L.cxh_map_fwd_adj(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adj), CXH_SOA, ld,
+                                  int(checkpoint_every), _stream_ptr(stream)), "cxh_map_fwd_adj")
+        return outs, adj
+
+    def ckpt_size(self, n, checkpoint_every):
+        v = L.I64()
+        L.check(L.cxh_ckpt_size(self._h, n, int(checkpoint_every), -1, C.byref(v)))
+        return v.value
+
+    def fwd_ckpt(self, args, n, checkpoint_every, stream=None):
+        torch = _torch()
+        like = self._like(args)
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        outs = self._alloc(self.nnz_out, n, like)
+        ckpt = torch.empty(max(self.ckpt_size(n, checkpoint_every), 1), dtype=torch.float64, device=like.device)
+        L.check(L.cxh_map_fwd_ckpt(self._h, n, _ptrs(args), _ptrs(outs), CXH_SOA, ld, int(checkpoint_every),
+                                   ckpt.data_ptr(), _stream_ptr(stream)), "cxh_map_fwd_ckpt")
+        return outs, ckpt
+
+    def rev_ckpt(self, args, seeds, n, checkpoint_every, ckpt, stream=None):
+        like = self._like(args)
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        _check_soa(seeds, self.nnz_out, n, "adj_out")
+        adj = self._alloc(self.nnz_in, n, like)
+        L.check(L.cxh_map_rev_ckpt(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(adj), CXH_SOA, ld,
+                                   int(checkpoint_every), ckpt.data_ptr(), _stream_ptr(stream)),
+                "cxh_map_rev_ckpt")
+        return adj
+
+
+class HipJacobian:
+    """Fused primal + coloured forward seed sweeps + scatter into the nonzeros of
+    the Jacobian of output 0 w.r.t. input 0 (XFunction::jac evaluated
+    numerically).  ``symmetric`` selects star colouring (Hessians)."""
+
+    def __init__(self, f, symmetric=False, tape=None):
+        self.f = f
+        self.tape = tape if tape is not None else HipTape(f)
+        self.plan = f.jac_plan(symmetric)
+        p = self.plan
+        if not p["forward"]:
+            raise NotImplementedError("adjoint-mode colourings run through HipTape.fwd_adj per direction")
+        self.jsp = p["jsp"]
+        self.nnz_jac = self.jsp.nnz()
+        h = L.P()
+        sp, si, sd, sn = (np.ascontiguousarray(p[k], np.int64) for k in ("seed_ptr", "seed_idx", "src_dir", "src_nz"))
+        L.check(L.cxh_jac_create(self.tape._h, 1, p["ndir"], sp.ctypes.data_as(L.PI64), si.ctypes.data_as(L.PI64),
+                                 self.nnz_jac, sd.ctypes.data_as(L.PI64), sn.ctypes.data_as(L.PI64), C.byref(h)),
+                "cxh_jac_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L.cxh_jac_destroy is not None:
+            L.cxh_jac_destroy(h)
+            self._h = None
+
+    def __call__(self, args, n=None, res=True, stream=None):
+        torch = _torch()
+        like = self.tape._like(args)
+        n = n if n is not None else like.shape[1]
+        ld = _check_soa(args, self.tape.nnz_in, n, "arg")
+        outs = self.tape._alloc(self.tape.nnz_out, n, like) if res else [None] * len(self.tape.nnz_out)
+        jac = torch.empty((max(self.nnz_jac, 1), ld), dtype=torch.float64, device=like.device)
+        L.check(L.cxh_map_jac(self._h, n, _ptrs(args), _ptrs(outs), jac.data_ptr(), CXH_SOA, ld,
+                              _stream_ptr(stream)), "cxh_map_jac")
+        return outs, jac[: self.nnz_jac, :n]
+
+
+def device_info():
+    ncu, lds = L.I32(), L.I32()
+    name = C.create_string_buffer(64)
+    L.check(L.cxh_device_info(C.byref(ncu), C.byref(lds), name, 64))
+    return dict(n_cu=ncu.value, lds_per_cu_kb=lds.value, arch=name.value.decode())

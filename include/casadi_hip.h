@@ -1,0 +1,265 @@
+/*
+ * casadi_hip.h -- C ABI of the MI355X (gfx950) batched evaluator for CasADi
+ * SX tapes and sparse LDL^T / QR solves.
+ *
+ * Conventions (in the style of casadi/casadi_c.h:109-131):
+ *  - every function returns int: 0 = success, negative = invalid argument or
+ *    HIP error (text from cxh_last_error()), positive = numerical failure
+ *    count (e.g. instances with a zero pivot);
+ *  - no C++ exception crosses this boundary;
+ *  - device pointers are caller-owned; handles own their device copies of the
+ *    tape / sparsity structure;
+ *  - a handle is not re-entrant: use one handle per host thread and stream.
+ *
+ * Two groups of entry points:
+ *  cxh_*  the GPU boundary: tapes, batched evaluation, derivative sweeps,
+ *         heterogeneous batches, batched sparse LDL^T / QR.
+ *  cxs_*  the host symbolic layer (SX graphs, Function construction, tape
+ *         compilation, derivative Functions, colourings, orderings) that
+ *         produces what the GPU boundary consumes when CasADi itself is not
+ *         present.
+ */
+#ifndef CASADI_HIP_H
+#define CASADI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message of the calling thread ("" if none). */
+const char* cxh_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* GPU boundary                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct cxh_tape_s* cxh_tape;
+typedef struct cxh_jac_s* cxh_jac;
+typedef struct cxh_batch_s* cxh_batch;
+typedef struct cxh_ldl_s* cxh_ldl;
+typedef struct cxh_qr_s* cxh_qr;
+
+/* Layout of per-instance data.
+ *  CXH_SOA: nonzero j of instance k at p[j*ld + k]   (ld >= n; coalesced)
+ *  CXH_AOS: nonzero j of instance k at p[k*nnz + j]  (CasADi Map layout,
+ *           casadi/core/map.cpp:141-157)                                    */
+#define CXH_SOA 0
+#define CXH_AOS 1
+
+/* Create a device tape from a CasADi SXFunction algorithm.
+ * Replaces: the numeric state SXFunction holds for eval --
+ *   algorithm_ (std::vector<ScalarAtomic>, casadi/core/sx_function.hpp:37-44,
+ *   245-261), worksize_ and the nnz_in / nnz_out of the Function.
+ * instr: n_instr records of 16 bytes laid out exactly as ScalarAtomic
+ *   {int op; int i0; union {double d; struct {int i1, i2;};}}, with
+ *   casadi::Operation codes (casadi/core/calculus.hpp:60-218).
+ * The tape is validated (slot and I/O indices in range, supported ops);
+ * OP_CALL and free parameters are rejected.                                 */
+int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize,
+                    int32_t n_in, const int64_t* nnz_in,
+                    int32_t n_out, const int64_t* nnz_out, cxh_tape* out);
+int cxh_tape_destroy(cxh_tape t);
+/* Kernel configuration: threads per block (64/128/256) and whether the work
+ * vector lives in LDS (1) or in HBM scratch (0); -1 keeps the current value. */
+int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds);
+
+/* Batched forward evaluation of one tape over n instances, device pointers.
+ * Replaces: Map::eval / Map::eval_gen (casadi/core/map.cpp:141-157, 327-334)
+ *   looping SXFunction::eval (casadi/core/sx_function.cpp:72-127).
+ * arg[i] / res[i]: per-input / per-output device arrays (NULL arg = zeros,
+ *   NULL res = not requested, as in SXFunction::eval).
+ * ld: leading dimension for CXH_SOA (ignored for CXH_AOS).
+ * stream: hipStream_t (NULL = default stream). Asynchronous.               */
+int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const* res,
+                 int32_t layout, int64_t ld, void* stream);
+
+/* Same as cxh_map_eval on HOST arrays in CasADi's AoS Map layout
+ * (the drop-in for Map::eval's arg/res); synchronous.                       */
+int cxh_map_eval_host(cxh_tape t, int64_t n, const double* const* arg, double* const* res);
+
+/* Fused forward + adjoint sweep (one launch): primal outputs res and the
+ * gradient adj_in = (d out / d in)^T adj_out for every instance.
+ * Replaces: evaluating f and then its reverse Function adj1_<f>
+ *   (XFunction::get_reverse, casadi/core/x_function.hpp:892-935, built by
+ *   SXFunction::ad_reverse, casadi/core/sx_function.cpp:1368-1569).
+ * checkpoint_every: instructions per checkpointed segment of the reverse
+ *   sweep (0 = keep the whole trail).  res or adj_in entries may be NULL.   */
+int cxh_map_fwd_adj(cxh_tape t, int64_t n, const double* const* arg,
+                    const double* const* adj_out, double* const* res,
+                    double* const* adj_in, int32_t layout, int64_t ld,
+                    int64_t checkpoint_every, void* stream);
+
+/* Split adjoint for autograd: a forward pass that stores the work vector at
+ * the start of every segment into a caller-owned device buffer `ckpt`
+ * (cxh_ckpt_size doubles), then the reverse sweep restarting from those
+ * checkpoints.  Same results as cxh_map_fwd_adj with the same
+ * checkpoint_every.                                                         */
+int cxh_ckpt_size(cxh_tape t, int64_t n, int64_t checkpoint_every, int32_t block,
+                  int64_t* n_doubles);
+int cxh_map_fwd_ckpt(cxh_tape t, int64_t n, const double* const* arg, double* const* res,
+                     int32_t layout, int64_t ld, int64_t checkpoint_every, double* ckpt,
+                     void* stream);
+int cxh_map_rev_ckpt(cxh_tape t, int64_t n, const double* const* arg,
+                     const double* const* adj_out, double* const* adj_in,
+                     int32_t layout, int64_t ld, int64_t checkpoint_every,
+                     const double* ckpt, void* stream);
+
+/* Compressed-Jacobian sweep plan (colouring + scatter) for a tape with one
+ * output block: forward (1) or adjoint (0) directions, ndir seeds given as
+ * CSR (seed_ptr[ndir+1], seed_idx) over input (fwd) / output (adj) nonzeros
+ * of input/output block 0, and for each of nnz_jac Jacobian nonzeros the
+ * (direction, sensitivity nonzero) it is read from (-1 = structural zero).
+ * Replaces: the seed / scatter bookkeeping of XFunction::jac
+ *   (casadi/core/x_function.hpp:567-826) with colourings from
+ *   FunctionInternal::get_partition (casadi/core/function_internal.cpp:2068-2165). */
+int cxh_jac_create(cxh_tape t, int32_t forward, int64_t ndir, const int64_t* seed_ptr,
+                   const int64_t* seed_idx, int64_t nnz_jac, const int64_t* src_dir,
+                   const int64_t* src_nz, cxh_jac* out);
+int cxh_jac_destroy(cxh_jac j);
+/* Fused primal + all seed directions + scatter into the Jacobian nonzeros
+ * jac (device, nnz_jac per instance) for n instances; res may be NULL.      */
+int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* res,
+                double* jac, int32_t layout, int64_t ld, void* stream);
+
+/* Heterogeneous batch: instances of several tapes in one persistent launch,
+ * captured once into a hipGraph and replayed.  For tape f: counts[f]
+ * instances, SoA device arrays arg[f][i], adj_out[f][o] (may be NULL for a
+ * forward-only batch), res[f][o], adj_in[f][i].
+ * Replaces: one Map::eval per Function (casadi/core/map.cpp:141-157).       */
+int cxh_batch_create(int32_t n_fun, const cxh_tape* tapes, const int64_t* counts,
+                     const double* const* const* arg, const double* const* const* adj_out,
+                     double* const* const* res, double* const* const* adj_in,
+                     int32_t with_adjoint, void* stream, cxh_batch* out);
+/* Per-tape lanes-per-instance / tile choices, -1 = autotune.               */
+int cxh_batch_launch(cxh_batch b, void* stream);
+int cxh_batch_destroy(cxh_batch b);
+/* Launch every tape of the batch separately (one launch per Function). */
+int cxh_batch_launch_plain(cxh_batch b, void* stream);
+
+/* Batched sparse LDL^T on CasADi's symbolic factorisation.
+ * sp_a, sp_lt: CasADi compact sparsity [nrow, ncol, colind..., row...] of A
+ *   and of L^T (strictly upper), p: AMD permutation (Sparsity::ldl,
+ *   casadi/core/sparsity.cpp:622-652).
+ * Replaces: LinsolLdl::nfact / solve (casadi/solvers/linsol_ldl.cpp:119-132)
+ *   = casadi_ldl + casadi_ldl_solve (casadi/core/runtime/casadi_ldl.hpp). */
+int cxh_ldl_create(const int64_t* sp_a, const int64_t* sp_lt, const int64_t* p, cxh_ldl* out);
+int cxh_ldl_destroy(cxh_ldl l);
+/* a: nnz(A) values per instance, x: n*nrhs rhs values per instance,
+ * overwritten by the solution; both in `layout` (SOA: [nz][n_inst]).
+ * status (device, may be NULL): per instance, number of zero pivots.
+ * Returns 0; use cxh_ldl_count_failures for the host-side count.           */
+int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, int32_t nrhs,
+                         int32_t* status, int32_t layout, void* stream);
+
+/* Batched sparse Householder QR (casadi_qr / casadi_qr_solve,
+ * casadi/core/runtime/casadi_qr.hpp:53-200, LinsolQr
+ * casadi/solvers/linsol_qr.cpp:126-178) on Sparsity::qr_sparse output.
+ * status: per instance, number of |R_ii| < eps (casadi_qr_singular).        */
+int cxh_qr_create(const int64_t* sp_a, const int64_t* sp_v, const int64_t* sp_r,
+                  const int64_t* prinv, const int64_t* pc, double eps, cxh_qr* out);
+int cxh_qr_destroy(cxh_qr q);
+int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, int32_t nrhs,
+                        int32_t tr, int32_t* status, int32_t layout, void* stream);
+
+/* Launch duration in ms of the last kernel issued through a handle-less
+ * timing probe: event-timed average of `reps` launches of cxh_map_eval.    */
+int cxh_device_info(int32_t* n_cu, int32_t* lds_per_cu_kb, char* name, int32_t name_len);
+
+/* ------------------------------------------------------------------------ */
+/* Host symbolic layer                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef uint64_t cxs_sx;                 /* scalar expression node          */
+typedef struct cxs_fun_s* cxs_fun;       /* SX Function                     */
+typedef struct cxs_buf_s* cxs_buf;       /* result buffer (int64 or nodes)  */
+
+/* SXElem construction (casadi/core/sx_elem.cpp:71-263) */
+cxs_sx cxs_sym(const char* name);
+cxs_sx cxs_const(double v);
+cxs_sx cxs_unary(int32_t op, cxs_sx x);
+cxs_sx cxs_binary(int32_t op, cxs_sx x, cxs_sx y);
+cxs_sx cxs_neg(cxs_sx x);
+cxs_sx cxs_if_else(cxs_sx c, cxs_sx a, cxs_sx b);
+/* elementwise over arrays; incx/incy = 0 broadcasts a scalar               */
+int cxs_unary_n(int32_t op, int64_t n, const cxs_sx* x, cxs_sx* out);
+int cxs_binary_n(int32_t op, int64_t n, const cxs_sx* x, int64_t incx, const cxs_sx* y,
+                 int64_t incy, cxs_sx* out);
+int cxs_node_info(cxs_sx x, int32_t* op, double* value, cxs_sx* dep0, cxs_sx* dep1);
+const char* cxs_sym_name(cxs_sx x);
+int cxs_is_equal(cxs_sx x, cxs_sx y, int32_t depth);
+int64_t cxs_node_count(void);
+
+/* Function(name, ex_in, ex_out) (casadi/core/function.cpp:96-160 ->
+ * SXFunction::init, casadi/core/sx_function.cpp:476-841).
+ * sp_in[i]: compact sparsity of input i, nz_in[i]: its nonzero nodes.      */
+int cxs_function_new(const char* name, int32_t n_in, const int64_t* const* sp_in,
+                     const cxs_sx* const* nz_in, const char* const* names_in,
+                     int32_t n_out, const int64_t* const* sp_out, const cxs_sx* const* nz_out,
+                     const char* const* names_out, int32_t allow_free, cxs_fun* out);
+int cxs_function_free(cxs_fun f);
+int cxs_function_info(cxs_fun f, int32_t* n_in, int32_t* n_out, int64_t* n_instr,
+                      int64_t* worksize, int64_t* n_free);
+const char* cxs_function_name(cxs_fun f);
+const char* cxs_function_name_in(cxs_fun f, int32_t i);
+const char* cxs_function_name_out(cxs_fun f, int32_t i);
+/* sparsity (compact form) of input / output i, as a buffer                 */
+int cxs_function_sparsity_in(cxs_fun f, int32_t i, cxs_buf* out);
+int cxs_function_sparsity_out(cxs_fun f, int32_t i, cxs_buf* out);
+/* expressions of output i / input i (nodes)                                 */
+int cxs_function_nz_out(cxs_fun f, int32_t i, cxs_buf* out);
+int cxs_function_nz_in(cxs_fun f, int32_t i, cxs_buf* out);
+/* the compiled tape as 16-byte ScalarAtomic records                         */
+int cxs_function_tape(cxs_fun f, void* instr_out);
+/* numeric evaluation on the host VM, one instance                           */
+int cxs_function_eval(cxs_fun f, const double* const* arg, double* const* res);
+/* Map serial / openmp on host AoS arrays (casadi/core/map.cpp:141-157, 340-386) */
+int cxs_function_map_eval(cxs_fun f, int64_t n, const double* const* arg, double* const* res,
+                          int32_t n_threads);
+/* symbolic call: arg_nz[i] nodes for input i (pattern of input i)          */
+int cxs_function_call(cxs_fun f, const cxs_sx* const* arg_nz, cxs_sx* const* res_nz);
+/* derivative Functions (casadi/core/function_internal.cpp:2220-2440)       */
+int cxs_function_jacobian(cxs_fun f, cxs_fun* out);
+int cxs_function_forward(cxs_fun f, int64_t nfwd, cxs_fun* out);
+int cxs_function_reverse(cxs_fun f, int64_t nadj, cxs_fun* out);
+/* Jacobian / Hessian plan for the numeric sweeps: jsp (compact pattern),
+ * direction seeds CSR and scatter sources; symmetric selects star colouring. */
+int cxs_function_jac_plan(cxs_fun f, int32_t symmetric, int32_t* forward, cxs_buf* jsp,
+                          cxs_buf* seed_ptr, cxs_buf* seed_idx, cxs_buf* src_dir,
+                          cxs_buf* src_nz);
+/* Structural Jacobian pattern of output oind w.r.t. input iind             */
+int cxs_function_jac_sparsity(cxs_fun f, int32_t oind, int32_t iind, int32_t compact,
+                              int32_t symmetric, cxs_buf* out);
+
+/* SX-level calculus (casadi/core/sx_instantiator.cpp:1155-1177). f and x
+ * are (compact sparsity, nodes) pairs; the result is a sparsity buffer plus
+ * a node buffer.                                                            */
+int cxs_jacobian(const int64_t* sp_f, const cxs_sx* nz_f, const int64_t* sp_x,
+                 const cxs_sx* nz_x, int32_t symmetric, cxs_buf* sp_out, cxs_buf* nz_out);
+int cxs_gradient(const int64_t* sp_f, const cxs_sx* nz_f, const int64_t* sp_x,
+                 const cxs_sx* nz_x, cxs_buf* sp_out, cxs_buf* nz_out);
+int cxs_hessian(const int64_t* sp_f, const cxs_sx* nz_f, const int64_t* sp_x,
+                const cxs_sx* nz_x, cxs_buf* sp_out, cxs_buf* nz_out);
+
+/* Sparsity algorithms on compact patterns (casadi/core/sparsity_internal.cpp) */
+int cxs_sp_uni_coloring(const int64_t* sp, int64_t cutoff, cxs_buf* out);
+int cxs_sp_star_coloring(const int64_t* sp, int64_t ordering, cxs_buf* out);
+int cxs_sp_largest_first(const int64_t* sp, cxs_buf* out);
+int cxs_sp_amd(const int64_t* sp, cxs_buf* out);
+int cxs_sp_etree(const int64_t* sp, int32_t ata, cxs_buf* out);
+int cxs_sp_ldl(const int64_t* sp, int32_t amd, cxs_buf* sp_lt, cxs_buf* p);
+int cxs_sp_qr(const int64_t* sp, int32_t amd, cxs_buf* sp_v, cxs_buf* sp_r, cxs_buf* prinv,
+              cxs_buf* pc);
+int cxs_sp_mtimes(const int64_t* sp_a, const int64_t* sp_b, cxs_buf* out);
+
+/* result buffers: element count (int64 or node entries) and copy-out        */
+int64_t cxs_buf_size(cxs_buf b);
+int cxs_buf_copy(cxs_buf b, void* dst);
+int cxs_buf_free(cxs_buf b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CASADI_HIP_H */

@@ -1,0 +1,415 @@
+/*
+ * ORACLE -- test infrastructure only.  Nothing in casadi_amd/ links, imports
+ * or calls this file; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it, as the checker / CPU baseline.
+ *
+ * Plain-C restatement of CasADi's CPU path for batched SX evaluation and the
+ * sparse LDL^T / QR runtime:
+ *   or_fun / or_der     casadi_math<double>::fun / der
+ *                       (casadi/core/calculus.hpp:598-1018, 1302-1439)
+ *   or_eval             SXFunction::eval (casadi/core/sx_function.cpp:72-127)
+ *   or_map_eval         Map::eval_gen serial loop (casadi/core/map.cpp:141-157)
+ *                       and OmpMap::eval (casadi/core/map.cpp:340-386)
+ *   or_ldl / or_ldl_solve  casadi_ldl / casadi_ldl_trs / casadi_ldl_solve
+ *                       (casadi/core/runtime/casadi_ldl.hpp:24-109)
+ *   or_qr / or_qr_solve / or_qr_singular
+ *                       casadi_house / casadi_qr / casadi_qr_mv / casadi_qr_trs /
+ *                       casadi_qr_solve / casadi_qr_singular
+ *                       (casadi/core/runtime/casadi_qr.hpp:24-230)
+ * How each function is pinned is listed in tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+  int32_t op, i0;
+  union {
+    double d;
+    struct {
+      int32_t i1, i2;
+    } s;
+  } u;
+} or_instr;
+
+enum {
+  O_ASSIGN = 0, O_ADD, O_SUB, O_MUL, O_DIV, O_NEG, O_EXP, O_LOG, O_POW, O_CONSTPOW, O_SQRT, O_SQ,
+  O_TWICE, O_SIN, O_COS, O_TAN, O_ASIN, O_ACOS, O_ATAN, O_LT, O_LE, O_EQ, O_NE, O_NOT, O_AND, O_OR,
+  O_FLOOR, O_CEIL, O_FMOD, O_FABS, O_SIGN, O_COPYSIGN, O_IF_ELSE_ZERO, O_ERF, O_FMIN, O_FMAX, O_INV,
+  O_SINH, O_COSH, O_TANH, O_ASINH, O_ACOSH, O_ATANH, O_ATAN2, O_CONST, O_INPUT, O_OUTPUT,
+  O_PARAMETER, O_CALL, O_ERFINV = 86, O_PRINTME = 87, O_LIFT = 88, O_LOG1P = 93, O_EXPM1 = 94,
+  O_HYPOT = 95, O_REMAINDER = 97
+};
+
+static double or_sign(double x) { return x < 0 ? -1 : x > 0 ? 1 : x; }
+
+/* calculus.hpp:300-327 (used when the C library has no erfinv) */
+static double or_erfinv(double x) {
+  const double pi = 3.14159265358979323846;
+  double y, z;
+  if (x >= 1) return x == 1 ? INFINITY : NAN;
+  if (x <= -1) return x == -1 ? -INFINITY : NAN;
+  if (x < -0.7) {
+    z = sqrt(-log((1.0 + x) / 2.0));
+    return -(((1.641345311 * z + 3.429567803) * z - 1.624906493) * z - 1.970840454) /
+           ((1.637067800 * z + 3.543889200) * z + 1.0);
+  }
+  if (x < 0.7) {
+    z = x * x;
+    y = x * (((-0.140543331 * z + 0.914624893) * z - 1.645349621) * z + 0.886226899) /
+        ((((-0.329097515 * z + 0.012229801) * z + 1.442710462) * z - 2.118377725) * z + 1.0);
+  } else {
+    z = sqrt(-log((1.0 - x) / 2.0));
+    y = (((1.641345311 * z + 3.429567803) * z - 1.624906493) * z - 1.970840454) /
+        ((1.637067800 * z + 3.543889200) * z + 1.0);
+  }
+  y = y - (erf(y) - x) / (2.0 / sqrt(pi) * exp(-y * y));
+  y = y - (erf(y) - x) / (2.0 / sqrt(pi) * exp(-y * y));
+  return y;
+}
+
+double or_fun(int op, double x, double y) {
+  switch (op) {
+    case O_ASSIGN: return x;
+    case O_ADD: return x + y;
+    case O_SUB: return x - y;
+    case O_MUL: return x * y;
+    case O_DIV: return x / y;
+    case O_NEG: return -x;
+    case O_EXP: return exp(x);
+    case O_LOG: return log(x);
+    case O_POW: case O_CONSTPOW: return pow(x, y);
+    case O_SQRT: return sqrt(x);
+    case O_SQ: return x * x;
+    case O_TWICE: return 2. * x;
+    case O_SIN: return sin(x);
+    case O_COS: return cos(x);
+    case O_TAN: return tan(x);
+    case O_ASIN: return asin(x);
+    case O_ACOS: return acos(x);
+    case O_ATAN: return atan(x);
+    case O_LT: return x < y;
+    case O_LE: return x <= y;
+    case O_EQ: return x == y;
+    case O_NE: return x != y;
+    case O_NOT: return !x;
+    case O_AND: return x && y;
+    case O_OR: return x || y;
+    case O_FLOOR: return floor(x);
+    case O_CEIL: return ceil(x);
+    case O_FMOD: return fmod(x, y);
+    case O_REMAINDER: return remainder(x, y);
+    case O_FABS: return fabs(x);
+    case O_SIGN: return or_sign(x);
+    case O_COPYSIGN: return copysign(x, y);
+    case O_IF_ELSE_ZERO: return x == 0 ? 0 : y;
+    case O_ERF: return erf(x);
+    case O_FMIN: return fmin(x, y);
+    case O_FMAX: return fmax(x, y);
+    case O_INV: return 1. / x;
+    case O_SINH: return sinh(x);
+    case O_COSH: return cosh(x);
+    case O_TANH: return tanh(x);
+    case O_ASINH: return asinh(x);
+    case O_ACOSH: return acosh(x);
+    case O_ATANH: return atanh(x);
+    case O_ATAN2: return atan2(x, y);
+    case O_ERFINV: return or_erfinv(x);
+    case O_PRINTME: case O_LIFT: return x;
+    case O_LOG1P: return log1p(x);
+    case O_EXPM1: return expm1(x);
+    case O_HYPOT: return hypot(x, y);
+    default: return NAN;
+  }
+}
+
+void or_der(int op, double x, double y, double f, double* d) {
+  const double pi = 3.14159265358979323846;
+  double a, b, c, t;
+  d[0] = 0;
+  d[1] = 0;
+  switch (op) {
+    case O_ASSIGN: d[0] = 1; break;
+    case O_ADD: d[0] = d[1] = 1; break;
+    case O_SUB: d[0] = 1; d[1] = -1; break;
+    case O_MUL: d[0] = y; d[1] = x; break;
+    case O_DIV: d[0] = 1 / y; d[1] = -f / y; break;
+    case O_NEG: d[0] = -1; break;
+    case O_EXP: d[0] = f; break;
+    case O_LOG: d[0] = 1 / x; break;
+    case O_POW: d[0] = y * pow(x, y - 1); d[1] = log(x) * f; break;
+    case O_CONSTPOW: d[0] = y * pow(x, y - 1); break;
+    case O_SQRT: d[0] = 1 / (f + f); break;
+    case O_SQ: d[0] = x + x; break;
+    case O_TWICE: d[0] = 2; break;
+    case O_SIN: d[0] = cos(x); break;
+    case O_COS: d[0] = -sin(x); break;
+    case O_TAN: t = cos(x); d[0] = 1 / (t * t); break;
+    case O_ASIN: d[0] = 1 / sqrt(1 - x * x); break;
+    case O_ACOS: d[0] = -1 / sqrt(1 - x * x); break;
+    case O_ATAN: d[0] = 1 / (1 + x * x); break;
+    case O_FMOD: case O_REMAINDER: d[0] = 1; d[1] = (f - x) / y; break;
+    case O_FABS: d[0] = or_sign(x); break;
+    case O_COPYSIGN: d[0] = copysign(1.0, y); break;
+    case O_IF_ELSE_ZERO: d[1] = x; break;
+    case O_ERF: d[0] = (2 / sqrt(pi)) * exp(-x * x); break;
+    case O_FMIN: a = x <= y; b = y <= x; c = a + b; d[0] = a / c; d[1] = b / c; break;
+    case O_FMAX: a = y <= x; b = x <= y; c = a + b; d[0] = a / c; d[1] = b / c; break;
+    case O_INV: d[0] = -f * f; break;
+    case O_SINH: d[0] = cosh(x); break;
+    case O_COSH: d[0] = sinh(x); break;
+    case O_TANH: d[0] = 1 - f * f; break;
+    case O_ASINH: d[0] = 1 / sqrt(1 + x * x); break;
+    case O_ACOSH: d[0] = 1 / sqrt(x - 1) / sqrt(x + 1); break;
+    case O_ATANH: d[0] = 1 / (1 - x * x); break;
+    case O_ATAN2: t = x * x + y * y; d[0] = y / t; d[1] = -x / t; break;
+    case O_ERFINV: d[0] = (sqrt(pi) / 2) * exp(f * f); break;
+    case O_PRINTME: case O_LIFT: d[0] = 1; break;
+    case O_LOG1P: d[0] = 1 / (1 + x); break;
+    case O_EXPM1: d[0] = exp(x); break;
+    case O_HYPOT: d[0] = x / f; d[1] = y / f; break;
+    default: break; /* comparisons, floor, ceil, sign, not: zero */
+  }
+}
+
+/* SXFunction::eval: returns 1 on an unsupported instruction, else 0 */
+int or_eval(const void* instr, int64_t n_instr, const double** arg, double** res, double* w) {
+  const or_instr* a = (const or_instr*)instr;
+  for (int64_t k = 0; k < n_instr; ++k) {
+    const or_instr* e = a + k;
+    switch (e->op) {
+      case O_CONST: w[e->i0] = e->u.d; break;
+      case O_INPUT: w[e->i0] = arg[e->u.s.i1] ? arg[e->u.s.i1][e->u.s.i2] : 0; break;
+      case O_OUTPUT:
+        if (res[e->i0]) res[e->i0][e->u.s.i2] = w[e->u.s.i1];
+        break;
+      case O_PARAMETER: case O_CALL: return 1;
+      default: w[e->i0] = or_fun(e->op, w[e->u.s.i1], w[e->u.s.i2]);
+    }
+  }
+  return 0;
+}
+
+/* Map serial (n_threads <= 1) or OpenMP over AoS arrays:
+ * instance k of input i at arg[i] + k*nnz_in[i]. */
+int or_map_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const int64_t* nnz_in,
+                int n_out, const int64_t* nnz_out, int64_t n, const double** arg, double** res,
+                int n_threads) {
+  int flag = 0;
+  int64_t k;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(n_threads > 0 ? n_threads : 1) reduction(|| : flag)
+#endif
+  {
+    const double* a[64];
+    double* r[64];
+    double wbuf[4096];
+    double* w = wbuf;
+    int i;
+    if (ws > 4096 || n_in > 64 || n_out > 64) {
+      flag = 1;
+    } else {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (k = 0; k < n; ++k) {
+        for (i = 0; i < n_in; ++i) a[i] = arg[i] ? arg[i] + k * nnz_in[i] : 0;
+        for (i = 0; i < n_out; ++i) r[i] = res[i] ? res[i] + k * nnz_out[i] : 0;
+        if (or_eval(instr, n_instr, a, r, w)) flag = 1;
+      }
+    }
+  }
+  return flag;
+}
+
+/* ---------------------------------------------------------------- LDL^T --- */
+/* casadi_ldl: sp_* in CasADi compact form [nrow, ncol, colind[ncol+1], row[nnz]] */
+void or_ldl(const int64_t* sp_a, const double* a, const int64_t* sp_lt, double* lt, double* d,
+            const int64_t* p, double* w) {
+  int64_t n = sp_lt[1], r, c, c1, k, k2;
+  const int64_t *lt_colind = sp_lt + 2, *lt_row = sp_lt + 2 + n + 1;
+  const int64_t *a_colind = sp_a + 2, *a_row = sp_a + 2 + n + 1;
+  for (r = 0; r < n; ++r) w[r] = 0;
+  for (c = 0; c < n; ++c) {
+    c1 = p[c];
+    for (k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = a[k];
+    for (k = lt_colind[c]; k < lt_colind[c + 1]; ++k) lt[k] = w[p[lt_row[k]]];
+    d[c] = w[p[c]];
+    for (k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = 0;
+  }
+  for (c = 0; c < n; ++c) {
+    for (k = lt_colind[c]; k < lt_colind[c + 1]; ++k) {
+      r = lt_row[k];
+      for (k2 = lt_colind[r]; k2 < lt_colind[r + 1]; ++k2) lt[k] -= lt[k2] * w[lt_row[k2]];
+      w[r] = lt[k];
+      lt[k] /= d[r];
+      d[c] -= w[r] * lt[k];
+    }
+    for (k = lt_colind[c]; k < lt_colind[c + 1]; ++k) w[lt_row[k]] = 0;
+  }
+}
+
+static void or_ldl_trs(const int64_t* sp_r, const double* nz_r, double* x, int tr) {
+  int64_t ncol = sp_r[1], c, k;
+  const int64_t *colind = sp_r + 2, *row = sp_r + 2 + ncol + 1;
+  if (tr) {
+    for (c = 0; c < ncol; ++c)
+      for (k = colind[c]; k < colind[c + 1]; ++k) x[c] -= nz_r[k] * x[row[k]];
+  } else {
+    for (c = ncol - 1; c >= 0; --c)
+      for (k = colind[c + 1] - 1; k >= colind[c]; --k) x[row[k]] -= nz_r[k] * x[c];
+  }
+}
+
+void or_ldl_solve(double* x, int64_t nrhs, const int64_t* sp_lt, const double* lt, const double* d,
+                  const int64_t* p, double* w) {
+  int64_t i, k, n = sp_lt[1];
+  for (k = 0; k < nrhs; ++k) {
+    for (i = 0; i < n; ++i) w[i] = x[p[i]];
+    or_ldl_trs(sp_lt, lt, w, 1);
+    for (i = 0; i < n; ++i) w[i] /= d[i];
+    or_ldl_trs(sp_lt, lt, w, 0);
+    for (i = 0; i < n; ++i) x[p[i]] = w[i];
+    x += n;
+  }
+}
+
+/* ------------------------------------------------------------------- QR --- */
+static double or_house(double* v, double* beta, int64_t nv) {
+  int64_t i;
+  double v0 = v[0], sigma = 0, s;
+  for (i = 1; i < nv; ++i) sigma += v[i] * v[i];
+  s = sqrt(v0 * v0 + sigma);
+  if (sigma == 0) {
+    v[0] = 1;
+    *beta = 2 * (double)(v0 <= 0);
+  } else {
+    v[0] = v0 <= 0 ? v0 - s : -sigma / (v0 + s);
+    *beta = -1 / (s * v[0]);
+  }
+  return s;
+}
+
+void or_qr(const int64_t* sp_a, const double* nz_a, double* x, const int64_t* sp_v, double* nz_v,
+           const int64_t* sp_r, double* nz_r, double* beta, const int64_t* prinv, const int64_t* pc) {
+  int64_t ncol = sp_a[1], nrow = sp_v[0], r, c, k, k1;
+  double alpha;
+  const int64_t *a_colind = sp_a + 2, *a_row = sp_a + 2 + ncol + 1;
+  const int64_t *v_colind = sp_v + 2, *v_row = sp_v + 2 + ncol + 1;
+  const int64_t *r_colind = sp_r + 2, *r_row = sp_r + 2 + ncol + 1;
+  for (r = 0; r < nrow; ++r) x[r] = 0;
+  for (c = 0; c < ncol; ++c) {
+    for (k = a_colind[pc[c]]; k < a_colind[pc[c] + 1]; ++k) x[prinv[a_row[k]]] = nz_a[k];
+    for (k = r_colind[c]; k < r_colind[c + 1] && (r = r_row[k]) < c; ++k) {
+      alpha = 0;
+      for (k1 = v_colind[r]; k1 < v_colind[r + 1]; ++k1) alpha += nz_v[k1] * x[v_row[k1]];
+      alpha *= beta[r];
+      for (k1 = v_colind[r]; k1 < v_colind[r + 1]; ++k1) x[v_row[k1]] -= alpha * nz_v[k1];
+      *nz_r++ = x[r];
+      x[r] = 0;
+    }
+    for (k = v_colind[c]; k < v_colind[c + 1]; ++k) {
+      nz_v[k] = x[v_row[k]];
+      x[v_row[k]] = 0;
+    }
+    *nz_r++ = or_house(nz_v + v_colind[c], beta + c, v_colind[c + 1] - v_colind[c]);
+  }
+}
+
+static void or_qr_mv(const int64_t* sp_v, const double* v, const double* beta, double* x, int tr) {
+  int64_t ncol = sp_v[1], c, c1, k;
+  double alpha;
+  const int64_t *colind = sp_v + 2, *row = sp_v + 2 + ncol + 1;
+  for (c1 = 0; c1 < ncol; ++c1) {
+    c = tr ? c1 : ncol - 1 - c1;
+    alpha = 0;
+    for (k = colind[c]; k < colind[c + 1]; ++k) alpha += v[k] * x[row[k]];
+    alpha *= beta[c];
+    for (k = colind[c]; k < colind[c + 1]; ++k) x[row[k]] -= alpha * v[k];
+  }
+}
+
+static void or_qr_trs(const int64_t* sp_r, const double* nz_r, double* x, int tr) {
+  int64_t ncol = sp_r[1], r, c, k;
+  const int64_t *colind = sp_r + 2, *row = sp_r + 2 + ncol + 1;
+  if (tr) {
+    for (c = 0; c < ncol; ++c)
+      for (k = colind[c]; k < colind[c + 1]; ++k) {
+        r = row[k];
+        if (r == c) x[c] /= nz_r[k];
+        else x[c] -= nz_r[k] * x[r];
+      }
+  } else {
+    for (c = ncol - 1; c >= 0; --c)
+      for (k = colind[c + 1] - 1; k >= colind[c]; --k) {
+        r = row[k];
+        if (r == c) x[r] /= nz_r[k];
+        else x[r] -= nz_r[k] * x[c];
+      }
+  }
+}
+
+void or_qr_solve(double* x, int64_t nrhs, int tr, const int64_t* sp_v, const double* v,
+                 const int64_t* sp_r, const double* r, const double* beta, const int64_t* prinv,
+                 const int64_t* pc, double* w) {
+  int64_t k, c, nrow_ext = sp_v[0], ncol = sp_v[1];
+  for (k = 0; k < nrhs; ++k) {
+    if (tr) {
+      for (c = 0; c < ncol; ++c) w[c] = x[pc[c]];
+      or_qr_trs(sp_r, r, w, 1);
+      or_qr_mv(sp_v, v, beta, w, 0);
+      for (c = 0; c < ncol; ++c) x[c] = w[prinv[c]];
+    } else {
+      for (c = 0; c < nrow_ext; ++c) w[c] = 0;
+      for (c = 0; c < ncol; ++c) w[prinv[c]] = x[c];
+      or_qr_mv(sp_v, v, beta, w, 1);
+      or_qr_trs(sp_r, r, w, 0);
+      for (c = 0; c < ncol; ++c) x[pc[c]] = w[c];
+    }
+    x += ncol;
+  }
+}
+
+int64_t or_qr_singular(double* rmin, int64_t* irmin, const double* nz_r, const int64_t* sp_r,
+                       const int64_t* pc, double eps) {
+  int64_t ncol = sp_r[1], c, nullity = 0;
+  const int64_t* r_colind = sp_r + 2;
+  double rd, rd_min = 0;
+  for (c = 0; c < ncol; ++c) {
+    rd = fabs(nz_r[r_colind[c + 1] - 1]);
+    if (rd < eps) nullity++;
+    if (c == 0 || rd < rd_min) {
+      rd_min = rd;
+      if (rmin) *rmin = rd;
+      if (irmin) *irmin = pc[c];
+    }
+  }
+  return nullity;
+}
+
+/* Batched factor + solve over instance-major arrays, for the CPU baseline:
+ * a: n_inst x nnz(A), x: n_inst x n. Returns the number of zero pivots. */
+int64_t or_ldl_batch(const int64_t* sp_a, const int64_t* sp_lt, const int64_t* p, int64_t n_inst,
+                     const double* a, double* x, double* work, int n_threads) {
+  int64_t n = sp_lt[1], nnz_a = sp_a[2 + sp_a[1]], nnz_lt = sp_lt[2 + n], fails = 0, k;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(n_threads > 0 ? n_threads : 1) reduction(+ : fails) schedule(static)
+#endif
+  for (k = 0; k < n_inst; ++k) {
+#ifdef _OPENMP
+    double* wk = work + (int64_t)omp_get_thread_num() * (nnz_lt + 2 * n);
+#else
+    double* wk = work;
+#endif
+    double *lt = wk, *d = wk + nnz_lt, *w = wk + nnz_lt + n;
+    int64_t i;
+    or_ldl(sp_a, a + k * nnz_a, sp_lt, lt, d, p, w);
+    for (i = 0; i < n; ++i)
+      if (d[i] == 0) fails++;
+    or_ldl_solve(x + k * n, 1, sp_lt, lt, d, p, w);
+  }
+  return fails;
+}

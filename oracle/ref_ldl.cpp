@@ -1,0 +1,20 @@
+// Test infrastructure: exposes the reference's own casadi_ldl runtime, compiled
+// from casadi/core/runtime/casadi_ldl.hpp where it lies in the reference tree
+// (include path set by oracle/Makefile), through a C entry point for ctypes.
+// casadi_types.hpp supplies casadi_int, exactly as in the reference build.
+#include "casadi/core/casadi_types.hpp"
+#include "casadi/core/runtime/casadi_ldl.hpp"
+
+extern "C" {
+
+void ref_ldl(const long long* sp_a, const double* a, const long long* sp_lt, double* lt, double* d,
+             const long long* p, double* w) {
+  casadi_ldl(sp_a, a, sp_lt, lt, d, p, w);
+}
+
+void ref_ldl_solve(double* x, long long nrhs, const long long* sp_lt, const double* lt, const double* d,
+                   const long long* p, double* w) {
+  casadi_ldl_solve(x, nrhs, sp_lt, lt, d, p, w);
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+"""Host symbolic layer: simplification, tape compilation, derivative Functions.
+
+Pinned against closed forms (numpy re-implementations of the model equations),
+finite differences, and CasADi's documented rewrite rules
+(casadi/core/calculus.hpp:1812-2139, casadi/core/sx_elem.cpp:189-263).
+"""
+import numpy as np
+import pytest
+
+import casadi_amd as ca
+import oracle_ref as O
+from casadi_amd import workloads as W
+from casadi_amd.ops import OP
+from casadi_amd.sx import _node_op
+
+
+def node(e):
+    return int(e.nz[0])
+
+
+def op_of(e):
+    return _node_op(node(e))
+
+
+def test_simplification_rules():
+    x, y = ca.SX.sym("x"), ca.SX.sym("y")
+    assert node(x + 0) == node(x)
+    assert node(0 + x) == node(x)
+    assert node(x * 1) == node(x)
+    assert op_of(x - x) == OP.CONST and float(x - x) == 0
+    assert op_of(x * 2) == OP.TWICE                 # 2*x -> twice(x)
+    assert op_of(x + x) == OP.TWICE                 # x+x -> 2*x -> twice(x)
+    assert op_of(x * x) == OP.SQ                    # x*x -> sq(x)
+    assert node(-(-x)) == node(x)
+    assert node(ca.sqrt(x) * ca.sqrt(x)) == node(x)   # x*x -> sq(x), sq(sqrt(x)) -> x
+    assert node(ca.sq(ca.sqrt(x))) == node(x)       # sqrt(x)^2 = x
+    assert node(ca.log(ca.exp(x))) == node(x)
+    assert op_of(x ** 0.5) == OP.SQRT
+    assert op_of(x ** 3) == OP.MUL                  # x*x^2
+    assert op_of(x ** 2.5) == OP.CONSTPOW
+    assert float(ca.sin(x) ** 2 + ca.cos(x) ** 2) == 1.0
+    assert op_of(1 / x) == OP.INV
+    assert float(ca.SX(3.0) * 4.0) == 12.0          # constant folding
+    # x * if_else_zero(c, y) -> if_else_zero(c, x*y) (NaN-safe rule 1)
+    c = ca.SX.sym("c")
+    e = x * ca.if_else_zero(c, y)
+    assert op_of(e) == OP.IF_ELSE_ZERO
+
+
+def _np_cartpole_ode(s, u):
+    mc, mp, l, g = W.CP_MC, W.CP_MP, W.CP_L, W.CP_G
+    th, xd, thd = s[1], s[2], s[3]
+    sn, cs = np.sin(th), np.cos(th)
+    temp = (u + mp * l * thd * thd * sn) / (mc + mp)
+    thacc = (g * sn - cs * temp) / (l * (4.0 / 3.0 - mp * cs * cs / (mc + mp)))
+    xacc = temp - mp * l * thacc * cs / (mc + mp)
+    return np.array([xd, thd, xacc, thacc])
+
+
+def _np_rk4(s, u, dt):
+    k1 = _np_cartpole_ode(s, u)
+    k2 = _np_cartpole_ode(s + dt / 2 * k1, u)
+    k3 = _np_cartpole_ode(s + dt / 2 * k2, u)
+    k4 = _np_cartpole_ode(s + dt * k3, u)
+    return s + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+def test_cartpole_tape_matches_closed_form():
+    f = W.cartpole_step()
+    rng = np.random.default_rng(1)
+    n = 200
+    x = rng.uniform(-1, 1, (n, 4))
+    u = rng.uniform(-10, 10, (n, 1))
+    (y,) = O.map_eval(f, n, [x, u])
+    ref = np.array([_np_rk4(x[k], u[k, 0], 0.05) for k in range(n)])
+    assert O.rel_err(y, ref) < 1e-13
+
+
+def test_rollout_matches_closed_form():
+    f = W.cartpole_rollout(20)
+    rng = np.random.default_rng(2)
+    x0 = rng.uniform(-1, 1, 4)
+    us = rng.uniform(-10, 10, 20)
+    s, J = x0.copy(), 0.0
+    for t in range(20):
+        J += s @ s + 0.01 * us[t] ** 2
+        s = _np_rk4(s, us[t], 0.05)
+    J += s @ s
+    assert abs(float(f(x0, us)) - J) <= 1e-13 * max(1, abs(J))
+
+
+def test_host_vm_equals_oracle_vm_bitwise():
+    f = W.quadrotor_step()
+    rng = np.random.default_rng(3)
+    n = 64
+    x = rng.uniform(-1, 1, (n, 12))
+    u = rng.uniform(0, 2, (n, 4))
+    (ref,) = O.map_eval(f, n, [x, u])
+    got = f.map(n, "serial")(x.T, u.T)
+    assert np.array_equal(got, ref.T)
+    got_omp = f.map(n, "openmp", 4)(x.T, u.T)
+    assert np.array_equal(got_omp, ref.T)
+
+
+def test_map_equals_loop():
+    f = W.arm_dynamics()
+    rng = np.random.default_rng(4)
+    n = 10
+    z = rng.uniform(-1, 1, (6, n))
+    F = f.map(n, "serial")(z)
+    for k in range(n):
+        assert np.array_equal(F[:, k], f(z[:, k]).reshape(-1))
+
+
+def test_tape_structure():
+    f = W.cartpole_step()
+    a = f.instructions()
+    assert np.sum(a["op"] == OP.INPUT) == 5
+    assert np.sum(a["op"] == OP.OUTPUT) == 4
+    assert f.sz_w() < f.n_instructions()          # live-variable slot reuse
+    assert a["i0"][a["op"] != OP.OUTPUT].max() < f.sz_w()
+    # unary instructions carry i2 == i1
+    un = np.isin(a["op"], [OP.SIN, OP.COS, OP.SQ, OP.NEG, OP.TWICE])
+    assert np.array_equal(a["i1"][un], a["i2"][un])
+
+
+def test_free_variable_error():
+    x, y = ca.SX.sym("x"), ca.SX.sym("y")
+    with pytest.raises(ca.CasadiHipError, match="free"):
+        ca.Function("f", [x], [x * y])
+    f = ca.Function("f", [x], [x * y], opts={"allow_free": True})
+    assert f.has_free()
+
+
+def _fd_jac(fun, x, h=1e-6):
+    cols = []
+    for i in range(x.size):
+        e = np.zeros_like(x)
+        e.flat[i] = h
+        cols.append((fun(x + e) - fun(x - e)) / (2 * h))
+    return np.stack(cols, axis=1)
+
+
+def test_jacobian_function_matches_fd():
+    f = W.cartpole_step()
+    J = f.jacobian()
+    assert J.name() == "jac_cartpole"
+    assert J.name_out() == ["jac_xf_x", "jac_xf_u"]
+    x0, u0 = np.array([0.1, -0.4, 0.3, 0.2]), 2.0
+    jx, ju = J(x0, u0, 0)
+    fdx = _fd_jac(lambda v: f(v, u0).reshape(-1), x0)
+    fdu = _fd_jac(lambda v: f(x0, v[0]).reshape(-1), np.array([u0]))
+    assert np.allclose(jx, fdx, atol=1e-7)
+    assert np.allclose(ju, fdu, atol=1e-7)
+    # structural zeros: the cart position does not enter the dynamics
+    sp = J.sparsity_out(0)
+    assert not sp.dense_mask()[1, 0] and sp.dense_mask()[0, 0]
+
+
+def test_forward_and_reverse_functions():
+    f = W.arm_dynamics()
+    z0 = np.array([0.3, -0.2, 0.5, 0.1, 1.0, -0.5])
+    fd = _fd_jac(lambda v: f(v).reshape(-1), z0)
+    v = np.array([0.2, -1.0, 0.3, 0.0, 0.7, 0.1])
+    Ff = f.forward(1)
+    assert Ff.name_in() == ["z", "out_zdot", "fwd_z"]
+    assert np.allclose(Ff(z0, 0, v).reshape(-1), fd @ v, atol=1e-7)
+    w = np.array([1.0, -2.0, 0.5, 0.25])
+    Fr = f.reverse(1)
+    assert Fr.name_in() == ["z", "out_zdot", "adj_zdot"] and Fr.name_out() == ["adj_z"]
+    assert np.allclose(Fr(z0, 0, w).reshape(-1), fd.T @ w, atol=1e-7)
+
+
+def test_hessian_symmetric_and_correct():
+    x = ca.SX.sym("x", 3)
+    e = ca.sin(x[0] * x[1]) + x[2] ** 3 * x[0] + ca.exp(x[1])
+    H, g = ca.hessian(e, x)
+    assert H.sparsity.is_symmetric()
+    fH = ca.Function("h", [x], [H])
+    x0 = np.array([0.3, 0.6, -0.2])
+    Hv = fH(x0)
+    a, b, c = x0
+    ref = np.array([[-b * b * np.sin(a * b), np.cos(a * b) - a * b * np.sin(a * b), 3 * c * c],
+                    [np.cos(a * b) - a * b * np.sin(a * b), -a * a * np.sin(a * b) + np.exp(b), 0],
+                    [3 * c * c, 0, 6 * c * a]])
+    assert np.allclose(Hv, ref, atol=1e-14)
+
+
+def test_symbolic_call_inlines():
+    f = W.cartpole_step()
+    x, u = ca.SX.sym("x", 4), ca.SX.sym("u")
+    y = f(f(x, u), u)
+    g = ca.Function("g", [x, u], [y])
+    x0 = np.array([0.1, 0.2, 0.3, 0.4])
+    assert np.array_equal(g(x0, 1.0), f(f(x0, 1.0), 1.0))
