@@ -623,6 +623,16 @@ SXFunction::JacPlan SXFunction::jac_plan(bool symmetric) const {
   return plan;
 }
 
+// Plan over all inputs / outputs flattened (veccat), as Function::jacobian
+// builds it (casadi/core/x_function.hpp:939-960).
+SXFunction::JacPlan SXFunction::jac_plan_all(bool symmetric) const {
+  if (in_.size() == 1 && out_.size() == 1) return jac_plan(symmetric);
+  FunctionOptions o;
+  o.allow_free = true;
+  SXFunction tmp("flattened_" + name_, {veccat(in_)}, {veccat(out_)}, {}, {}, o);
+  return tmp.jac_plan(symmetric);
+}
+
 // XFunction::jac (casadi/core/x_function.hpp:442-840)
 SXMat SXFunction::jac(bool symmetric, bool allow_forward, bool allow_reverse) const {
   (void)allow_forward;

@@ -101,6 +101,8 @@ class SXFunction {
     ivec src_dir, src_nz;
   };
   JacPlan jac_plan(bool symmetric) const;
+  // the same over all inputs and outputs concatenated (flat nonzero indices)
+  JacPlan jac_plan_all(bool symmetric) const;
 
   int64_t n_instructions() const { return static_cast<int64_t>(algorithm_.size()); }
 

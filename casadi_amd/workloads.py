@@ -145,63 +145,101 @@ def arm_dynamics():
 
 
 # --- KKT matrix for the batched sparse LDL -----------------------------------------
-def ocp_kkt_sparsity(N=30, nx=4, nu=1):
-    """Symmetric KKT pattern of an OCP with horizon N: primal w = [x_0, u_0, ...,
-    x_{N-1}, u_{N-1}, x_N] (N*(nx+nu)+nx) and duals for the N+1 equality blocks
-    x_0 = x0, x_{k+1} = A x_k + B u_k.  Dimension N*(nx+nu) + (N+1)*nx."""
-    from .sx import Sparsity
-    nw = N * (nx + nu) + nx
+def _kkt_blocks(N, nx, nu):
+    """Block layout of the horizon-N OCP KKT system: primal w = [x_0, u_0, ...,
+    x_{N-1}, u_{N-1}] (N*(nx+nu)) and N+1 equality blocks of nx rows:
+    x_0 = x_init, x_{k+1} = A_k x_k + B_k u_k for k < N-1, and the terminal
+    condition A_{N-1} x_{N-1} + B_{N-1} u_{N-1} = x_target.
+    Dimension N*(nx+nu) + (N+1)*nx (= 274 for N=30, nx=4, nu=1)."""
+    nw = N * (nx + nu)
     nl = (N + 1) * nx
+    return nw, nl
+
+
+def ocp_kkt_sparsity(N=30, nx=4, nu=1):
+    """Symmetric pattern [[H, G^T], [G, -reg I]] of the OCP KKT matrix."""
+    from .sx import Sparsity
+    nw, nl = _kkt_blocks(N, nx, nu)
     n = nw + nl
     rows, cols = [], []
 
     def block(r0, c0, nr, nc):
         for i in range(nr):
             for j in range(nc):
-                rows.append(r0 + i)
-                cols.append(c0 + j)
-                rows.append(c0 + j)
-                cols.append(r0 + i)
+                rows.extend((r0 + i, c0 + j))
+                cols.extend((c0 + j, r0 + i))
 
     for k in range(N):
         o = k * (nx + nu)
-        block(o, o, nx + nu, nx + nu)  # stage Hessian
-    block(N * (nx + nu), N * (nx + nu), nx, nx)  # terminal Hessian
-    # constraint Jacobian blocks (rows of lambda, columns of w)
-    block(nw, 0, nx, nx)  # x_0 = x0: -I
+        block(o, o, nx + nu, nx + nu)          # stage Hessian
+    block(nw, 0, nx, nx)                       # x_0 = x_init: -I
     for k in range(N):
         ro = nw + (k + 1) * nx
         o = k * (nx + nu)
-        block(ro, o, nx, nx + nu)          # [A B]
-        block(ro, o + nx + nu, nx, nx)     # -I
-    for i in range(nl):  # dual diagonal (regularisation)
+        block(ro, o, nx, nx + nu)              # [A_k B_k]
+        if k + 1 < N:
+            block(ro, o + nx + nu, nx, nx)     # -I on x_{k+1}
+    for i in range(nl):                        # dual diagonal (regularisation)
         rows.append(nw + i)
         cols.append(nw + i)
     return Sparsity.triplet(n, n, rows, cols), nw, nl
 
 
 def ocp_kkt_values(rng, N=30, nx=4, nu=1, reg=-1e-8):
-    """Dense KKT matrix with random SPD stage Hessians A^T A + I (A ~ N(0,1)),
-    dynamics blocks [A B] ~ N(0, 0.3) (+ identity on A) and -reg on the dual
-    diagonal; returns the dense matrix (symmetric)."""
-    nw = N * (nx + nu) + nx
-    nl = (N + 1) * nx
+    """Dense KKT matrix: stage Hessians A^T A + I (A ~ N(0,1)), dynamics blocks
+    [A B] ~ N(0, 0.3) plus identity on A, -I couplings, and reg on the dual
+    diagonal."""
+    nw, nl = _kkt_blocks(N, nx, nu)
     n = nw + nl
     K = np.zeros((n, n))
-    for k in range(N + 1):
-        m = nx + nu if k < N else nx
-        o = k * (nx + nu)
+    m = nx + nu
+    for k in range(N):
+        o = k * m
         A = rng.standard_normal((m, m))
         K[o:o + m, o:o + m] = A.T @ A + np.eye(m)
     K[nw:nw + nx, 0:nx] = -np.eye(nx)
     for k in range(N):
         ro = nw + (k + 1) * nx
-        o = k * (nx + nu)
-        AB = rng.normal(0.0, 0.3, (nx, nx + nu))
+        o = k * m
+        AB = rng.normal(0.0, 0.3, (nx, m))
         AB[:, :nx] += np.eye(nx)
-        K[ro:ro + nx, o:o + nx + nu] = AB
-        K[ro:ro + nx, o + nx + nu:o + 2 * nx + nu] = -np.eye(nx)
-    K[nw:, :nw] = K[nw:, :nw]
+        K[ro:ro + nx, o:o + m] = AB
+        if k + 1 < N:
+            K[ro:ro + nx, o + m:o + m + nx] = -np.eye(nx)
     K[:nw, nw:] = K[nw:, :nw].T
     K[np.arange(nw, n), np.arange(nw, n)] = reg
     return K
+
+
+def ocp_kkt_batch(rng, n_inst, N=30, nx=4, nu=1, reg=-1e-8):
+    """Nonzeros of n_inst random KKT matrices (same distribution as
+    ocp_kkt_values), laid out [nonzero][instance]; returns (sp, values)."""
+    sp, nw, nl = ocp_kkt_sparsity(N, nx, nu)
+    m = nx + nu
+    vals = np.zeros((sp.nnz(), n_inst))
+    col = sp.get_col()
+    index = {(int(r), int(c)): k for k, (r, c) in enumerate(zip(sp.row, col))}
+
+    def put(r0, c0, block):  # block: (n_inst, nr, nc); writes it and its transpose
+        nr, nc = block.shape[1:]
+        for i in range(nr):
+            for j in range(nc):
+                vals[index[(r0 + i, c0 + j)]] = block[:, i, j]
+                vals[index[(c0 + j, r0 + i)]] = block[:, i, j]
+
+    A = rng.standard_normal((n_inst, N, m, m))
+    H = np.einsum("skij,skil->skjl", A, A) + np.eye(m)
+    for k in range(N):
+        put(k * m, k * m, H[:, k])
+    eye = np.broadcast_to(-np.eye(nx), (n_inst, nx, nx))
+    put(nw, 0, eye)
+    AB = rng.normal(0.0, 0.3, (n_inst, N, nx, m))
+    AB[:, :, :, :nx] += np.eye(nx)
+    for k in range(N):
+        ro = nw + (k + 1) * nx
+        put(ro, k * m, AB[:, k])
+        if k + 1 < N:
+            put(ro, k * m + m, eye)
+    for i in range(nl):
+        vals[index[(nw + i, nw + i)]] = reg
+    return sp, vals

@@ -20,6 +20,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -46,6 +47,19 @@ enum {
 
 static double or_sign(double x) { return x < 0 ? -1 : x > 0 ? 1 : x; }
 
+/* CASADI_MATH_BINARY_BUILTIN (casadi/core/calculus.hpp:1506-1527) */
+static int or_is_binary(int op) {
+  switch (op) {
+    case O_ADD: case O_SUB: case O_MUL: case O_DIV: case O_POW: case O_CONSTPOW: case O_LT:
+    case O_LE: case O_EQ: case O_NE: case O_AND: case O_OR: case O_COPYSIGN: case O_FMOD:
+    case O_REMAINDER: case O_FMIN: case O_FMAX: case O_ATAN2: case O_PRINTME: case O_LIFT:
+    case O_HYPOT:
+      return 1;
+    default:
+      return 0;
+  }
+}
+
 /* calculus.hpp:300-327 (used when the C library has no erfinv) */
 static double or_erfinv(double x) {
   const double pi = 3.14159265358979323846;
@@ -71,7 +85,35 @@ static double or_erfinv(double x) {
   return y;
 }
 
+/* Conditioning probe: when nonzero, every transcendental result is moved by
+ * one ulp (towards +inf for odd calls, -inf for even ones), so a test can
+ * measure how far a workload's outputs move under 1-ulp differences between
+ * two correct libm implementations.  Off by default. */
+static int or_perturb = 0;
+static unsigned long or_perturb_count = 0;
+void or_set_perturb(int on) {
+  or_perturb = on;
+  or_perturb_count = 0;
+}
+static double or_nudge(double v) {
+  if (!or_perturb) return v;
+  return nextafter(v, (or_perturb_count++ & 1) ? INFINITY : -INFINITY);
+}
+
+static double or_fun_raw(int op, double x, double y);
 double or_fun(int op, double x, double y) {
+  double v = or_fun_raw(op, x, y);
+  switch (op) {
+    case O_EXP: case O_LOG: case O_POW: case O_CONSTPOW: case O_SIN: case O_COS: case O_TAN:
+    case O_ASIN: case O_ACOS: case O_ATAN: case O_ERF: case O_SINH: case O_COSH: case O_TANH:
+    case O_ASINH: case O_ACOSH: case O_ATANH: case O_ATAN2: case O_LOG1P: case O_EXPM1: case O_HYPOT:
+      return or_nudge(v);
+    default:
+      return v;
+  }
+}
+
+static double or_fun_raw(int op, double x, double y) {
   switch (op) {
     case O_ASSIGN: return x;
     case O_ADD: return x + y;
@@ -206,10 +248,9 @@ int or_map_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const 
   {
     const double* a[64];
     double* r[64];
-    double wbuf[4096];
-    double* w = wbuf;
+    double* w = (double*)malloc(sizeof(double) * (size_t)(ws > 0 ? ws : 1));
     int i;
-    if (ws > 4096 || n_in > 64 || n_out > 64) {
+    if (!w || n_in > 64 || n_out > 64) {
       flag = 1;
     } else {
 #ifdef _OPENMP
@@ -221,6 +262,115 @@ int or_map_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const 
         if (or_eval(instr, n_instr, a, r, w)) flag = 1;
       }
     }
+    free(w);
+  }
+  return flag;
+}
+
+/* Numeric reverse sweep: SXFunction::ad_reverse's rules
+ * (casadi/core/sx_function.cpp:1454-1560) applied to numbers.  Forward pass
+ * recording the value each instruction overwrites (trail, n_instr doubles),
+ * then the backward walk: f = w[i0], restore w[i0], x = w[i1], y = w[i2],
+ * seed = aw[i0], aw[i0] = 0, aw[i1] += d0*seed, aw[i2] += d1*seed
+ * (if_else_zero: aw[i2] += if_else_zero(d1, seed)).
+ * asens[i] (nnz_in[i] each) is zeroed first; res may hold NULL entries. */
+int or_adj_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const int64_t* nnz_in,
+                const double** arg, const double** aseed, double** res, double** asens,
+                double* w, double* aw, double* trail) {
+  const or_instr* a = (const or_instr*)instr;
+  int64_t k, i;
+  double d[2], f, x, y, seed;
+  for (i = 0; i < n_in; ++i)
+    if (asens[i]) memset(asens[i], 0, sizeof(double) * (size_t)nnz_in[i]);
+  for (i = 0; i < ws; ++i) {
+    w[i] = 0;
+    aw[i] = 0;
+  }
+  for (k = 0; k < n_instr; ++k) {
+    const or_instr* e = a + k;
+    if (e->op == O_PARAMETER || e->op == O_CALL) return 1;
+    if (e->op != O_OUTPUT) trail[k] = w[e->i0];
+    switch (e->op) {
+      case O_CONST: w[e->i0] = e->u.d; break;
+      case O_INPUT: w[e->i0] = arg[e->u.s.i1] ? arg[e->u.s.i1][e->u.s.i2] : 0; break;
+      case O_OUTPUT:
+        if (res && res[e->i0]) res[e->i0][e->u.s.i2] = w[e->u.s.i1];
+        break;
+      default: w[e->i0] = or_fun(e->op, w[e->u.s.i1], w[e->u.s.i2]);
+    }
+  }
+  for (k = n_instr - 1; k >= 0; --k) {
+    const or_instr* e = a + k;
+    switch (e->op) {
+      case O_OUTPUT:
+        if (aseed[e->i0]) aw[e->u.s.i1] += aseed[e->i0][e->u.s.i2];
+        break;
+      case O_INPUT:
+        if (asens[e->u.s.i1]) asens[e->u.s.i1][e->u.s.i2] = aw[e->i0];
+        aw[e->i0] = 0;
+        w[e->i0] = trail[k];
+        break;
+      case O_CONST:
+        aw[e->i0] = 0;
+        w[e->i0] = trail[k];
+        break;
+      default:
+        f = w[e->i0];
+        w[e->i0] = trail[k];
+        x = w[e->u.s.i1];
+        y = w[e->u.s.i2];
+        seed = aw[e->i0];
+        aw[e->i0] = 0;
+        or_der(e->op, x, y, f, d);
+        if (e->op == O_IF_ELSE_ZERO) {
+          aw[e->u.s.i2] += d[1] == 0 ? 0 : seed;
+        } else if (or_is_binary(e->op)) {
+          aw[e->u.s.i1] += d[0] * seed;
+          aw[e->u.s.i2] += d[1] * seed;
+        } else {
+          aw[e->u.s.i1] += d[0] * seed; /* unary: i2 mirrors i1 */
+        }
+    }
+  }
+  return 0;
+}
+
+/* Batched numeric reverse sweep over AoS arrays (serial or OpenMP). */
+int or_adj_map(const void* instr, int64_t n_instr, int64_t ws, int n_in, const int64_t* nnz_in,
+               int n_out, const int64_t* nnz_out, int64_t n, const double** arg, const double** aseed,
+               double** res, double** asens, int n_threads) {
+  int flag = 0;
+  int64_t k;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(n_threads > 0 ? n_threads : 1) reduction(|| : flag)
+#endif
+  {
+    const double* a[64];
+    const double* s[64];
+    double* r[64];
+    double* g[64];
+    double* buf = (double*)malloc(sizeof(double) * (size_t)(2 * ws + n_instr + 2));
+    int i;
+    if (!buf || n_in > 64 || n_out > 64) {
+      flag = 1;
+    } else {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (k = 0; k < n; ++k) {
+        for (i = 0; i < n_in; ++i) {
+          a[i] = arg[i] ? arg[i] + k * nnz_in[i] : 0;
+          g[i] = asens[i] ? asens[i] + k * nnz_in[i] : 0;
+        }
+        for (i = 0; i < n_out; ++i) {
+          s[i] = aseed[i] ? aseed[i] + k * nnz_out[i] : 0;
+          r[i] = res && res[i] ? res[i] + k * nnz_out[i] : 0;
+        }
+        if (or_adj_eval(instr, n_instr, ws, n_in, nnz_in, a, s, r, g, buf, buf + ws, buf + 2 * ws))
+          flag = 1;
+      }
+    }
+    free(buf);
   }
   return flag;
 }

@@ -42,6 +42,10 @@ def _load():
     lib.or_qr_solve.argtypes = [PD, I64, C.c_int, PI64, PD, PI64, PD, PD, PI64, PI64, PD]
     lib.or_qr_singular.restype = I64
     lib.or_qr_singular.argtypes = [PD, PI64, PD, PI64, PI64, C.c_double]
+    lib.or_set_perturb.restype = None
+    lib.or_set_perturb.argtypes = [C.c_int]
+    lib.or_adj_map.restype = C.c_int
+    lib.or_adj_map.argtypes = [P, I64, I64, C.c_int, PI64, C.c_int, PI64, I64, P, P, P, P, C.c_int]
     lib.or_ldl_batch.restype = I64
     lib.or_ldl_batch.argtypes = [PI64, PI64, PI64, I64, PD, PD, PD, C.c_int]
     return lib
@@ -87,6 +91,22 @@ def map_eval(f, n, args_aos, n_threads=1):
                            len(nnz_out), _pi(nnz_out), n, _ptrs(arg), _ptrs(res), n_threads)
     assert flag == 0, "oracle VM rejected the tape"
     return res
+
+
+def adj_map(f, n, args_aos, seeds_aos, n_threads=1):
+    """Numeric reverse sweep (ad_reverse rules on numbers) over n instances.
+    Returns (res, asens) lists of (n, nnz) arrays."""
+    instr = f.instructions()
+    nnz_in = np.array([f.nnz_in(i) for i in range(f.n_in())], np.int64)
+    nnz_out = np.array([f.nnz_out(i) for i in range(f.n_out())], np.int64)
+    arg = [np.ascontiguousarray(a, dtype=float) if a is not None else None for a in args_aos]
+    seed = [np.ascontiguousarray(a, dtype=float) if a is not None else None for a in seeds_aos]
+    res = [np.zeros((n, k)) for k in nnz_out]
+    sens = [np.zeros((n, k)) for k in nnz_in]
+    flag = lib.or_adj_map(instr.ctypes.data, instr.size, f.sz_w(), len(nnz_in), _pi(nnz_in), len(nnz_out),
+                          _pi(nnz_out), n, _ptrs(arg), _ptrs(seed), _ptrs(res), _ptrs(sens), n_threads)
+    assert flag == 0, "oracle reverse sweep rejected the tape"
+    return res, sens
 
 
 def ldl(sp_a, a, sp_lt, p):
@@ -142,6 +162,19 @@ def qr_singular(r, sp_r, pc, eps):
                              _pi(np.ascontiguousarray(sp_r, np.int64)), _pi(np.ascontiguousarray(pc, np.int64)),
                              eps)
     return nul, rmin.value, irmin.value
+
+
+class perturbed_transcendentals:
+    """Context manager: every transcendental result of the oracle moves by one
+    ulp (alternating direction) -- the rounding envelope of a workload."""
+
+    def __enter__(self):
+        lib.or_set_perturb(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib.or_set_perturb(0)
+        return False
 
 
 def rel_err(y, yref):

@@ -146,11 +146,48 @@ def test_rollout_adjoint_checkpointed(ca, cuda, rollout):
     # checkpointing does not change a single bit
     assert np.array_equal(gx.cpu().numpy(), gx0.cpu().numpy())
     assert np.array_equal(gu.cpu().numpy(), gu0.cpu().numpy())
+    g = np.concatenate([gx.cpu().numpy(), gu.cpu().numpy()]).T          # (n, 104)
+    # The 100-step rollout with |u| <= 10 is chaotic: a 1-ulp difference in one
+    # sin/cos (OCML vs glibc, both correct to < 1 ulp) grows by ~1e4 along the
+    # trajectory.  Errors are therefore measured per instance, scaled by the
+    # instance's gradient norm; the median must meet 1e-13 and the maximum
+    # must stay inside the oracle's own envelope: how far the CPU oracle moves
+    # when every transcendental result is nudged by one ulp.
+    (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
+    go = np.concatenate([ox, ou], axis=1)
+    scale = np.maximum(np.max(np.abs(go), axis=1, keepdims=True), 1.0)
+    with O.perturbed_transcendentals():
+        (Jp,), (px, pu) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
+    env = float(np.max(np.abs(np.concatenate([px, pu], axis=1) - go) / scale))
+    env_J = O.rel_err(Jp, Jo)
+    per_inst = np.max(np.abs(g - go) / scale, axis=1)
+    # 1) the numeric reverse sweep on the CPU (same algorithm)
+    err_num, med_num = float(per_inst.max()), float(np.median(per_inst))
+    # 2) CasADi's symbolic reverse(1) tape evaluated on the oracle VM
     R = f.reverse(1)
     rx, ru = O.map_eval(R, n, [x0.T, us.T, None, seed.T])
-    assert O.rel_err(gx.cpu().numpy(), rx.T) <= TOL
-    assert O.rel_err(gu.cpu().numpy(), ru.T) <= TOL
-    assert O.rel_err(J.cpu().numpy(), O.map_eval(f, n, [x0.T, us.T])[0].T) <= TOL
+    gr = np.concatenate([rx, ru], axis=1)
+    per_sym = np.max(np.abs(g - gr) / scale, axis=1)
+    print(f"[rollout] vs numeric sweep: max {err_num:.2e} median {med_num:.2e}; vs reverse(1) tape: "
+          f"max {per_sym.max():.2e} median {np.median(per_sym):.2e}; 1-ulp envelope {env:.2e}", flush=True)
+    assert med_num <= TOL and np.median(per_sym) <= TOL
+    assert err_num <= max(TOL, env) and per_sym.max() <= max(TOL, env)
+    assert O.rel_err(J.cpu().numpy(), Jo.T) <= max(TOL, env_J)
+    # 3) central finite differences (h = 1e-6) on 64 sampled instances
+    idx = np.random.default_rng(5).choice(n, 64, replace=False)
+    base = np.concatenate([x0[:, idx], us[:, idx]]).T                 # (64, 104)
+    h = 1e-6
+    pert = np.repeat(base, 2 * 104, axis=0)
+    for j in range(104):
+        pert[j::2 * 104, j] += h
+        pert[104 + j::2 * 104, j] -= h
+    (vals,) = O.map_eval(f, pert.shape[0], [pert[:, :4], pert[:, 4:]])
+    vals = vals.reshape(64, 2, 104)
+    fd = (vals[:, 0, :] - vals[:, 1, :]) / (2 * h)
+    gs = g[idx]
+    fd_err = np.max(np.abs(fd - gs) / np.maximum(np.max(np.abs(gs), axis=1, keepdims=True), 1.0))
+    print(f"[rollout] central FD (h=1e-6, 64 instances) max scaled err {fd_err:.2e}", flush=True)
+    assert fd_err <= 1e-3  # FD truncation on a chaotic 5 s rollout: ~1e-4 at worst
     # split forward / reverse through caller-owned checkpoints
     outs, ck = tape.fwd_ckpt(_to_dev(x0, us), n, every)
     gx2, gu2 = tape.rev_ckpt(_to_dev(x0, us), _to_dev(seed), n, every, ck)
