@@ -92,7 +92,8 @@ __global__ void __launch_bounds__(64) ldl_kernel(LdlArgs A) {
     for (int k = A.lt_colind[c]; k < A.lt_colind[c + 1]; ++k) w[A.lt_row[k]] = 0;
   }
   int zero_pivots = 0;
-  for (int c = 0; c < n; ++c) zero_pivots += d[c] == 0;
+  // zero pivots, or a breakdown that already turned D non-finite
+  for (int c = 0; c < n; ++c) zero_pivots += !(d[c] != 0 && isfinite(d[c]));
   if (active && A.status) A.status[j] = zero_pivots;
   // solve: x = P' L'^-1 D^-1 L^-1 P b for every right-hand side
   for (int q = 0; q < A.nrhs; ++q) {
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
   }
   // casadi_qr_singular: number of |R_cc| < eps
   int nullity = 0;
-  for (int c = 0; c < ncol; ++c) nullity += fabs(r[A.r_colind[c + 1] - 1]) < A.eps;
+  for (int c = 0; c < ncol; ++c) nullity += !(fabs(r[A.r_colind[c + 1] - 1]) >= A.eps);
   if (active && A.status) A.status[j] = nullity;
   // casadi_qr_solve
   Col w = x;
