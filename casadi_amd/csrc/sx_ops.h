@@ -11,9 +11,14 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SXH_HD __host__ __device__ __forceinline__
+// Out-of-line: keeps the constants and temporaries of the transcendental
+// implementations out of the interpreter loops (otherwise hoisted there and
+// held in registers for the whole kernel).
+#define SXH_HD_NOINLINE static __host__ __device__ __attribute__((noinline))
 #else
 #include <cmath>
 #define SXH_HD inline
+#define SXH_HD_NOINLINE static __attribute__((noinline, unused))
 #endif
 
 namespace sxh {
@@ -124,8 +129,52 @@ SXH_HD double sxh_erfinv(double x) {
   return y;
 }
 
-// f = op(x, y).  Unary operations ignore y.
+SXH_HD_NOINLINE double op_fun_all(int op, double x, double y);
+SXH_HD_NOINLINE void op_der_all(int op, double x, double y, double f, double* d);
+
+// f = op(x, y).  Unary operations ignore y.  The IEEE-exact and comparison
+// operations are inline; transcendental ones go through op_fun_all.
 SXH_HD double op_fun(int op, double x, double y) {
+  switch (op) {
+    case OP_ADD: return x + y;
+    case OP_SUB: return x - y;
+    case OP_MUL: return x * y;
+    case OP_DIV: return x / y;
+    case OP_NEG: return -x;
+    case OP_SQ: return x * x;
+    case OP_TWICE: return 2. * x;
+    case OP_SQRT: return sqrt(x);
+    case OP_INV: return 1. / x;
+    case OP_FABS: return fabs(x);
+    case OP_LT: return x < y;
+    case OP_LE: return x <= y;
+    case OP_EQ: return x == y;
+    case OP_NE: return x != y;
+    case OP_IF_ELSE_ZERO: return sxh_if_else_zero(x, y);
+    case OP_FMIN: return fmin(x, y);
+    case OP_FMAX: return fmax(x, y);
+    default: return op_fun_all(op, x, y);
+  }
+}
+
+// Partial derivatives, split the same way.
+SXH_HD void op_der(int op, double x, double y, double f, double* d) {
+  switch (op) {
+    case OP_ADD: d[0] = 1; d[1] = 1; return;
+    case OP_SUB: d[0] = 1; d[1] = -1; return;
+    case OP_MUL: d[0] = y; d[1] = x; return;
+    case OP_DIV: d[0] = 1 / y; d[1] = -f / y; return;
+    case OP_NEG: d[0] = -1; d[1] = 0; return;
+    case OP_SQ: d[0] = x + x; d[1] = 0; return;
+    case OP_TWICE: d[0] = 2; d[1] = 0; return;
+    case OP_SQRT: d[0] = 1 / (f + f); d[1] = 0; return;
+    case OP_INV: d[0] = -f * f; d[1] = 0; return;
+    case OP_IF_ELSE_ZERO: d[0] = 0; d[1] = x; return;
+    default: op_der_all(op, x, y, f, d);
+  }
+}
+
+SXH_HD_NOINLINE double op_fun_all(int op, double x, double y) {
   switch (op) {
     case OP_ASSIGN: return x;
     case OP_ADD: return x + y;
@@ -183,7 +232,7 @@ SXH_HD double op_fun(int op, double x, double y) {
 }
 
 // Partial derivatives d[0] = df/dx, d[1] = df/dy given f = op(x, y).
-SXH_HD void op_der(int op, double x, double y, double f, double* d) {
+SXH_HD_NOINLINE void op_der_all(int op, double x, double y, double f, double* d) {
   const double pi = 3.14159265358979323846;
   d[1] = 0;
   switch (op) {

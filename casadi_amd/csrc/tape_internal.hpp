@@ -64,7 +64,16 @@ struct cxh_tape_s {
   sxh::DeviceBuffer scratch, trail, ckpt;
   int block = 64;
   int w_lds = 1;
+  int ipw = 64;  // instances per wavefront (8, 16, 32, 64)
 };
+
+namespace sxh {
+struct GridShape {
+  int bs, ipw, cols;
+  int64_t grid, slots;
+};
+GridShape grid_shape(const cxh_tape_s* t, int64_t n);
+}  // namespace sxh
 
 struct cxh_jac_s {
   cxh_tape_s* tape = nullptr;

@@ -106,15 +106,83 @@ __device__ __forceinline__ const double* io_addr(const IOArg& a, int j, int64_t 
 
 __device__ __forceinline__ double instr_const(const Instr& e) { return e.d; }
 
-// Load one instruction with a single 16-byte wave-uniform load.
-__device__ __forceinline__ Instr load_instr(const Instr* __restrict__ tape, int t) {
+// ---------------------------------------------------------------------------
+// Instruction streaming.  The tape is read 64 instructions at a time: lane j
+// of the wavefront loads instruction (chunk + j) into four VGPRs with one
+// coalesced 16-byte load (1 KB per wave), and instruction j is broadcast to
+// SGPRs with v_readlane.  The next chunk is in flight while the current one
+// executes, so instruction fetch is off the dependent-latency path and the op
+// dispatch is a scalar branch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+
+// Instances per wavefront.  With few instances the interpreter is bound by
+// the latency of its dependent LDS round trips, not by VALU throughput, so a
+// wave may carry only `ipw` (8, 16, 32 or 64) instances: lanes >= ipw mirror
+// lane (lane % ipw) -- same instance, same LDS column, identical writes -- and
+// store nothing.  Fewer instances per wave means proportionally more waves to
+// hide each other's latency.
+struct LaneMap {
+  int64_t slot;  // instance slot of this lane (index into per-slot scratch)
+  int64_t kk;    // instance read by this lane (clamped into range)
+  int col;       // LDS column of this lane within the block
+  int stride;    // LDS columns per block
+  bool active;   // this lane owns its instance and writes results
+};
+
+__device__ __forceinline__ LaneMap map_lane(int ipw, int64_t n) {
+  const int lane = lane_id(), wave = static_cast<int>(threadIdx.x >> 6);
+  const int waves = static_cast<int>(blockDim.x >> 6);
+  const int sub = lane & (ipw - 1);
+  LaneMap m;
+  m.slot = (static_cast<int64_t>(blockIdx.x) * waves + wave) * ipw + sub;
+  m.kk = m.slot < n ? m.slot : n - 1;
+  m.col = wave * ipw + sub;
+  m.stride = waves * ipw;
+  m.active = lane < ipw && m.slot < n;
+  return m;
+}
+
+__device__ __forceinline__ int4 fetch_chunk(const Instr* __restrict__ tape, int base, int end) {
+  const int t = base + lane_id();
+  return t < end ? reinterpret_cast<const int4*>(tape)[t] : make_int4(0, 0, 0, 0);
+}
+
+__device__ __forceinline__ Instr bcast(const int4& v, int j) {
   Instr e;
-  const int4 v = reinterpret_cast<const int4*>(tape)[t];
-  e.op = v.x;
-  e.i0 = v.y;
-  e.i1 = v.z;
-  e.i2 = v.w;
+  e.op = __builtin_amdgcn_readlane(v.x, j);
+  e.i0 = __builtin_amdgcn_readlane(v.y, j);
+  e.i1 = __builtin_amdgcn_readlane(v.z, j);
+  e.i2 = __builtin_amdgcn_readlane(v.w, j);
   return e;
+}
+
+// f(instr, t) for t = t0 .. t1-1
+template <class F>
+__device__ __forceinline__ void stream_fwd(const Instr* __restrict__ tape, int t0, int t1, F&& f) {
+  int4 next = fetch_chunk(tape, t0, t1);
+  for (int c = t0; c < t1; c += 64) {
+    const int4 cur = next;
+    if (c + 64 < t1) next = fetch_chunk(tape, c + 64, t1);
+    const int cnt = min(64, t1 - c);
+    for (int j = 0; j < cnt; ++j) f(bcast(cur, j), c + j);
+  }
+}
+
+// f(instr, t, j) for t = t1-1 down to t0, chunk-aligned from t0; pre(c0, c1)
+// runs before the instructions [c0, c1) of a chunk are visited.
+template <class P, class F>
+__device__ __forceinline__ void stream_rev(const Instr* __restrict__ tape, int t0, int t1, P&& pre, F&& f) {
+  if (t1 <= t0) return;
+  int c = t0 + ((t1 - t0 - 1) / 64) * 64;  // start of the last chunk
+  int4 next = fetch_chunk(tape, c, t1);
+  for (; c >= t0; c -= 64) {
+    const int4 cur = next;
+    if (c - 64 >= t0) next = fetch_chunk(tape, c - 64, c);
+    const int cnt = min(64, t1 - c);
+    pre(c, c + cnt);
+    for (int j = cnt - 1; j >= 0; --j) f(bcast(cur, j), c + j, j);
+  }
 }
 
 // Execute instruction e of the primal sweep on work vector W.
@@ -144,17 +212,15 @@ __device__ __forceinline__ void primal_step(const Instr& e, W& w, const IOTable&
 template <bool LDS>
 __global__ void __launch_bounds__(256)
 tape_fwd_kernel(const Instr* __restrict__ tape, int n_instr, int ws, const IOTable arg, const IOTable res,
-                int64_t n, double* __restrict__ wglob, int64_t wstride) {
+                int64_t n, double* __restrict__ wglob, int64_t wstride, int ipw) {
   extern __shared__ double lds[];
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool active = k < n;
-  const int64_t kk = active ? k : n - 1;  // inactive lanes re-read a valid instance
+  const LaneMap m = map_lane(ipw, n);
   if constexpr (LDS) {
-    WLds w{lds + threadIdx.x, static_cast<int>(blockDim.x)};
-    for (int t = 0; t < n_instr; ++t) primal_step(load_instr(tape, t), w, arg, res, kk, active);
+    WLds w{lds + m.col, m.stride};
+    stream_fwd(tape, 0, n_instr, [&](const Instr& e, int) { primal_step(e, w, arg, res, m.kk, m.active); });
   } else {
-    WGlb w{wglob + k, wstride};
-    for (int t = 0; t < n_instr; ++t) primal_step(load_instr(tape, t), w, arg, res, kk, active);
+    WGlb w{wglob + m.slot, wstride};
+    stream_fwd(tape, 0, n_instr, [&](const Instr& e, int) { primal_step(e, w, arg, res, m.kk, m.active); });
   }
 }
 
@@ -183,8 +249,9 @@ struct AdjArgs {
   double* trail;         // [seg_len][lanes]
   double* ckpt;          // [n_seg][ws][lanes]
   double* wglob;         // [2*ws][lanes] when the work vectors live in HBM
-  int64_t lanes;         // stride of the HBM buffers
+  int64_t lanes;         // stride of the HBM buffers (instance slots of the grid)
   int mode;              // 0 fwd+adj, 1 fwd + all checkpoints, 2 adj from checkpoints
+  int ipw;               // instances per wavefront
 };
 
 // lane: index into the per-lane scratch (trail); inst: instance for I/O
@@ -192,8 +259,7 @@ struct AdjArgs {
 template <class W>
 __device__ __forceinline__ void fwd_segment(const AdjArgs& A, W& w, int t0, int t1, int64_t lane,
                                             int64_t inst, bool active, bool record, bool write_res) {
-  for (int t = t0; t < t1; ++t) {
-    const Instr e = load_instr(A.tape, t);
+  stream_fwd(A.tape, t0, t1, [&](const Instr& e, int t) {
     if (record && e.op != OP_OUTPUT) A.trail[static_cast<int64_t>(t - t0) * A.lanes + lane] = w[e.i0];
     if (e.op == OP_OUTPUT) {
       if (write_res) {
@@ -203,52 +269,60 @@ __device__ __forceinline__ void fwd_segment(const AdjArgs& A, W& w, int t0, int 
     } else {
       primal_step(e, w, A.arg, A.res, inst, active);
     }
-  }
+  });
+}
+
+// tr: this lane's column of a 64-row LDS staging buffer for the trail values
+// of the chunk being reversed (loaded with 64 independent coalesced loads
+// before the chunk starts).
+template <class W>
+__device__ __forceinline__ void rev_segment(const AdjArgs& A, W& w, W& aw, WLds tr, int t0, int t1,
+                                            int64_t lane, int64_t k, bool active) {
+  stream_rev(
+      A.tape, t0, t1,
+      [&](int c0, int c1) {
+        const double* src = A.trail + static_cast<int64_t>(c0 - t0) * A.lanes + lane;
+        for (int j = 0; j < c1 - c0; ++j) tr[j] = src[static_cast<int64_t>(j) * A.lanes];
+      },
+      [&](const Instr& e, int, int j) {
+        if (e.op == OP_OUTPUT) {
+          const IOArg s = A.aseed.a[e.i0];
+          if (s.p) aw[e.i1] += *io_addr(s, e.i2, k);
+          return;
+        }
+        const double told = tr[j];
+        if (e.op == OP_INPUT) {
+          const IOArg s = A.asens.a[e.i1];
+          if (s.p && active) *const_cast<double*>(io_addr(s, e.i2, k)) = aw[e.i0];
+          aw[e.i0] = 0.0;
+          w[e.i0] = told;
+          return;
+        }
+        if (e.op == OP_CONST) {
+          aw[e.i0] = 0.0;
+          w[e.i0] = told;
+          return;
+        }
+        const double f = w[e.i0];
+        w[e.i0] = told;
+        const double x = w[e.i1], y = w[e.i2];
+        const double seed = aw[e.i0];
+        aw[e.i0] = 0.0;
+        double d[2];
+        op_der(e.op, x, y, f, d);
+        if (e.op == OP_IF_ELSE_ZERO) {
+          aw[e.i2] += sxh_if_else_zero(d[1], seed);
+        } else if (op_is_binary(e.op)) {
+          aw[e.i1] += d[0] * seed;
+          aw[e.i2] += d[1] * seed;
+        } else {
+          aw[e.i1] += d[0] * seed;
+        }
+      });
 }
 
 template <class W>
-__device__ __forceinline__ void rev_segment(const AdjArgs& A, W& w, W& aw, int t0, int t1, int64_t lane,
-                                            int64_t k, bool active) {
-  for (int t = t1 - 1; t >= t0; --t) {
-    const Instr e = load_instr(A.tape, t);
-    if (e.op == OP_OUTPUT) {
-      const IOArg s = A.aseed.a[e.i0];
-      if (s.p) aw[e.i1] += *io_addr(s, e.i2, k);
-      continue;
-    }
-    const double told = A.trail[static_cast<int64_t>(t - t0) * A.lanes + lane];
-    if (e.op == OP_INPUT) {
-      const IOArg s = A.asens.a[e.i1];
-      if (s.p && active) *const_cast<double*>(io_addr(s, e.i2, k)) = aw[e.i0];
-      aw[e.i0] = 0.0;
-      w[e.i0] = told;
-      continue;
-    }
-    if (e.op == OP_CONST) {
-      aw[e.i0] = 0.0;
-      w[e.i0] = told;
-      continue;
-    }
-    const double f = w[e.i0];
-    w[e.i0] = told;
-    const double x = w[e.i1], y = w[e.i2];
-    const double seed = aw[e.i0];
-    aw[e.i0] = 0.0;
-    double d[2];
-    op_der(e.op, x, y, f, d);
-    if (e.op == OP_IF_ELSE_ZERO) {
-      aw[e.i2] += sxh_if_else_zero(d[1], seed);
-    } else if (op_is_binary(e.op)) {
-      aw[e.i1] += d[0] * seed;
-      aw[e.i2] += d[1] * seed;
-    } else {
-      aw[e.i1] += d[0] * seed;
-    }
-  }
-}
-
-template <class W>
-__device__ void adj_body(const AdjArgs& A, W& w, W& aw, int64_t k, int64_t kk, bool active) {
+__device__ void adj_body(const AdjArgs& A, W& w, W& aw, WLds tr, int64_t k, int64_t kk, bool active) {
   const int ws = A.ws;
   for (int s = 0; s < ws; ++s) {
     w[s] = 0.0;
@@ -276,7 +350,7 @@ __device__ void adj_body(const AdjArgs& A, W& w, W& aw, int64_t k, int64_t kk, b
         for (int s = 0; s < ws; ++s) w[s] = c[static_cast<int64_t>(s) * A.lanes];
       fwd_segment(A, w, t0, t1, k, kk, active, true, false);
     }
-    rev_segment(A, w, aw, t0, t1, k, kk, active);
+    rev_segment(A, w, aw, tr, t0, t1, k, kk, active);
   }
   if (active) {
     for (int u = 0; u < A.n_unused; ++u) {
@@ -290,17 +364,17 @@ __device__ void adj_body(const AdjArgs& A, W& w, W& aw, int64_t k, int64_t kk, b
 template <bool LDS>
 __global__ void __launch_bounds__(256) tape_adj_kernel(AdjArgs A) {
   extern __shared__ double lds[];
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool active = k < A.n;
-  const int64_t kk = active ? k : A.n - 1;
+  const LaneMap m = map_lane(A.ipw, A.n);
+  // LDS: [trail chunk: 64 rows][w: ws rows][aw: ws rows], one column per slot
+  WLds tr{lds + m.col, m.stride};
   if constexpr (LDS) {
-    WLds w{lds + threadIdx.x, static_cast<int>(blockDim.x)};
-    WLds aw{lds + static_cast<int64_t>(A.ws) * blockDim.x + threadIdx.x, static_cast<int>(blockDim.x)};
-    adj_body(A, w, aw, k, kk, active);
+    WLds w{lds + 64 * m.stride + m.col, m.stride};
+    WLds aw{lds + (64 + static_cast<int64_t>(A.ws)) * m.stride + m.col, m.stride};
+    adj_body(A, w, aw, tr, m.slot, m.kk, m.active);
   } else {
-    WGlb w{A.wglob + k, A.lanes};
-    WGlb aw{A.wglob + static_cast<int64_t>(A.ws) * A.lanes + k, A.lanes};
-    adj_body(A, w, aw, k, kk, active);
+    WGlb w{A.wglob + m.slot, A.lanes};
+    WGlb aw{A.wglob + static_cast<int64_t>(A.ws) * A.lanes + m.slot, A.lanes};
+    adj_body(A, w, aw, tr, m.slot, m.kk, m.active);
   }
 }
 
@@ -322,13 +396,13 @@ struct JacArgs {
   int64_t n;
   double* wglob;          // [(1+ndir)*ws][lanes] when not in LDS
   int64_t lanes;
+  int ipw;
 };
 
 template <class W>
 __device__ void jac_body(const JacArgs& A, W& w, W& tw, int64_t k, int64_t kk, bool active) {
   const int nd = A.ndir;
-  for (int t = 0; t < A.n_instr; ++t) {
-    const Instr e = load_instr(A.tape, t);
+  stream_fwd(A.tape, 0, A.n_instr, [&](const Instr& e, int) {
     switch (e.op) {
       case OP_CONST:
         w[e.i0] = instr_const(e);
@@ -369,23 +443,21 @@ __device__ void jac_body(const JacArgs& A, W& w, W& tw, int64_t k, int64_t kk, b
         w[e.i0] = f;
       }
     }
-  }
+  });
 }
 
 template <bool LDS>
 __global__ void __launch_bounds__(256) tape_jac_kernel(JacArgs A) {
   extern __shared__ double lds[];
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool active = k < A.n;
-  const int64_t kk = active ? k : A.n - 1;
+  const LaneMap m = map_lane(A.ipw, A.n);
   if constexpr (LDS) {
-    WLds w{lds + threadIdx.x, static_cast<int>(blockDim.x)};
-    WLds tw{lds + static_cast<int64_t>(A.ws) * blockDim.x + threadIdx.x, static_cast<int>(blockDim.x)};
-    jac_body(A, w, tw, k, kk, active);
+    WLds w{lds + m.col, m.stride};
+    WLds tw{lds + static_cast<int64_t>(A.ws) * m.stride + m.col, m.stride};
+    jac_body(A, w, tw, m.slot, m.kk, m.active);
   } else {
-    WGlb w{A.wglob + k, A.lanes};
-    WGlb tw{A.wglob + static_cast<int64_t>(A.ws) * A.lanes + k, A.lanes};
-    jac_body(A, w, tw, k, kk, active);
+    WGlb w{A.wglob + m.slot, A.lanes};
+    WGlb tw{A.wglob + static_cast<int64_t>(A.ws) * A.lanes + m.slot, A.lanes};
+    jac_body(A, w, tw, m.slot, m.kk, m.active);
   }
 }
 
@@ -446,6 +518,18 @@ IOTable make_io(const double* const* p, const std::vector<int64_t>& nnz, int32_t
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// Launch geometry for n instances: `cols` instance slots per block (waves
+// per block times instances per wave), `slots` in the whole grid.
+GridShape grid_shape(const cxh_tape_s* t, int64_t n) {
+  GridShape g;
+  g.bs = t->block;
+  g.ipw = t->ipw;
+  g.cols = (g.bs / 64) * g.ipw;
+  g.grid = (n + g.cols - 1) / g.cols;
+  g.slots = g.grid * g.cols;
+  return g;
+}
+
 }  // namespace sxh
 
 extern "C" {
@@ -485,13 +569,17 @@ int cxh_tape_destroy(cxh_tape t) {
   return 0;
 }
 
-int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds) {
+int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw) {
   if (!t) return cxh_fail("null tape");
   if (block != -1) {
     if (block != 64 && block != 128 && block != 256) return cxh_fail("block must be 64, 128 or 256");
     t->block = block;
   }
   if (w_in_lds != -1) t->w_lds = w_in_lds;
+  if (ipw != -1) {
+    if (ipw != 8 && ipw != 16 && ipw != 32 && ipw != 64) return cxh_fail("ipw must be 8, 16, 32 or 64");
+    t->ipw = ipw;
+  }
   return 0;
 }
 
@@ -506,21 +594,19 @@ int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const*
   hipStream_t s = static_cast<hipStream_t>(stream);
   const IOTable d_arg = make_io(arg, t->nnz_in, layout, ld);
   const IOTable d_res = make_io(res, t->nnz_out, layout, ld);
-  const int bs = t->block;
-  const int64_t grid = (n + bs - 1) / bs;
-  const size_t lds = static_cast<size_t>(t->ws) * bs * sizeof(double);
+  const GridShape g = grid_shape(t, n);
+  const size_t lds = static_cast<size_t>(t->ws) * g.cols * sizeof(double);
   const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
   if (use_lds) {
     allow_lds(reinterpret_cast<const void*>(tape_fwd_kernel<true>), lds);
-    hipLaunchKernelGGL(tape_fwd_kernel<true>, dim3(grid), dim3(bs), lds, s, t->d_tape.as<Instr>(),
+    hipLaunchKernelGGL(tape_fwd_kernel<true>, dim3(g.grid), dim3(g.bs), lds, s, t->d_tape.as<Instr>(),
                        static_cast<int>(t->alg.size()), static_cast<int>(t->ws), d_arg, d_res, n,
-                       nullptr, int64_t(0));
+                       nullptr, int64_t(0), g.ipw);
   } else {
-    const int64_t lanes = grid * bs;
-    t->scratch.reserve(static_cast<size_t>(t->ws) * lanes * sizeof(double));
-    hipLaunchKernelGGL(tape_fwd_kernel<false>, dim3(grid), dim3(bs), 0, s, t->d_tape.as<Instr>(),
+    t->scratch.reserve(static_cast<size_t>(t->ws) * g.slots * sizeof(double));
+    hipLaunchKernelGGL(tape_fwd_kernel<false>, dim3(g.grid), dim3(g.bs), 0, s, t->d_tape.as<Instr>(),
                        static_cast<int>(t->alg.size()), static_cast<int>(t->ws), d_arg, d_res, n,
-                       t->scratch.as<double>(), lanes);
+                       t->scratch.as<double>(), g.slots, g.ipw);
   }
   hip_check(hipGetLastError(), "tape_fwd_kernel launch");
   return 0;
@@ -595,9 +681,9 @@ static int launch_adj(cxh_tape t, int mode, int64_t n, const double* const* arg,
   A.n_unused = static_cast<int>(t->unused_in.size());
   A.n = n;
   A.mode = mode;
-  const int bs = t->block;
-  const int64_t grid = (n + bs - 1) / bs;
-  A.lanes = grid * bs;
+  const GridShape g = grid_shape(t, n);
+  A.ipw = g.ipw;
+  A.lanes = g.slots;
   if (mode != 1) t->trail.reserve(static_cast<size_t>(A.seg_len) * A.lanes * sizeof(double));
   A.trail = t->trail.as<double>();
   if (ckpt_ext) {
@@ -606,15 +692,17 @@ static int launch_adj(cxh_tape t, int mode, int64_t n, const double* const* arg,
     t->ckpt.reserve(static_cast<size_t>(A.n_seg) * A.ws * A.lanes * sizeof(double));
     A.ckpt = t->ckpt.as<double>();
   }
-  const size_t lds = 2 * static_cast<size_t>(A.ws) * bs * sizeof(double);
+  const size_t lds_tr = 64 * static_cast<size_t>(g.cols) * sizeof(double);  // trail chunk staging
+  const size_t lds = lds_tr + 2 * static_cast<size_t>(A.ws) * g.cols * sizeof(double);
   const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
   if (use_lds) {
     allow_lds(reinterpret_cast<const void*>(tape_adj_kernel<true>), lds);
-    hipLaunchKernelGGL(tape_adj_kernel<true>, dim3(grid), dim3(bs), lds, s, A);
+    hipLaunchKernelGGL(tape_adj_kernel<true>, dim3(g.grid), dim3(g.bs), lds, s, A);
   } else {
     t->scratch.reserve(2 * static_cast<size_t>(A.ws) * A.lanes * sizeof(double));
     A.wglob = t->scratch.as<double>();
-    hipLaunchKernelGGL(tape_adj_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
+    allow_lds(reinterpret_cast<const void*>(tape_adj_kernel<false>), lds_tr);
+    hipLaunchKernelGGL(tape_adj_kernel<false>, dim3(g.grid), dim3(g.bs), lds_tr, s, A);
   }
   hip_check(hipGetLastError(), "tape_adj_kernel launch");
   return 0;
@@ -638,9 +726,8 @@ int cxh_ckpt_size(cxh_tape t, int64_t n, int64_t checkpoint_every, int32_t block
   if (!t || !n_doubles) return cxh_fail("null argument");
   int seg_len, n_seg;
   segments(t, checkpoint_every, seg_len, n_seg);
-  const int bs = block > 0 ? block : t->block;
-  const int64_t lanes = (n + bs - 1) / bs * bs;
-  *n_doubles = static_cast<int64_t>(n_seg) * t->ws * lanes;
+  (void)block;
+  *n_doubles = static_cast<int64_t>(n_seg) * t->ws * grid_shape(t, n).slots;
   return 0;
   CXH_CATCH
 }
@@ -745,18 +832,18 @@ int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* r
     A.jac_s_inst = j->nnz_jac;
   }
   A.n = n;
-  const int bs = t->block;
-  const int64_t grid = (n + bs - 1) / bs;
-  A.lanes = grid * bs;
-  const size_t lds = static_cast<size_t>(1 + A.ndir) * A.ws * bs * sizeof(double);
+  const GridShape g = grid_shape(t, n);
+  A.ipw = g.ipw;
+  A.lanes = g.slots;
+  const size_t lds = static_cast<size_t>(1 + A.ndir) * A.ws * g.cols * sizeof(double);
   const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
   if (use_lds) {
     allow_lds(reinterpret_cast<const void*>(tape_jac_kernel<true>), lds);
-    hipLaunchKernelGGL(tape_jac_kernel<true>, dim3(grid), dim3(bs), lds, s, A);
+    hipLaunchKernelGGL(tape_jac_kernel<true>, dim3(g.grid), dim3(g.bs), lds, s, A);
   } else {
     t->scratch.reserve(static_cast<size_t>(1 + A.ndir) * A.ws * A.lanes * sizeof(double));
     A.wglob = t->scratch.as<double>();
-    hipLaunchKernelGGL(tape_jac_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
+    hipLaunchKernelGGL(tape_jac_kernel<false>, dim3(g.grid), dim3(g.bs), 0, s, A);
   }
   hip_check(hipGetLastError(), "tape_jac_kernel launch");
   return 0;

@@ -73,10 +73,14 @@ class HipTape:
             L.cxh_tape_destroy(h)
             self._h = None
 
-    def config(self, block=-1, w_in_lds=-1):
-        L.check(L.cxh_tape_config(self._h, int(block), int(w_in_lds) if w_in_lds != -1 else -1))
+    def config(self, block=-1, w_in_lds=-1, ipw=-1):
+        """block: threads per workgroup (64/128/256); w_in_lds: work vector in
+        LDS (True) or HBM (False); ipw: instances per wavefront (8/16/32/64)."""
+        L.check(L.cxh_tape_config(self._h, int(block), int(w_in_lds) if w_in_lds != -1 else -1, int(ipw)))
         if block != -1:
             self.block = int(block)
+        if ipw != -1:
+            self.ipw = int(ipw)
 
     def _alloc(self, nnz, n, like):
         torch = _torch()

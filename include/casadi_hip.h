@@ -63,7 +63,7 @@ int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize,
 int cxh_tape_destroy(cxh_tape t);
 /* Kernel configuration: threads per block (64/128/256) and whether the work
  * vector lives in LDS (1) or in HBM scratch (0); -1 keeps the current value. */
-int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds);
+int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw);
 
 /* Batched forward evaluation of one tape over n instances, device pointers.
  * Replaces: Map::eval / Map::eval_gen (casadi/core/map.cpp:141-157, 327-334)
