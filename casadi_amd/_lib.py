@@ -8,6 +8,9 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcasadi_hip.so")
+# Code objects of the specialised kernels are cached by hiprtc (comgr) on disk;
+# keep that cache beside the package unless the caller chose a location.
+os.environ.setdefault("AMD_COMGR_CACHE_DIR", os.path.join(os.path.dirname(_HERE), "_build", "comgr"))
 
 
 class CasadiHipError(RuntimeError):
@@ -68,6 +71,11 @@ cxh_jac_create = _fn("cxh_jac_create", I32, P, I32, I64, PI64, PI64, I64, PI64, 
 cxh_jac_destroy = _fn("cxh_jac_destroy", I32, P)
 cxh_map_jac = _fn("cxh_map_jac", I32, P, I64, PP, PP, P, I32, I64, P)
 cxh_device_info = _fn("cxh_device_info", I32, C.POINTER(I32), C.POINTER(I32), C.c_char_p, I32)
+cxh_tape_engine = _fn("cxh_tape_engine", I32, P, I32)
+cxh_tape_jit_prepare = _fn("cxh_tape_jit_prepare", I32, P, I64, I32, I32)
+cxh_tape_jit_source = _fn("cxh_tape_jit_source", I32, P, I64, I32, I32, C.c_char_p, I64, PI64,
+                          C.POINTER(I32))
+cxh_jit_compile_check = _fn("cxh_jit_compile_check", I32, CSTR, CSTR)
 
 # optional groups (linear solvers, heterogeneous batches)
 def _opt(name, res, *args):

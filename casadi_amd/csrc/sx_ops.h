@@ -8,7 +8,17 @@
 // (casadi/core/calculus.hpp:598-1018, dispatch at 1302-1439).
 #pragma once
 
-#if defined(__HIPCC__)
+#if defined(__HIPCC_RTC__)
+// Run-time compiled straight-line kernels: device math comes with hiprtc.
+#define SXH_HD __device__ __forceinline__
+#define SXH_HD_NOINLINE static __device__ __attribute__((noinline))
+#ifndef NAN
+#define NAN __builtin_nan("")
+#endif
+#ifndef HUGE_VAL
+#define HUGE_VAL __builtin_huge_val()
+#endif
+#elif defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SXH_HD __host__ __device__ __forceinline__
 // Out-of-line: keeps the constants and temporaries of the transcendental
@@ -129,8 +139,14 @@ SXH_HD double sxh_erfinv(double x) {
   return y;
 }
 
-SXH_HD_NOINLINE double op_fun_all(int op, double x, double y);
-SXH_HD_NOINLINE void op_der_all(int op, double x, double y, double f, double* d);
+SXH_HD double op_fun_full(int op, double x, double y);
+SXH_HD void op_der_full(int op, double x, double y, double f, double* d);
+
+// Out-of-line entry points of the full switches (one copy per kernel).
+SXH_HD_NOINLINE double op_fun_all(int op, double x, double y) { return op_fun_full(op, x, y); }
+SXH_HD_NOINLINE void op_der_all(int op, double x, double y, double f, double* d) {
+  op_der_full(op, x, y, f, d);
+}
 
 // f = op(x, y).  Unary operations ignore y.  The IEEE-exact and comparison
 // operations are inline; transcendental ones go through op_fun_all.
@@ -174,7 +190,8 @@ SXH_HD void op_der(int op, double x, double y, double f, double* d) {
   }
 }
 
-SXH_HD_NOINLINE double op_fun_all(int op, double x, double y) {
+// Every operation.  Inlined with a constant op it folds to the one case.
+SXH_HD double op_fun_full(int op, double x, double y) {
   switch (op) {
     case OP_ASSIGN: return x;
     case OP_ADD: return x + y;
@@ -232,7 +249,7 @@ SXH_HD_NOINLINE double op_fun_all(int op, double x, double y) {
 }
 
 // Partial derivatives d[0] = df/dx, d[1] = df/dy given f = op(x, y).
-SXH_HD_NOINLINE void op_der_all(int op, double x, double y, double f, double* d) {
+SXH_HD void op_der_full(int op, double x, double y, double f, double* d) {
   const double pi = 3.14159265358979323846;
   d[1] = 0;
   switch (op) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -53,6 +54,8 @@ IOTable make_io(const double* const* p, const std::vector<int64_t>& nnz, int32_t
 void validate_tape(const std::vector<Instr>& a, int64_t ws, const std::vector<int64_t>& nnz_in,
                    const std::vector<int64_t>& nnz_out);
 
+struct JitTape;
+
 }  // namespace sxh
 
 struct cxh_tape_s {
@@ -60,11 +63,14 @@ struct cxh_tape_s {
   int64_t ws = 1;
   std::vector<int64_t> nnz_in, nnz_out;
   std::vector<int2> unused_in;
-  sxh::DeviceBuffer d_tape, d_unused;
+  sxh::DeviceBuffer d_tape, d_unused;  // uploaded at first interpreter launch
+  bool uploaded = false;
   sxh::DeviceBuffer scratch, trail, ckpt;
   int block = 64;
   int w_lds = 1;
   int ipw = 64;  // instances per wavefront (8, 16, 32, 64)
+  int engine = 0;  // CXH_ENGINE_INTERP or CXH_ENGINE_JIT
+  std::shared_ptr<sxh::JitTape> jit;  // specialised kernels (jit.hip)
 };
 
 namespace sxh {
@@ -73,6 +79,13 @@ struct GridShape {
   int64_t grid, slots;
 };
 GridShape grid_shape(const cxh_tape_s* t, int64_t n);
+const Instr* device_tape(cxh_tape_s* t);
+
+// Straight-line specialised kernels (jit.hip).
+int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);
+int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
+            const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);
+int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every);
 }  // namespace sxh
 
 struct cxh_jac_s {

@@ -518,6 +518,18 @@ IOTable make_io(const double* const* p, const std::vector<int64_t>& nnz, int32_t
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// The instruction stream (and the list of unread input nonzeros) go to the
+// device at the first interpreter launch, so that creating a tape, generating
+// specialised source from it and the host VM need no GPU.
+const Instr* device_tape(cxh_tape_s* t) {
+  if (!t->uploaded) {
+    t->d_tape.upload(t->alg.data(), t->alg.size() * sizeof(Instr));
+    t->d_unused.upload(t->unused_in.data(), t->unused_in.size() * sizeof(int2));
+    t->uploaded = true;
+  }
+  return t->d_tape.as<Instr>();
+}
+
 // Launch geometry for n instances: `cols` instance slots per block (waves
 // per block times instances per wave), `slots` in the whole grid.
 GridShape grid_shape(const cxh_tape_s* t, int64_t n) {
@@ -557,8 +569,6 @@ int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize, int32_
   for (int i = 0; i < n_in; ++i)
     for (int64_t j = 0; j < t->nnz_in[i]; ++j)
       if (!used[i][j]) t->unused_in.push_back({i, static_cast<int>(j)});
-  t->d_tape.upload(t->alg.data(), t->alg.size() * sizeof(Instr));
-  t->d_unused.upload(t->unused_in.data(), t->unused_in.size() * sizeof(int2));
   *out = t.release();
   return 0;
   CXH_CATCH
@@ -594,17 +604,18 @@ int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const*
   hipStream_t s = static_cast<hipStream_t>(stream);
   const IOTable d_arg = make_io(arg, t->nnz_in, layout, ld);
   const IOTable d_res = make_io(res, t->nnz_out, layout, ld);
+  if (t->engine == CXH_ENGINE_JIT) return jit_map_eval(t, n, d_arg, d_res, s);
   const GridShape g = grid_shape(t, n);
   const size_t lds = static_cast<size_t>(t->ws) * g.cols * sizeof(double);
   const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
   if (use_lds) {
     allow_lds(reinterpret_cast<const void*>(tape_fwd_kernel<true>), lds);
-    hipLaunchKernelGGL(tape_fwd_kernel<true>, dim3(g.grid), dim3(g.bs), lds, s, t->d_tape.as<Instr>(),
+    hipLaunchKernelGGL(tape_fwd_kernel<true>, dim3(g.grid), dim3(g.bs), lds, s, device_tape(t),
                        static_cast<int>(t->alg.size()), static_cast<int>(t->ws), d_arg, d_res, n,
                        nullptr, int64_t(0), g.ipw);
   } else {
     t->scratch.reserve(static_cast<size_t>(t->ws) * g.slots * sizeof(double));
-    hipLaunchKernelGGL(tape_fwd_kernel<false>, dim3(g.grid), dim3(g.bs), 0, s, t->d_tape.as<Instr>(),
+    hipLaunchKernelGGL(tape_fwd_kernel<false>, dim3(g.grid), dim3(g.bs), 0, s, device_tape(t),
                        static_cast<int>(t->alg.size()), static_cast<int>(t->ws), d_arg, d_res, n,
                        t->scratch.as<double>(), g.slots, g.ipw);
   }
@@ -668,8 +679,11 @@ static int launch_adj(cxh_tape t, int mode, int64_t n, const double* const* arg,
   if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
   if (n == 0 || t->alg.empty()) return 0;
+  if (t->engine == CXH_ENGINE_JIT)
+    return jit_adj(t, mode, n, make_io(arg, t->nnz_in, layout, ld), make_io(aseed, t->nnz_out, layout, ld),
+                   make_io(res, t->nnz_out, layout, ld), make_io(asens, t->nnz_in, layout, ld), every, ckpt_ext, s);
   AdjArgs A{};
-  A.tape = t->d_tape.as<Instr>();
+  A.tape = device_tape(t);
   A.n_instr = static_cast<int>(t->alg.size());
   A.ws = static_cast<int>(t->ws);
   segments(t, every, A.seg_len, A.n_seg);
@@ -724,9 +738,13 @@ int cxh_map_fwd_adj(cxh_tape t, int64_t n, const double* const* arg, const doubl
 int cxh_ckpt_size(cxh_tape t, int64_t n, int64_t checkpoint_every, int32_t block, int64_t* n_doubles) {
   CXH_TRY
   if (!t || !n_doubles) return cxh_fail("null argument");
+  (void)block;
+  if (t->engine == CXH_ENGINE_JIT) {
+    *n_doubles = t->alg.empty() ? 0 : jit_ckpt_rows(t, checkpoint_every) * round_up(n, 64);
+    return 0;
+  }
   int seg_len, n_seg;
   segments(t, checkpoint_every, seg_len, n_seg);
-  (void)block;
   *n_doubles = static_cast<int64_t>(n_seg) * t->ws * grid_shape(t, n).slots;
   return 0;
   CXH_CATCH
@@ -813,7 +831,7 @@ int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* r
   if (n == 0 || t->alg.empty()) return 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   JacArgs A{};
-  A.tape = t->d_tape.as<Instr>();
+  A.tape = device_tape(t);
   A.n_instr = static_cast<int>(t->alg.size());
   A.ws = static_cast<int>(t->ws);
   A.ndir = j->ndir;

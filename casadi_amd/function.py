@@ -285,6 +285,12 @@ class Function:
         return self._cache[key]
 
     # -- batching ---------------------------------------------------------------
+    def to_torch(self, checkpoint_every=0, engine="jit"):
+        """Batched GPU evaluation as a differentiable torch op whose backward
+        is the checkpointed reverse sweep (casadi_amd.autograd.TorchFunction)."""
+        from .autograd import TorchFunction
+        return TorchFunction(self, checkpoint_every, engine)
+
     def map(self, n, parallelization="serial", max_num_threads=None, opts=None):
         """Map::create (casadi/core/map.cpp:39-51)."""
         if parallelization not in PARALLELIZATIONS:

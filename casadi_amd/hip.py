@@ -13,6 +13,7 @@ import numpy as np
 from . import _lib as L
 
 CXH_SOA, CXH_AOS = 0, 1
+ENGINES = {"interp": 0, "jit": 1}
 
 
 def _torch():
@@ -53,7 +54,7 @@ def _check_soa(tensors, nnz, n, what):
 class HipTape:
     """A Function's tape resident on the GPU (cxh_tape)."""
 
-    def __init__(self, f, block=64, w_in_lds=True):
+    def __init__(self, f, block=64, w_in_lds=True, engine="interp"):
         self.f = f
         self.nnz_in = [f.nnz_in(i) for i in range(f.n_in())]
         self.nnz_out = [f.nnz_out(i) for i in range(f.n_out())]
@@ -66,6 +67,30 @@ class HipTape:
                                   C.byref(h)), f"cxh_tape_create({f.name()})")
         self._h = h
         self.config(block, w_in_lds)
+        self.set_engine(engine)
+
+    def set_engine(self, engine):
+        """'interp': tape interpreter kernels; 'jit': straight-line kernels
+        generated from the tape and compiled at first use (hiprtc)."""
+        if engine not in ENGINES:
+            raise ValueError(f"engine must be one of {sorted(ENGINES)}")
+        L.check(L.cxh_tape_engine(self._h, ENGINES[engine]), "cxh_tape_engine")
+        self.engine = engine
+
+    def jit_prepare(self, checkpoint_every=0, forward=True, reverse=True):
+        """Compile the specialised kernels now instead of at first launch."""
+        L.check(L.cxh_tape_jit_prepare(self._h, int(checkpoint_every), int(forward), int(reverse)),
+                "cxh_tape_jit_prepare")
+
+    def jit_source(self, checkpoint_every=0, segment=0, reverse=False):
+        """Generated HIP source of one segment kernel and the segment count."""
+        n, ns = L.I64(), L.I32()
+        L.check(L.cxh_tape_jit_source(self._h, int(checkpoint_every), int(segment), int(reverse), None, 0,
+                                      C.byref(n), C.byref(ns)), "cxh_tape_jit_source")
+        buf = C.create_string_buffer(n.value + 1)
+        L.check(L.cxh_tape_jit_source(self._h, int(checkpoint_every), int(segment), int(reverse), buf,
+                                      n.value + 1, C.byref(n), C.byref(ns)), "cxh_tape_jit_source")
+        return buf.value.decode(), ns.value
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -192,6 +217,11 @@ class HipJacobian:
         L.check(L.cxh_map_jac(self._h, n, _ptrs(args), _ptrs(outs), jac.data_ptr(), CXH_SOA, ld,
                               _stream_ptr(stream)), "cxh_map_jac")
         return outs, jac[: self.nnz_jac, :n]
+
+
+def jit_compile_check(src, arch="gfx950"):
+    """Compile generated source through hiprtc without a GPU (raises on error)."""
+    L.check(L.cxh_jit_compile_check(src.encode(), arch.encode()), "cxh_jit_compile_check")
 
 
 def device_info():

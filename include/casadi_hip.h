@@ -61,9 +61,36 @@ int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize,
                     int32_t n_in, const int64_t* nnz_in,
                     int32_t n_out, const int64_t* nnz_out, cxh_tape* out);
 int cxh_tape_destroy(cxh_tape t);
-/* Kernel configuration: threads per block (64/128/256) and whether the work
- * vector lives in LDS (1) or in HBM scratch (0); -1 keeps the current value. */
+/* Kernel configuration: threads per block (64/128/256), whether the
+ * interpreter's work vector lives in LDS (1) or in HBM scratch (0), and
+ * instances per wavefront of the interpreter (8/16/32/64; the 64/ipw lanes of
+ * an instance group share one wavefront's instruction stream); -1 keeps the
+ * current value.  Replaces nothing in the reference: launch tuning of the
+ * "hip" Map.                                                                */
 int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw);
+
+/* Execution engine of a tape for cxh_map_eval / cxh_map_fwd_adj /
+ * cxh_ckpt_size / cxh_map_fwd_ckpt / cxh_map_rev_ckpt:
+ *  CXH_ENGINE_INTERP: the tape interpreter kernels (no compilation);
+ *  CXH_ENGINE_JIT: straight-line kernels generated from the tape and compiled
+ *    at first use through hiprtc, one per checkpoint segment and direction
+ *    (the GPU counterpart of CasADi's C code generation of an SXFunction,
+ *    SXFunction::codegen_body, casadi/core/sx_function.cpp:344-434).
+ * Results agree between engines; checkpoint buffers do not carry over.      */
+#define CXH_ENGINE_INTERP 0
+#define CXH_ENGINE_JIT 1
+int cxh_tape_engine(cxh_tape t, int32_t engine);
+/* Compile (ahead of the first launch) the specialised kernels for a
+ * checkpoint interval: forward segments and/or reverse segments.            */
+int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, int32_t reverse);
+/* Introspection: generated HIP source of one segment kernel (reverse != 0:
+ * the recompute + reverse-sweep kernel) into buf (NUL-terminated, truncated
+ * to cap), its full length in *len and the number of segments.               */
+int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, int32_t reverse,
+                        char* buf, int64_t cap, int64_t* len, int32_t* n_segments);
+/* Compile a generated source for arch (e.g. "gfx950") without loading it;
+ * 0 when it compiles, the compiler log in cxh_last_error otherwise.        */
+int cxh_jit_compile_check(const char* src, const char* arch);
 
 /* Batched forward evaluation of one tape over n instances, device pointers.
  * Replaces: Map::eval / Map::eval_gen (casadi/core/map.cpp:141-157, 327-334)

@@ -1,0 +1,160 @@
+"""GPU parity of the straight-line specialised kernels (engine 'jit').
+
+The specialised kernels evaluate the same operations (op_fun_full /
+op_der_full) in the same order as the interpreter, so forward values and
+adjoints must agree bit for bit with the interpreter kernels, which are in
+turn held to the CPU oracle in test_gpu_tape.py.  Every test also checks the
+oracle directly at the 1e-13 bar.
+"""
+import numpy as np
+import pytest
+
+import oracle_ref as O
+from conftest import seed_of
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+@pytest.fixture(scope="module")
+def ca():
+    import casadi_amd
+    return casadi_amd
+
+
+def _dev(*arrs):
+    import torch
+    return [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda") for a in arrs]
+
+
+def _np(ts):
+    return [t.cpu().numpy() for t in ts]
+
+
+def _all_ops_function(ca):
+    """One output per operation, chained through the inputs so that every
+    partial derivative reaches an input."""
+    a = ca.SX.sym("a")      # in (0.2, 0.8)
+    b = ca.SX.sym("b")      # in (1.5, 3)
+    c = ca.SX.sym("c", 2)   # in (-2, 2)
+    c0, c1 = c[0], c[1]
+    e = [a + c0, a - c1, a * c0, c1 / b, -c0, ca.exp(c0), ca.log(b), b ** c1, ca.constpow(b, 2.5),
+         ca.sqrt(b), ca.sq(c0 + a), 2 * c1, ca.sin(c0 * b), ca.cos(c1 + a), ca.tan(a), ca.asin(a), ca.acos(a),
+         ca.atan(c0), c0 < c1, c0 <= a, c0.eq(c1), c0.ne(a), ca.logic_not(c0 < 0), ca.logic_and(c0, c1),
+         ca.logic_or(c0 < 0, c1), ca.floor(c0 * 3), ca.ceil(c1 * 3), ca.fmod(b * 5, a), ca.fabs(c0),
+         ca.sign(c1), ca.copysign(b, c0), ca.if_else_zero(c0 < c1, b * c1), ca.erf(c0), ca.fmin(c0, c1),
+         ca.fmax(a, c1), 1 / b, ca.sinh(c0), ca.cosh(c1), ca.tanh(c0 * 2), ca.asinh(c1), ca.acosh(b),
+         ca.atanh(a), ca.atan2(c0, b), ca.erfinv(a - 0.5), ca.log1p(a), ca.expm1(c1), ca.hypot(c0, b)]
+    return ca.Function("all_ops", [a, b, c], [ca.vertcat(*e)]), len(e)
+
+
+def _all_ops_inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(0.2, 0.8, (1, n)), rng.uniform(1.5, 3, (1, n)), rng.uniform(-2, 2, (2, n)))
+
+
+def test_jit_all_operations_match_interpreter(ca, cuda):
+    from casadi_amd.hip import HipTape
+    f, m = _all_ops_function(ca)
+    n = 3000
+    a, b, c = _all_ops_inputs(n, seed_of(10))
+    seed = np.random.default_rng(seed_of(11)).standard_normal((m, n))
+    print(f"[case] jit all-ops n={n} ops={m}", flush=True)
+    ti, tj = HipTape(f), HipTape(f, engine="jit")
+    (yi,) = _np(ti.eval(_dev(a, b, c)))
+    (yj,) = _np(tj.eval(_dev(a, b, c)))
+    assert np.array_equal(yi, yj, equal_nan=True)
+    ref = O.map_eval(f, n, [a.T, b.T, c.T])[0].T
+    assert O.rel_err(yj, ref) <= TOL
+    (oi,), gi = ti.fwd_adj(_dev(a, b, c), _dev(seed))
+    (oj,), gj = tj.fwd_adj(_dev(a, b, c), _dev(seed))
+    assert np.array_equal(oi.cpu().numpy(), oj.cpu().numpy(), equal_nan=True)
+    for x, y in zip(_np(gi), _np(gj)):
+        assert np.array_equal(x, y, equal_nan=True)
+    R = f.reverse(1)
+    rr = O.map_eval(R, n, [a.T, b.T, c.T, None, seed.T])
+    for x, r in zip(_np(gj), rr):
+        assert O.rel_err(x, r.T) <= TOL
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 65536])
+def test_jit_cartpole_map(ca, cuda, n):
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_step()
+    rng = np.random.default_rng(seed_of(0) + n)
+    x, u = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (1, n))
+    print(f"[case] jit cartpole map n={n}", flush=True)
+    (yj,) = _np(HipTape(f, engine="jit").eval(_dev(x, u)))
+    (yi,) = _np(HipTape(f).eval(_dev(x, u)))
+    assert np.array_equal(yi, yj)
+    assert O.rel_err(yj, O.map_eval(f, n, [x.T, u.T])[0].T) <= TOL
+
+
+def test_jit_aos_layout_and_null_blocks(ca, cuda):
+    """CasADi Map layout (instance-major) and NULL arg / res / seed blocks."""
+    import torch
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape, CXH_AOS
+    from casadi_amd import _lib as L
+    f = W.cartpole_step()
+    n = 777
+    rng = np.random.default_rng(seed_of(12))
+    x, u = rng.uniform(-1, 1, (n, 4)), rng.uniform(-10, 10, (n, 1))
+    tj = HipTape(f, engine="jit")
+    X, U = (torch.tensor(v, dtype=torch.float64, device="cuda") for v in (x, u))
+    Y = torch.empty((n, 4), dtype=torch.float64, device="cuda")
+    L.check(L.cxh_map_eval(tj._h, n, L.ptr_array([X.data_ptr(), U.data_ptr()]), L.ptr_array([Y.data_ptr()]),
+                           CXH_AOS, n, torch.cuda.current_stream().cuda_stream))
+    assert O.rel_err(Y.cpu().numpy(), O.map_eval(f, n, [x, u])[0]) <= TOL
+    # u = NULL means zeros
+    Y0 = torch.empty((n, 4), dtype=torch.float64, device="cuda")
+    L.check(L.cxh_map_eval(tj._h, n, L.ptr_array([X.data_ptr(), None]), L.ptr_array([Y0.data_ptr()]),
+                           CXH_AOS, n, torch.cuda.current_stream().cuda_stream))
+    assert O.rel_err(Y0.cpu().numpy(), O.map_eval(f, n, [x, np.zeros((n, 1))])[0]) <= TOL
+    # adjoint without primal outputs
+    seed = np.random.default_rng(3).standard_normal((4, n))
+    outs, (ax, au) = tj.fwd_adj(_dev(x.T, u.T), _dev(seed), res=False)
+    R = f.reverse(1)
+    rx, ru = O.map_eval(R, n, [x, u, None, seed.T])
+    assert O.rel_err(ax.cpu().numpy(), rx.T) <= TOL and O.rel_err(au.cpu().numpy(), ru.T) <= TOL
+
+
+def test_jit_rollout_segments_match_interpreter(ca, cuda):
+    """Checkpointed fused adjoint, split forward/reverse and forward-only
+    evaluation of a 20-step rollout cut into 4 segments."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(20)
+    n = 2048
+    rng = np.random.default_rng(seed_of(2))
+    x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (20, n))
+    seed = np.ones((1, n))
+    every = (f.n_instructions() + 3) // 4
+    print(f"[case] jit rollout20 n={n} every={every}", flush=True)
+    ti, tj = HipTape(f), HipTape(f, engine="jit")
+    (Ji,), gi = ti.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
+    (Jj,), gj = tj.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
+    assert np.array_equal(Ji.cpu().numpy(), Jj.cpu().numpy())
+    for x, y in zip(_np(gi), _np(gj)):
+        assert np.array_equal(x, y)
+    # one segment gives the same bits
+    (J1,), g1 = tj.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=0)
+    for x, y in zip(_np(g1), _np(gj)):
+        assert np.array_equal(x, y)
+    # split through caller-owned (liveness-compacted) checkpoints
+    outs, ck = tj.fwd_ckpt(_dev(x0, us), n, every)
+    assert ck.numel() < ti.ckpt_size(n, every)
+    g2 = tj.rev_ckpt(_dev(x0, us), _dev(seed), n, every, ck)
+    assert np.array_equal(outs[0].cpu().numpy(), Jj.cpu().numpy())
+    for x, y in zip(_np(g2), _np(gj)):
+        assert np.array_equal(x, y)
+    # oracle
+    (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
+    g = np.concatenate(_np(gj)).T
+    go = np.concatenate([ox, ou], axis=1)
+    scale = np.maximum(np.max(np.abs(go), axis=1, keepdims=True), 1.0)
+    assert np.median(np.max(np.abs(g - go) / scale, axis=1)) <= TOL
+    # forward-only evaluation through the forward segment kernels
+    (Je,) = _np(tj.eval(_dev(x0, us)))
+    assert np.array_equal(Je, Jj.cpu().numpy())

@@ -1,0 +1,105 @@
+"""Host side of the tape specialiser: segment plans and generated source.
+
+No GPU is needed: tapes are created on the host, the generated HIP source is
+compiled for gfx950 through hiprtc, and the structure of the checkpoint plan
+is checked against the tape's own slot liveness computed here in numpy.
+"""
+import re
+
+import numpy as np
+import pytest
+
+from casadi_amd.ops import OP
+
+
+@pytest.fixture(scope="module")
+def ca():
+    import casadi_amd
+    return casadi_amd
+
+
+def _live_at(f, bounds):
+    """Slots live at each boundary (written before, read at or after)."""
+    a = f.instructions()
+    live = set()
+    out = {}
+    b = sorted(bounds, reverse=True)
+    if len(a) in b:
+        out[len(a)] = set()
+    for k in range(len(a) - 1, -1, -1):
+        op, i0, i1, i2 = (int(a[k][x]) for x in ("op", "i0", "i1", "i2"))
+        if op == OP.OUTPUT:
+            live.add(i1)
+        else:
+            live.discard(i0)
+            if op not in (OP.CONST, OP.INPUT):
+                live.add(i1)
+                if op in (OP.ADD, OP.SUB, OP.MUL, OP.DIV, OP.POW, OP.CONSTPOW, OP.LT, OP.LE, OP.EQ, OP.NE,
+                          OP.AND, OP.OR, OP.COPYSIGN, OP.FMOD, OP.REMAINDER, OP.FMIN, OP.FMAX, OP.ATAN2,
+                          OP.PRINTME, OP.LIFT, OP.HYPOT, OP.IF_ELSE_ZERO):
+                    live.add(i2)
+        if k in b:
+            out[k] = set(live)
+    return out
+
+
+def test_segment_state_is_the_live_set(ca):
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(5)
+    t = HipTape(f, engine="jit")
+    every = 200
+    ni = f.n_instructions()
+    nseg = (ni + every - 1) // every
+    live = _live_at(f, [s * every for s in range(nseg + 1)] + [ni])
+    for s in range(nseg):
+        src, ns = t.jit_source(every, s, reverse=False)
+        assert ns == nseg
+        loads = sorted(int(m) for m in re.findall(r"const double w(\d+) = CXJ_ST_IN", src))
+        stores = len(re.findall(r"CXJ_ST_OUT\(\d+\) =", src))
+        assert set(loads) == (live[s * every] if s > 0 else set())
+        assert stores == (len(live[min((s + 1) * every, ni)]) if s + 1 < nseg else 0)
+        rsrc, _ = t.jit_source(every, s, reverse=True)
+        assert len(re.findall(r"CXJ_ADJ_OUT\(\d+\) =", rsrc)) == (len(live[s * every]) if s > 0 else 0)
+    # checkpoint rows = sum of the interior live sets, far below nseg * worksize
+    ck = t.ckpt_size(64, every)
+    rows = sum(len(live[s * every]) for s in range(1, nseg))
+    assert ck == rows * 64
+    assert rows < (nseg - 1) * f.sz_w()
+
+
+def test_generated_sources_compile_for_gfx950(ca):
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_gpu_jit import _all_ops_function
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape, jit_compile_check
+    f, _ = _all_ops_function(ca)
+    t = HipTape(f, engine="jit")
+    for rev in (False, True):
+        src, ns = t.jit_source(0, 0, rev)
+        assert ns == 1
+        jit_compile_check(src)
+    g = W.cartpole_rollout(3)
+    t = HipTape(g, engine="jit")
+    src, ns = t.jit_source(150, 1, True)
+    assert ns == (g.n_instructions() + 149) // 150
+    jit_compile_check(src)
+
+
+def test_compile_errors_are_reported(ca):
+    from casadi_amd.hip import jit_compile_check
+    from casadi_amd import CasadiHipError
+    with pytest.raises(CasadiHipError, match="hiprtc compile failed"):
+        jit_compile_check("extern \"C\" __global__ void k() { undefined_symbol(); }")
+
+
+def test_engine_selection_is_validated(ca):
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    t = HipTape(W.cartpole_step())
+    with pytest.raises(ValueError):
+        t.set_engine("cuda")
+    t.set_engine("jit")
+    assert t.engine == "jit"
