@@ -91,10 +91,15 @@ cxh_ldl_factor_solve = _opt("cxh_ldl_factor_solve", I32, P, I64, P, P, I32, P, I
 cxh_qr_create = _opt("cxh_qr_create", I32, PI64, PI64, PI64, PI64, PI64, D, PP)
 cxh_qr_destroy = _opt("cxh_qr_destroy", I32, P)
 cxh_qr_factor_solve = _opt("cxh_qr_factor_solve", I32, P, I64, P, P, I32, I32, P, I32, P)
-cxh_batch_create = _opt("cxh_batch_create", I32, I32, PP, PI64, PP, PP, PP, PP, I32, P, PP)
+cxh_batch_create = _opt("cxh_batch_create", I32, I32, PP, PI64, PP)
+cxh_batch_bind = _opt("cxh_batch_bind", I32, P, I32, PP, PP, PP, PP, I32, I64)
+cxh_batch_config = _opt("cxh_batch_config", I32, P, I32, I32, I32)
+cxh_batch_tune = _opt("cxh_batch_tune", I32, P, P, I32, C.POINTER(I32), C.POINTER(D))
 cxh_batch_launch = _opt("cxh_batch_launch", I32, P, P)
-cxh_batch_launch_plain = _opt("cxh_batch_launch_plain", I32, P, P)
+cxh_batch_launch_direct = _opt("cxh_batch_launch_direct", I32, P, P)
+cxh_batch_info = _opt("cxh_batch_info", I32, P, C.POINTER(I32), PI64)
 cxh_batch_destroy = _opt("cxh_batch_destroy", I32, P)
+cxh_batch_source = _opt("cxh_batch_source", I32, I32, PP, C.c_char_p, I64, PI64)
 
 # --- host symbolic layer ------------------------------------------------------
 cxs_sym = _fn("cxs_sym", U64, CSTR)
@@ -127,7 +132,7 @@ cxs_function_call = _fn("cxs_function_call", I32, P, PP, PP)
 cxs_function_jacobian = _fn("cxs_function_jacobian", I32, P, PP)
 cxs_function_forward = _fn("cxs_function_forward", I32, P, I64, PP)
 cxs_function_reverse = _fn("cxs_function_reverse", I32, P, I64, PP)
-cxs_function_jac_plan = _fn("cxs_function_jac_plan", I32, P, I32, C.POINTER(I32), PP, PP, PP, PP, PP)
+cxs_function_jac_plan = _fn("cxs_function_jac_plan", I32, P, I32, I32, C.POINTER(I32), PP, PP, PP, PP, PP)
 cxs_function_jac_sparsity = _fn("cxs_function_jac_sparsity", I32, P, I32, I32, I32, I32, PP)
 cxs_jacobian = _fn("cxs_jacobian", I32, PI64, PU64, PI64, PU64, I32, PP, PP)
 cxs_gradient = _fn("cxs_gradient", I32, PI64, PU64, PI64, PU64, PP, PP)

@@ -243,10 +243,10 @@ int cxs_function_reverse(cxs_fun f, int64_t nadj, cxs_fun* out) {
   CXH_CATCH
 }
 
-int cxs_function_jac_plan(cxs_fun f, int32_t symmetric, int32_t* forward, cxs_buf* jsp,
+int cxs_function_jac_plan(cxs_fun f, int32_t symmetric, int32_t first_block, int32_t* forward, cxs_buf* jsp,
                           cxs_buf* seed_ptr, cxs_buf* seed_idx, cxs_buf* src_dir, cxs_buf* src_nz) {
   CXH_TRY
-  auto plan = f->f->jac_plan_all(symmetric != 0);
+  auto plan = first_block ? f->f->jac_plan(symmetric != 0) : f->f->jac_plan_all(symmetric != 0);
   *forward = plan.forward ? 1 : 0;
   *jsp = make_buf(plan.jsp.compressed());
   ivec ptr{0}, idx;

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -43,6 +44,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/casadi_hip.h"
@@ -85,6 +87,12 @@ struct CxjArgs {
 #define CXJ_ST_OUT(r) A.st_out[(long long)(r) * A.lds + k]
 #define CXJ_ADJ_IN(r) A.adj_in[(long long)(r) * A.lds + k]
 #define CXJ_ADJ_OUT(r) A.adj_out[(long long)(r) * A.lds + k]
+#define CXJ_SNAP(p, r) (p)[(long long)(r) * A.lds + k]
+#ifndef CXJ_KEEP
+#define CXJ_KEEP(x) asm volatile("" : "+v"(x))
+#define CXJ_KEEP_PTR(x) asm volatile("" : "+s"(x))
+#define CXJ_AFTER(p, x) asm volatile("" : "+s"(p) : "v"(x))
+#endif
 )CXJ";
 
 // ---------------------------------------------------------------- plans ---
@@ -98,13 +106,59 @@ struct JitPlan {
   std::vector<int64_t> ck_off;        // first checkpoint row of boundary s
   int64_t ck_rows = 0;
   int max_live = 0;
+  int64_t snap_rows = 0;  // sub-block boundary states of one reverse kernel (max over segments)
   std::vector<hipFunction_t> fwd, rev;
 };
 
 struct JitTape {
   std::map<int64_t, JitPlan> plans;  // keyed by segment length
-  DeviceBuffer st, adj, ck;
+  DeviceBuffer st, adj, ck, snap;
 };
+
+// Instructions per sub-block of the reverse kernels (0: whole segment).
+static int sub_block_len() {
+  if (const char* e = std::getenv("CXH_JIT_SUBBLOCK")) return std::max(0, std::atoi(e));
+  return 128;
+}
+
+// Slots live at each of the boundaries bnd (ascending, inside [t0, t1]),
+// given the live set at t1.
+static std::vector<std::vector<int>> live_sets(const cxh_tape_s* t, int t0, int t1, const std::vector<int>& live_end,
+                                               const std::vector<int>& bnd) {
+  std::vector<std::vector<int>> out(bnd.size());
+  std::vector<char> live(t->ws, 0);
+  for (int j : live_end) live[j] = 1;
+  int b = static_cast<int>(bnd.size()) - 1;
+  auto snap = [&](int i) {
+    for (int64_t j = 0; j < t->ws; ++j)
+      if (live[j]) out[i].push_back(static_cast<int>(j));
+  };
+  while (b >= 0 && bnd[b] == t1) snap(b--);
+  for (int k = t1 - 1; k >= t0 && b >= 0; --k) {
+    const Instr& e = t->alg[k];
+    if (e.op == OP_OUTPUT) {
+      live[e.i1] = 1;
+    } else {
+      live[e.i0] = 0;
+      if (op_is_arith(e.op)) {
+        live[e.i1] = 1;
+        if (op_is_binary(e.op)) live[e.i2] = 1;
+      }
+    }
+    while (b >= 0 && bnd[b] == k) snap(b--);
+  }
+  return out;
+}
+
+// Sub-block boundaries of the reverse kernel of [t0, t1).
+static std::vector<int> sub_bounds(int t0, int t1) {
+  const int sub = sub_block_len();
+  std::vector<int> c{t0};
+  if (sub > 0)
+    for (int x = t0 + sub; x < t1 - sub / 4; x += sub) c.push_back(x);
+  c.push_back(t1);
+  return c;
+}
 
 static int64_t default_every(int64_t n_instr) { return n_instr <= 4096 ? n_instr : 2048; }
 
@@ -149,6 +203,13 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
   }
   P.ck_rows = rows;
   for (auto& L : P.live) P.max_live = std::max<int>(P.max_live, static_cast<int>(L.size()));
+  for (int s = 0; s < P.n_seg; ++s) {
+    const std::vector<int> c = sub_bounds(P.t0[s], P.t0[s + 1]);
+    const auto lc = live_sets(t, P.t0[s], P.t0[s + 1], P.live[s + 1], c);
+    int64_t rows = 0;
+    for (size_t j = 1; j + 1 < c.size(); ++j) rows += static_cast<int64_t>(lc[j].size());
+    P.snap_rows = std::max(P.snap_rows, rows);
+  }
   P.fwd.assign(P.n_seg, nullptr);
   P.rev.assign(P.n_seg, nullptr);
   return t->jit->plans.emplace(every, std::move(P)).first->second;
@@ -164,126 +225,219 @@ static std::string lit(double d) {
   return b;
 }
 
-// Value names: instruction t's result is v<t>; the live-in value of slot j is
-// w<j>; their adjoints are a<t> / b<j>.
-struct Names {
-  std::vector<int64_t> cur;  // slot -> code (t >= 0, or -(j+1) for live-in)
-  static std::string val(int64_t c) { return c >= 0 ? "v" + std::to_string(c) : "w" + std::to_string(-c - 1); }
-  static std::string adj(int64_t c) { return c >= 0 ? "a" + std::to_string(c) : "b" + std::to_string(-c - 1); }
-};
+// Value codes: the live-in value of slot j is -(j+1); the value instruction t
+// produces in pass p is 4t+p (p = 0: first evaluation, p = 1: recomputation
+// inside the reverse kernel); the copy of slot j's value taken at the start of
+// recomputed sub-block b is 4(b*ws + j)+2.  Names: w<j>, v<t>, r<t>, h<b>_<j>;
+// adjoints b<j>, a<t>, q<t>, g<b>_<j>.
+static std::string val_name(int64_t c, int64_t ws) {
+  if (c < 0) return "w" + std::to_string(-c - 1);
+  const int64_t x = c >> 2;
+  switch (c & 3) {
+    case 0: return "v" + std::to_string(x);
+    case 1: return "r" + std::to_string(x);
+    default: return "h" + std::to_string(x / ws) + "_" + std::to_string(x % ws);
+  }
+}
+static std::string adj_name(int64_t c, int64_t ws) {
+  if (c < 0) return "b" + std::to_string(-c - 1);
+  const int64_t x = c >> 2;
+  switch (c & 3) {
+    case 0: return "a" + std::to_string(x);
+    case 1: return "q" + std::to_string(x);
+    default: return "g" + std::to_string(x / ws) + "_" + std::to_string(x % ws);
+  }
+}
 
-static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev) {
+// Segment s of the tape as a device function `name(A, k)` evaluating
+// instance k: the forward sweep (rev = false) or the recompute + reverse sweep.
+//
+// The reverse sweep needs every value of the segment that a partial derivative
+// reads; for long segments those exceed the register file and spill to
+// scratch.  The reverse kernel therefore cuts its segment into sub-blocks
+// c_0 < c_1 < ... < c_m: one forward pass keeps only the slots live at each
+// sub-block boundary (in registers), then for j = m-1 .. 0 sub-block j is
+// recomputed from its boundary state and swept backwards, handing the
+// adjoints of its live-in slots to sub-block j-1 (two-level checkpointing
+// inside the kernel; the HBM checkpoints stay at the segment boundaries).  The
+// boundary values enter each recomputation through an empty asm statement so
+// that the compiler cannot merge the recomputation with the first pass (which
+// would keep every first-pass value live again).
+static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, const std::string& name,
+                            bool sub_blocks = true) {
   const int t0 = P.t0[s], t1 = P.t0[s + 1];
+  const int len = t1 - t0;
+  const int64_t ws = t->ws;
   std::string o;
-  o.reserve(static_cast<size_t>(t1 - t0) * (rev ? 160 : 64) + 4096);
-  o += kSxOpsSource;
-  o += kPrologue;
-  o += "extern \"C\" __global__ void __launch_bounds__(256) cxj_kernel(const CxjArgs A) {\n";
-  o += "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n";
-  o += "  if (k >= A.n) return;\n";
-  Names N;
-  N.cur.assign(t->ws, 0);
+  o.reserve(static_cast<size_t>(len) * (rev ? 220 : 64) + 1024);
+  o += "__device__ __forceinline__ void " + name + "(const CxjArgs& A, const long long k) {\n";
+  std::vector<int64_t> cur(ws, 0);
   const std::vector<int>& lin = P.live[s];
   const std::vector<int>& lout = P.live[s + 1];
   for (size_t r = 0; r < lin.size(); ++r) {
     const int j = lin[r];
-    N.cur[j] = -(j + 1);
+    cur[j] = -(j + 1);
     o += "  const double w" + std::to_string(j) + " = ";
     o += s > 0 ? "CXJ_ST_IN(" + std::to_string(r) + ");\n" : "0.0;\n";
   }
-  // operand codes per instruction, for the reverse sweep
-  std::vector<int64_t> ox(t1 - t0, 0), oy(t1 - t0, 0);
+  auto V = [&](int64_t c) { return val_name(c, ws); };
+  auto Adj = [&](int64_t c) { return adj_name(c, ws); };
+  std::vector<int64_t> ox(len, 0), oy(len, 0), vc(len, 0);
   char buf[256];
-  for (int k = t0; k < t1; ++k) {
-    const Instr& e = t->alg[k];
-    switch (e.op) {
-      case OP_CONST:
-        o += "  const double v" + std::to_string(k) + " = " + lit(e.d) + ";\n";
-        N.cur[e.i0] = k;
-        break;
-      case OP_INPUT:
-        std::snprintf(buf, sizeof buf, "  const double v%d = CXJ_IN(%d, %d);\n", k, e.i1, e.i2);
-        o += buf;
-        N.cur[e.i0] = k;
-        break;
-      case OP_OUTPUT:
-        ox[k - t0] = N.cur[e.i1];
-        std::snprintf(buf, sizeof buf, "  CXJ_OUT(%d, %d, %s);\n", e.i0, e.i2, Names::val(N.cur[e.i1]).c_str());
-        o += buf;
-        break;
-      default: {
-        const int64_t x = N.cur[e.i1];
-        const int64_t y = op_is_binary(e.op) ? N.cur[e.i2] : x;
-        ox[k - t0] = x;
-        oy[k - t0] = y;
-        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, %s, %s);\n", k, e.op,
-                      Names::val(x).c_str(), Names::val(y).c_str());
-        o += buf;
-        N.cur[e.i0] = k;
+  auto emit_fwd = [&](int a, int b, int pass, bool outputs) {
+    for (int k = a; k < b; ++k) {
+      const Instr& e = t->alg[k];
+      const int64_t code = 4 * static_cast<int64_t>(k) + pass;
+      const std::string nm = V(code);
+      switch (e.op) {
+        case OP_CONST:
+          o += "  const double " + nm + " = " + lit(e.d) + ";\n";
+          cur[e.i0] = vc[k - t0] = code;
+          break;
+        case OP_INPUT:
+          std::snprintf(buf, sizeof buf, "  const double %s = CXJ_IN(%d, %d);\n", nm.c_str(), e.i1, e.i2);
+          o += buf;
+          cur[e.i0] = vc[k - t0] = code;
+          break;
+        case OP_OUTPUT:
+          ox[k - t0] = cur[e.i1];
+          if (outputs) {
+            std::snprintf(buf, sizeof buf, "  CXJ_OUT(%d, %d, %s);\n", e.i0, e.i2, V(cur[e.i1]).c_str());
+            o += buf;
+          }
+          break;
+        default: {
+          const int64_t x = cur[e.i1];
+          const int64_t y = op_is_binary(e.op) ? cur[e.i2] : x;
+          ox[k - t0] = x;
+          oy[k - t0] = y;
+          std::snprintf(buf, sizeof buf, "  const double %s = sxh::op_fun_full(%d, %s, %s);\n", nm.c_str(), e.op,
+                        V(x).c_str(), V(y).c_str());
+          o += buf;
+          cur[e.i0] = vc[k - t0] = code;
+        }
       }
     }
-  }
+  };
   if (!rev) {
+    emit_fwd(t0, t1, 0, true);
     if (s + 1 < P.n_seg)
       for (size_t r = 0; r < lout.size(); ++r)
-        o += "  CXJ_ST_OUT(" + std::to_string(r) + ") = " + Names::val(N.cur[lout[r]]) + ";\n";
+        o += "  CXJ_ST_OUT(" + std::to_string(r) + ") = " + V(cur[lout[r]]) + ";\n";
     o += "}\n";
     return o;
   }
-  // reverse sweep
-  std::vector<char> dv(t1 - t0, 0), dw(t->ws, 0);
-  auto declared = [&](int64_t c) -> char& { return c >= 0 ? dv[c - t0] : dw[-c - 1]; };
-  auto acc = [&](int64_t c, const std::string& term) {
-    char& d = declared(c);
-    if (d) {
-      o += "  " + Names::adj(c) + " += " + term + ";\n";
+  // sub-block boundaries and the live slots there
+  const std::vector<int> c = sub_blocks ? sub_bounds(t0, t1) : std::vector<int>{t0, t1};
+  const int m = static_cast<int>(c.size()) - 1;
+  const std::vector<std::vector<int>> lc = live_sets(t, t0, t1, lout, c);
+  // pass 1: forward over sub-blocks 0..m-2, storing the boundary states to the
+  // snapshot buffer (A.st_out, unused otherwise by reverse kernels)
+  std::vector<std::vector<int64_t>> snap(m + 1), in_code(m + 1), row(m + 1);
+  snap[0] = cur;
+  int64_t nrow = 0;
+  for (int j = 0; j + 1 < m; ++j) {
+    emit_fwd(c[j], c[j + 1], 0, true);
+    snap[j + 1] = cur;
+    row[j + 1].assign(ws, -1);
+    for (int slot : lc[j + 1]) {
+      row[j + 1][slot] = nrow;
+      o += "  CXJ_SNAP(A.st_out, " + std::to_string(nrow++) + ") = " + V(cur[slot]) + ";\n";
+    }
+  }
+  std::unordered_set<int64_t> decl;
+  auto acc = [&](int64_t code, const std::string& term) {
+    if (decl.count(code)) {
+      o += "  " + Adj(code) + " += " + term + ";\n";
     } else {
-      o += "  double " + Names::adj(c) + " = " + term + ";\n";
-      d = 1;
+      o += "  double " + Adj(code) + " = " + term + ";\n";
+      decl.insert(code);
     }
   };
-  if (s + 1 < P.n_seg)
-    for (size_t r = 0; r < lout.size(); ++r) {
-      const int64_t c = N.cur[lout[r]];
-      o += "  double " + Names::adj(c) + " = CXJ_ADJ_IN(" + std::to_string(r) + ");\n";
-      declared(c) = 1;
-    }
-  for (int k = t1 - 1; k >= t0; --k) {
-    const Instr& e = t->alg[k];
-    switch (e.op) {
-      case OP_CONST:
-        break;
-      case OP_OUTPUT:
-        std::snprintf(buf, sizeof buf, "CXJ_SEED(%d, %d)", e.i0, e.i2);
-        acc(ox[k - t0], buf);
-        break;
-      case OP_INPUT:
-        std::snprintf(buf, sizeof buf, "  CXJ_SENS(%d, %d, %s);\n", e.i1, e.i2,
-                      dv[k - t0] ? Names::adj(k).c_str() : "0.0");
-        o += buf;
-        break;
-      default: {
-        if (!dv[k - t0]) break;  // no contribution reaches this value
-        const std::string ks = std::to_string(k);
-        const std::string x = Names::val(ox[k - t0]), y = Names::val(oy[k - t0]);
-        o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + x + ", " + y +
-             ", v" + ks + ", d" + ks + ");\n";
-        if (e.op == OP_IF_ELSE_ZERO) {
-          acc(oy[k - t0], "sxh::sxh_if_else_zero(d" + ks + "[1], a" + ks + ")");
-        } else if (op_is_binary(e.op)) {
-          acc(ox[k - t0], "d" + ks + "[0] * a" + ks);
-          acc(oy[k - t0], "d" + ks + "[1] * a" + ks);
+  for (int j = m - 1; j >= 0; --j) {
+    const bool last = j == m - 1;
+    cur = snap[j];
+    if (!last && !lc[j].empty()) {
+      // reload the boundary state through an opaque pointer: neither the
+      // values nor the loads can be taken over from the first pass.  The
+      // pointer also takes the adjoints sub-block j+1 hands over as inputs, so
+      // that this recomputation cannot be scheduled before that reverse sweep
+      // (which would keep every recomputed value live at once).
+      const std::string sp = "cxj_sp" + std::to_string(j);
+      o += std::string("  const double* ") + sp + " = " + (j == 0 ? "A.st_in" : "A.st_out") + ";\n  CXJ_KEEP_PTR(" +
+           sp + ");\n";
+      for (int slot : lc[j + 1]) {
+        const int64_t from = in_code[j + 1][slot];
+        if (decl.count(from)) o += "  CXJ_AFTER(" + sp + ", " + Adj(from) + ");\n";
+      }
+      for (int slot : lc[j]) {
+        const int64_t h = 4 * (static_cast<int64_t>(j) * ws + slot) + 2;
+        if (j == 0 && s == 0) {
+          o += "  double " + V(h) + " = 0.0;\n  CXJ_KEEP(" + V(h) + ");\n";  // read before written
         } else {
-          acc(ox[k - t0], "d" + ks + "[0] * a" + ks);
+          const int64_t r = j == 0 ? std::find(lin.begin(), lin.end(), slot) - lin.begin() : row[j][slot];
+          o += "  double " + V(h) + " = CXJ_SNAP(" + sp + ", " + std::to_string(r) + ");\n";
+        }
+        cur[slot] = h;
+      }
+    }
+    in_code[j] = cur;
+    emit_fwd(c[j], c[j + 1], last ? 0 : 1, last);
+    if (last) {
+      if (s + 1 < P.n_seg)
+        for (size_t r = 0; r < lout.size(); ++r) {
+          const int64_t cd = cur[lout[r]];
+          o += "  double " + Adj(cd) + " = CXJ_ADJ_IN(" + std::to_string(r) + ");\n";
+          decl.insert(cd);
+        }
+    } else {
+      // adjoints of the slots live at c[j+1], from the names sub-block j+1
+      // read them by to the names this recomputation gives them
+      for (int slot : lc[j + 1]) {
+        const int64_t from = in_code[j + 1][slot], to = cur[slot];
+        if (from == to || !decl.count(from)) continue;
+        o += "  double " + Adj(to) + " = " + Adj(from) + ";\n";
+        decl.insert(to);
+      }
+    }
+    for (int k = c[j + 1] - 1; k >= c[j]; --k) {
+      const Instr& e = t->alg[k];
+      switch (e.op) {
+        case OP_CONST:
+          break;
+        case OP_OUTPUT:
+          std::snprintf(buf, sizeof buf, "CXJ_SEED(%d, %d)", e.i0, e.i2);
+          acc(ox[k - t0], buf);
+          break;
+        case OP_INPUT: {
+          const int64_t v = vc[k - t0];
+          std::snprintf(buf, sizeof buf, "  CXJ_SENS(%d, %d, %s);\n", e.i1, e.i2,
+                        decl.count(v) ? Adj(v).c_str() : "0.0");
+          o += buf;
+          break;
+        }
+        default: {
+          const int64_t v = vc[k - t0];
+          if (!decl.count(v)) break;  // no contribution reaches this value
+          const std::string ks = std::to_string(k), an = Adj(v);
+          o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + V(ox[k - t0]) +
+               ", " + V(oy[k - t0]) + ", " + V(v) + ", d" + ks + ");\n";
+          if (e.op == OP_IF_ELSE_ZERO) {
+            acc(oy[k - t0], "sxh::sxh_if_else_zero(d" + ks + "[1], " + an + ")");
+          } else if (op_is_binary(e.op)) {
+            acc(ox[k - t0], "d" + ks + "[0] * " + an);
+            acc(oy[k - t0], "d" + ks + "[1] * " + an);
+          } else {
+            acc(ox[k - t0], "d" + ks + "[0] * " + an);
+          }
         }
       }
     }
   }
   if (s > 0)
     for (size_t r = 0; r < lin.size(); ++r) {
-      const int64_t c = -(lin[r] + 1);
-      o += "  CXJ_ADJ_OUT(" + std::to_string(r) + ") = " + (dw[lin[r]] ? Names::adj(c) : std::string("0.0")) +
-           ";\n";
+      const int64_t cd = in_code[0][lin[r]];
+      o += "  CXJ_ADJ_OUT(" + std::to_string(r) + ") = " + (decl.count(cd) ? Adj(cd) : std::string("0.0")) + ";\n";
     }
   if (s == 0)
     for (const int2& u : t->unused_in) {
@@ -291,6 +445,19 @@ static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, boo
       o += buf;
     }
   o += "}\n";
+  return o;
+}
+
+// One segment kernel: one lane per instance, the arguments by value.
+static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev) {
+  std::string o = kSxOpsSource;
+  o += kPrologue;
+  o += gen_body(t, P, s, rev, "cxj_body");
+  o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") + (rev ? "cxj_rev" : "cxj_fwd") +
+       "(const CxjArgs A) {\n"
+       "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+       "  if (k < A.n) cxj_body(A, k);\n"
+       "}\n";
   return o;
 }
 
@@ -334,9 +501,10 @@ static std::vector<char> rtc_compile(const std::string& src, const std::string& 
 static std::mutex g_jit_mu;
 static std::unordered_map<std::string, hipFunction_t> g_jit_cache;
 
-static hipFunction_t jit_function(const std::string& src) {
+static hipFunction_t jit_function(const std::string& src, const char* name) {
   std::lock_guard<std::mutex> lk(g_jit_mu);
-  auto it = g_jit_cache.find(src);
+  const std::string key = std::string(name) + "\n" + src;
+  auto it = g_jit_cache.find(key);
   if (it != g_jit_cache.end()) return it->second;
   if (const char* dir = std::getenv("CXH_JIT_DUMP")) {
     const std::string path = std::string(dir) + "/cxj_" + std::to_string(g_jit_cache.size()) + ".hip";
@@ -349,14 +517,14 @@ static hipFunction_t jit_function(const std::string& src) {
   hipModule_t mod;
   hip_check(hipModuleLoadData(&mod, code.data()), "hipModuleLoadData");
   hipFunction_t fn;
-  hip_check(hipModuleGetFunction(&fn, mod, "cxj_kernel"), "hipModuleGetFunction");
-  g_jit_cache.emplace(src, fn);
+  hip_check(hipModuleGetFunction(&fn, mod, name), "hipModuleGetFunction");
+  g_jit_cache.emplace(key, fn);
   return fn;
 }
 
 static hipFunction_t segment_kernel(cxh_tape_s* t, JitPlan& P, int s, bool rev) {
   hipFunction_t& f = rev ? P.rev[s] : P.fwd[s];
-  if (!f) f = jit_function(gen_segment(t, P, s, rev));
+  if (!f) f = jit_function(gen_segment(t, P, s, rev), rev ? "cxj_rev" : "cxj_fwd");
   return f;
 }
 
@@ -427,13 +595,15 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
   if (mode == 1) return 0;
   if (P.n_seg > 1) t->jit->adj.reserve(2 * static_cast<size_t>(P.max_live) * lds * sizeof(double));
   double* adj = t->jit->adj.as<double>();
+  if (P.snap_rows) t->jit->snap.reserve(static_cast<size_t>(P.snap_rows) * lds * sizeof(double));
+  double* snap = P.snap_rows ? t->jit->snap.as<double>() : nullptr;
   A.aseed = aseed;
   A.asens = asens;
   for (int g = P.n_seg - 1; g >= 0; --g) {
     hipFunction_t f = segment_kernel(t, P, g, true);
     A.res = (mode == 0 && g == P.n_seg - 1) ? res : none;
     A.st_in = state(g);
-    A.st_out = nullptr;
+    A.st_out = snap;
     A.adj_in = adj ? adj + static_cast<int64_t>((g + 1) % 2) * P.max_live * lds : nullptr;
     A.adj_out = adj ? adj + static_cast<int64_t>(g % 2) * P.max_live * lds : nullptr;
     launch(f, n, t->block, A, s);
@@ -447,11 +617,306 @@ std::string jit_source(cxh_tape_s* t, int64_t every, int seg, bool rev) {
   return gen_segment(t, P, seg, rev);
 }
 
+// ----------------------------------------------------- heterogeneous batch ---
+
+// A tile is `block` consecutive instances of one Function.
+struct CxjTile {
+  long long start;
+  int fun;
+  int pad;
+};
+
+static const char kBatchKernel[] = R"CXJ(
+struct CxjTile { long long start; int fun; int pad; };
+)CXJ";
+
+// One kernel holding the fused forward + reverse body of every Function of the
+// batch.  Workgroups stride over the tile list (persistent when the grid is
+// smaller than the list); the Function of a tile is wave-uniform.
+static std::string gen_batch(const std::vector<cxh_tape_s*>& tapes) {
+  std::string o = kSxOpsSource;
+  o += kPrologue;
+  o += kBatchKernel;
+  for (size_t f = 0; f < tapes.size(); ++f) {
+    JitPlan& P = plan_for(tapes[f], 0);
+    if (P.n_seg != 1) throw std::runtime_error("batch: tape is not a single segment");
+    o += gen_body(tapes[f], P, 0, true, "cxj_f" + std::to_string(f), false);
+  }
+  o += "extern \"C\" __global__ void __launch_bounds__(256) cxj_batch(const CxjArgs* __restrict__ args, "
+       "const CxjTile* __restrict__ tiles, const long long n_tiles) {\n"
+       "  for (long long i = blockIdx.x; i < n_tiles; i += gridDim.x) {\n"
+       "    const CxjTile T = tiles[i];\n"
+       "    const long long k = T.start + threadIdx.x;\n"
+       "    switch (T.fun) {\n";
+  for (size_t f = 0; f < tapes.size(); ++f) {
+    const std::string fs = std::to_string(f);
+    o += "      case " + fs + ": { const CxjArgs& A = args[" + fs + "]; if (k < A.n) cxj_f" + fs + "(A, k); } break;\n";
+  }
+  o += "    }\n  }\n}\n";
+  return o;
+}
+
 }  // namespace sxh
 
 using namespace sxh;
 
+struct cxh_batch_s {
+  std::vector<cxh_tape_s*> tapes;
+  std::vector<int64_t> counts;
+  std::vector<CxjArgs> args;
+  std::vector<char> bound;
+  DeviceBuffer d_args, d_tiles;
+  hipFunction_t fn = nullptr;
+  int n_cu = 256;
+  int block = 256, grid_mult = 0, order = 0;  // grid_mult 0: one workgroup per tile
+  int64_t n_tiles = 0;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool dirty = true;
+  ~cxh_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace sxh {
+
+// Tile orders: 0 grouped by Function; 1 Functions interleaved in proportion
+// to their tile counts; 2 grouped, most expensive Function (tape length) first.
+static void build_tiles(cxh_batch_s* b) {
+  std::vector<CxjTile> tiles;
+  const int nf = static_cast<int>(b->tapes.size());
+  std::vector<int64_t> nt(nf);
+  for (int f = 0; f < nf; ++f) nt[f] = (b->counts[f] + b->block - 1) / b->block;
+  auto tile = [&](int f, int64_t i) { return CxjTile{static_cast<long long>(i * b->block), f, 0}; };
+  if (b->order == 1) {
+    std::vector<std::pair<double, CxjTile>> key;
+    for (int f = 0; f < nf; ++f)
+      for (int64_t i = 0; i < nt[f]; ++i) key.push_back({(i + 0.5) / static_cast<double>(nt[f]), tile(f, i)});
+    std::stable_sort(key.begin(), key.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    for (auto& kv : key) tiles.push_back(kv.second);
+  } else {
+    std::vector<int> fo(nf);
+    for (int f = 0; f < nf; ++f) fo[f] = f;
+    if (b->order == 2)
+      std::stable_sort(fo.begin(), fo.end(),
+                       [&](int x, int y) { return b->tapes[x]->alg.size() > b->tapes[y]->alg.size(); });
+    for (int f : fo)
+      for (int64_t i = 0; i < nt[f]; ++i) tiles.push_back(tile(f, i));
+  }
+  b->n_tiles = static_cast<int64_t>(tiles.size());
+  b->d_tiles.upload(tiles.data(), tiles.size() * sizeof(CxjTile));
+  b->dirty = true;
+}
+
+static void batch_launch_direct(cxh_batch_s* b, hipStream_t s) {
+  if (b->n_tiles == 0) return;
+  const int64_t g = b->grid_mult > 0 ? std::min<int64_t>(b->n_tiles, static_cast<int64_t>(b->grid_mult) * b->n_cu)
+                                     : b->n_tiles;
+  const CxjArgs* a = b->d_args.as<CxjArgs>();
+  const CxjTile* tl = b->d_tiles.as<CxjTile>();
+  long long nt = b->n_tiles;
+  void* params[] = {&a, &tl, &nt};
+  hip_check(hipModuleLaunchKernel(b->fn, static_cast<unsigned>(g), 1, 1, b->block, 1, 1, 0, s, params, nullptr),
+            "cxj_batch launch");
+}
+
+static void batch_check_bound(const cxh_batch_s* b) {
+  for (size_t f = 0; f < b->bound.size(); ++f)
+    if (!b->bound[f] && b->counts[f] > 0)
+      throw std::runtime_error("cxh_batch: Function " + std::to_string(f) + " has no bound buffers");
+}
+
+}  // namespace sxh
+
 extern "C" {
+
+int cxh_batch_create(int32_t n_fun, const cxh_tape* tapes, const int64_t* counts, cxh_batch* out) {
+  CXH_TRY
+  if (!out || !tapes || !counts || n_fun <= 0) return cxh_fail("cxh_batch_create: bad argument");
+  auto b = std::make_unique<cxh_batch_s>();
+  for (int f = 0; f < n_fun; ++f) {
+    if (!tapes[f]) return cxh_fail("cxh_batch_create: null tape");
+    if (counts[f] < 0) return cxh_fail("cxh_batch_create: negative count");
+    if (tapes[f]->alg.empty()) return cxh_fail("cxh_batch_create: empty tape");
+    b->tapes.push_back(tapes[f]);
+    b->counts.push_back(counts[f]);
+  }
+  b->args.assign(n_fun, CxjArgs{});
+  b->bound.assign(n_fun, 0);
+  b->fn = jit_function(gen_batch(b->tapes), "cxj_batch");
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+  hip_check(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  b->d_args.reserve(sizeof(CxjArgs) * n_fun);
+  build_tiles(b.get());
+  *out = b.release();
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_bind(cxh_batch b, int32_t fun, const double* const* arg, const double* const* adj_out,
+                   double* const* res, double* const* adj_in, int32_t layout, int64_t ld) {
+  CXH_TRY
+  if (!b || fun < 0 || fun >= static_cast<int32_t>(b->tapes.size())) return cxh_fail("cxh_batch_bind: bad Function");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  const cxh_tape_s* t = b->tapes[fun];
+  const int64_t n = b->counts[fun];
+  if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  CxjArgs& A = b->args[fun];
+  A = CxjArgs{};
+  A.arg = make_io(arg, t->nnz_in, layout, ld);
+  A.aseed = make_io(adj_out, t->nnz_out, layout, ld);
+  A.res = make_io(res, t->nnz_out, layout, ld);
+  A.asens = make_io(adj_in, t->nnz_in, layout, ld);
+  A.n = n;
+  A.lds = round_up(std::max<int64_t>(n, 1), 64);
+  hip_check(hipMemcpy(static_cast<char*>(b->d_args.ptr) + sizeof(CxjArgs) * fun, &A, sizeof(CxjArgs),
+                      hipMemcpyHostToDevice),
+            "cxh_batch_bind upload");
+  b->bound[fun] = 1;
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_config(cxh_batch b, int32_t block, int32_t grid_per_cu, int32_t order) {
+  CXH_TRY
+  if (!b) return cxh_fail("null batch");
+  if (block != -1) {
+    if (block != 64 && block != 128 && block != 256) return cxh_fail("block must be 64, 128 or 256");
+    b->block = block;
+  }
+  if (grid_per_cu != -1) {
+    if (grid_per_cu < 0 || grid_per_cu > 64) return cxh_fail("grid_per_cu must be in [0, 64]");
+    b->grid_mult = grid_per_cu;
+  }
+  if (order != -1) {
+    if (order < 0 || order > 2) return cxh_fail("order must be 0, 1 or 2");
+    b->order = order;
+  }
+  build_tiles(b);
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_tune(cxh_batch b, void* stream, int32_t reps, int32_t* best, double* best_ms) {
+  CXH_TRY
+  if (!b) return cxh_fail("null batch");
+  batch_check_bound(b);
+  (void)stream;
+  reps = std::max(reps, 1);
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  float best_t = 1e30f;
+  int bb = b->block, bg = b->grid_mult, bo = b->order;
+  const int blocks[] = {64, 128, 256};
+  const int grids[] = {0, 1, 2, 4, 8};
+  for (int blk : blocks)
+    for (int ord = 0; ord < 3; ++ord) {
+      b->block = blk;
+      b->order = ord;
+      build_tiles(b);
+      for (int gm : grids) {
+        b->grid_mult = gm;
+        batch_launch_direct(b, b->cap);  // warm-up
+        hip_check(hipEventRecord(e0, b->cap), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) batch_launch_direct(b, b->cap);
+        hip_check(hipEventRecord(e1, b->cap), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        ms /= reps;
+        if (ms < best_t) {
+          best_t = ms;
+          bb = blk;
+          bg = gm;
+          bo = ord;
+        }
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->block = bb;
+  b->grid_mult = bg;
+  b->order = bo;
+  build_tiles(b);
+  if (best) {
+    best[0] = bb;
+    best[1] = bg;
+    best[2] = bo;
+  }
+  if (best_ms) *best_ms = best_t;
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_launch(cxh_batch b, void* stream) {
+  CXH_TRY
+  if (!b) return cxh_fail("null batch");
+  batch_check_bound(b);
+  if (b->dirty) {
+    if (b->exec) hip_check(hipGraphExecDestroy(b->exec), "hipGraphExecDestroy");
+    if (b->graph) hip_check(hipGraphDestroy(b->graph), "hipGraphDestroy");
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hip_check(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    batch_launch_direct(b, b->cap);
+    hip_check(hipStreamEndCapture(b->cap, &b->graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    b->dirty = false;
+  }
+  hip_check(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_launch_direct(cxh_batch b, void* stream) {
+  CXH_TRY
+  if (!b) return cxh_fail("null batch");
+  batch_check_bound(b);
+  batch_launch_direct(b, static_cast<hipStream_t>(stream));
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_info(cxh_batch b, int32_t* config, int64_t* n_tiles) {
+  if (!b) return cxh_fail("null batch");
+  if (config) {
+    config[0] = b->block;
+    config[1] = b->grid_mult;
+    config[2] = b->order;
+  }
+  if (n_tiles) *n_tiles = b->n_tiles;
+  return 0;
+}
+
+int cxh_batch_destroy(cxh_batch b) {
+  delete b;
+  return 0;
+}
+
+int cxh_batch_source(int32_t n_fun, const cxh_tape* tapes, char* buf, int64_t cap, int64_t* len) {
+  CXH_TRY
+  if (!tapes || n_fun <= 0) return cxh_fail("cxh_batch_source: bad argument");
+  std::vector<cxh_tape_s*> v;
+  for (int f = 0; f < n_fun; ++f) {
+    if (!tapes[f] || tapes[f]->alg.empty()) return cxh_fail("cxh_batch_source: null or empty tape");
+    v.push_back(tapes[f]);
+  }
+  const std::string src = gen_batch(v);
+  if (len) *len = static_cast<int64_t>(src.size());
+  if (buf && cap > 0) {
+    const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, src.data(), m);
+    buf[m] = '\0';
+  }
+  return 0;
+  CXH_CATCH
+}
 
 int cxh_tape_engine(cxh_tape t, int32_t engine) {
   if (!t) return cxh_fail("null tape");

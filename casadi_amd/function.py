@@ -269,15 +269,17 @@ class Function:
         L.check(L.cxs_function_jac_sparsity(self._h, oind, iind, int(compact), int(symmetric), C.byref(h)))
         return Sparsity.from_compressed(L.take_buf(h))
 
-    def jac_plan(self, symmetric=False):
-        """Colouring + scatter plan of the compressed Jacobian of output 0 w.r.t.
-        input 0 (XFunction::jac bookkeeping): dict with jsp, forward, seeds
-        (CSR), src_dir, src_nz."""
-        key = ("plan", bool(symmetric))
+    def jac_plan(self, symmetric=False, first_block=False):
+        """Colouring + scatter plan of the compressed Jacobian (XFunction::jac
+        bookkeeping) of all outputs w.r.t. all inputs flattened, or with
+        ``first_block`` of output 0 w.r.t. input 0 only: dict with jsp,
+        forward, seeds (CSR), src_dir, src_nz."""
+        key = ("plan", bool(symmetric), bool(first_block))
         if key not in self._cache:
             fwd = L.I32()
             hs = [L.P() for _ in range(5)]
-            L.check(L.cxs_function_jac_plan(self._h, int(symmetric), C.byref(fwd), *[C.byref(h) for h in hs]))
+            L.check(L.cxs_function_jac_plan(self._h, int(symmetric), int(first_block), C.byref(fwd),
+                                            *[C.byref(h) for h in hs]))
             jsp, sptr, sidx, sdir, snz = [L.take_buf(h) for h in hs]
             self._cache[key] = dict(jsp=Sparsity.from_compressed(jsp), forward=bool(fwd.value),
                                     seed_ptr=sptr, seed_idx=sidx, src_dir=sdir, src_nz=snz,

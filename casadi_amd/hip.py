@@ -182,13 +182,15 @@ class HipTape:
 
 class HipJacobian:
     """Fused primal + coloured forward seed sweeps + scatter into the nonzeros of
-    the Jacobian of output 0 w.r.t. input 0 (XFunction::jac evaluated
-    numerically).  ``symmetric`` selects star colouring (Hessians)."""
+    the Jacobian (XFunction::jac evaluated numerically): of all outputs w.r.t.
+    all inputs flattened, or with ``first_block`` of output 0 w.r.t. input 0
+    (the other inputs are data).  ``symmetric`` selects star colouring
+    (Hessians: the Jacobian of a gradient)."""
 
-    def __init__(self, f, symmetric=False, tape=None):
+    def __init__(self, f, symmetric=False, tape=None, first_block=False):
         self.f = f
         self.tape = tape if tape is not None else HipTape(f)
-        self.plan = f.jac_plan(symmetric)
+        self.plan = f.jac_plan(symmetric, first_block)
         p = self.plan
         if not p["forward"]:
             raise NotImplementedError("adjoint-mode colourings run through HipTape.fwd_adj per direction")
@@ -217,6 +219,81 @@ class HipJacobian:
         L.check(L.cxh_map_jac(self._h, n, _ptrs(args), _ptrs(outs), jac.data_ptr(), CXH_SOA, ld,
                               _stream_ptr(stream)), "cxh_map_jac")
         return outs, jac[: self.nnz_jac, :n]
+
+
+class HipBatch:
+    """Heterogeneous batch: forward + adjoint of several Functions over their
+    own instance counts in ONE persistent kernel launch, replayed from a
+    captured hipGraph (cxh_batch).  ``tune()`` runs the autotuner over block
+    size, persistent grid and tile order."""
+
+    def __init__(self, functions, counts, tapes=None):
+        if len(functions) != len(counts):
+            raise ValueError("one count per Function")
+        self.functions = list(functions)
+        self.counts = [int(c) for c in counts]
+        self.tapes = tapes if tapes is not None else [HipTape(f) for f in self.functions]
+        h = L.P()
+        arr = L.ptr_array([t._h.value for t in self.tapes])
+        cnt = np.array(self.counts, np.int64)
+        L.check(L.cxh_batch_create(len(self.tapes), arr, cnt.ctypes.data_as(L.PI64), C.byref(h)),
+                "cxh_batch_create")
+        self._h = h
+        self._keep = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L.cxh_batch_destroy is not None:
+            L.cxh_batch_destroy(h)
+            self._h = None
+
+    def bind(self, fun, args, seeds, res=True, adj=True):
+        """Device buffers of Function ``fun``; allocates and returns
+        (outputs, adjoints) as lists of (nnz, count) tensors."""
+        t = self.tapes[fun]
+        n = self.counts[fun]
+        like = t._like(args)
+        ld = _check_soa(args, t.nnz_in, n, "arg")
+        _check_soa(seeds, t.nnz_out, n, "adj_out")
+        outs = t._alloc(t.nnz_out, n, like) if res else [None] * len(t.nnz_out)
+        adjs = t._alloc(t.nnz_in, n, like) if adj else [None] * len(t.nnz_in)
+        for blk in outs + adjs:
+            if blk is not None and blk.shape[0] > 1 and blk.stride(0) != ld:
+                raise ValueError("outputs must share the inputs' leading dimension")
+        L.check(L.cxh_batch_bind(self._h, fun, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adjs), CXH_SOA, ld),
+                "cxh_batch_bind")
+        self._keep[fun] = (args, seeds, outs, adjs)
+        return outs, adjs
+
+    def config(self, block=-1, grid_per_cu=-1, order=-1):
+        L.check(L.cxh_batch_config(self._h, int(block), int(grid_per_cu), int(order)), "cxh_batch_config")
+
+    def info(self):
+        cfg = (L.I32 * 3)()
+        nt = L.I64()
+        L.check(L.cxh_batch_info(self._h, cfg, C.byref(nt)))
+        return dict(block=cfg[0], grid_per_cu=cfg[1], order=cfg[2], n_tiles=nt.value)
+
+    def tune(self, reps=3, stream=None):
+        """Autotune the launch shape; returns (config dict, ms per launch)."""
+        best = (L.I32 * 3)()
+        ms = L.D()
+        L.check(L.cxh_batch_tune(self._h, _stream_ptr(stream), int(reps), best, C.byref(ms)), "cxh_batch_tune")
+        return dict(block=best[0], grid_per_cu=best[1], order=best[2]), ms.value
+
+    def launch(self, stream=None, graph=True):
+        fn = L.cxh_batch_launch if graph else L.cxh_batch_launch_direct
+        L.check(fn(self._h, _stream_ptr(stream)), "cxh_batch_launch")
+
+
+def batch_source(tapes):
+    """Generated source of the batch kernel over these HipTapes."""
+    arr = L.ptr_array([t._h.value for t in tapes])
+    n = L.I64()
+    L.check(L.cxh_batch_source(len(tapes), arr, None, 0, C.byref(n)), "cxh_batch_source")
+    buf = C.create_string_buffer(n.value + 1)
+    L.check(L.cxh_batch_source(len(tapes), arr, buf, n.value + 1, C.byref(n)), "cxh_batch_source")
+    return buf.value.decode()
 
 
 def jit_compile_check(src, arch="gfx950"):

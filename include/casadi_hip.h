@@ -150,20 +150,37 @@ int cxh_jac_destroy(cxh_jac j);
 int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* res,
                 double* jac, int32_t layout, int64_t ld, void* stream);
 
-/* Heterogeneous batch: instances of several tapes in one persistent launch,
- * captured once into a hipGraph and replayed.  For tape f: counts[f]
- * instances, SoA device arrays arg[f][i], adj_out[f][o] (may be NULL for a
- * forward-only batch), res[f][o], adj_in[f][i].
- * Replaces: one Map::eval per Function (casadi/core/map.cpp:141-157).       */
-int cxh_batch_create(int32_t n_fun, const cxh_tape* tapes, const int64_t* counts,
-                     const double* const* const* arg, const double* const* const* adj_out,
-                     double* const* const* res, double* const* const* adj_in,
-                     int32_t with_adjoint, void* stream, cxh_batch* out);
-/* Per-tape lanes-per-instance / tile choices, -1 = autotune.               */
+/* Heterogeneous batch: instances of several Functions, forward + adjoint each,
+ * in one persistent launch captured once into a hipGraph and replayed.
+ * Replaces: one Map::eval of f plus one of its reverse Function adj1_f per
+ *   Function (casadi/core/map.cpp:141-157; XFunction::get_reverse,
+ *   casadi/core/x_function.hpp:892-935).
+ * Every tape is compiled (specialised, single segment) into one kernel whose
+ * workgroups stride over a list of tiles (block consecutive instances of one
+ * Function).  counts[f]: instances of tape f.  The tapes must outlive the
+ * batch.                                                                     */
+int cxh_batch_create(int32_t n_fun, const cxh_tape* tapes, const int64_t* counts, cxh_batch* out);
+/* Device buffers of Function f (as cxh_map_fwd_adj): arg, adj_out (output
+ * cotangents; NULL blocks = 0), res and adj_in (NULL = not requested).       */
+int cxh_batch_bind(cxh_batch b, int32_t fun, const double* const* arg, const double* const* adj_out,
+                   double* const* res, double* const* adj_in, int32_t layout, int64_t ld);
+/* Launch shape: threads per block (64/128/256), persistent workgroups per
+ * CU (0 = one workgroup per tile), tile order (0 grouped by Function,
+ * 1 interleaved, 2 longest tape first); -1 keeps a value.                    */
+int cxh_batch_config(cxh_batch b, int32_t block, int32_t grid_per_cu, int32_t order);
+/* Autotuner: times every launch shape (reps launches each, hipEvents on an
+ * internal stream) and keeps the fastest; best[3] = {block, grid_per_cu,
+ * order}, *best_ms its time.  Runs the batch, so buffers must be bound.      */
+int cxh_batch_tune(cxh_batch b, void* stream, int32_t reps, int32_t* best, double* best_ms);
+/* Replay the captured graph on stream (captured again after a config change). */
 int cxh_batch_launch(cxh_batch b, void* stream);
+/* The same kernel launched without the graph. */
+int cxh_batch_launch_direct(cxh_batch b, void* stream);
+int cxh_batch_info(cxh_batch b, int32_t* config, int64_t* n_tiles);
 int cxh_batch_destroy(cxh_batch b);
-/* Launch every tape of the batch separately (one launch per Function). */
-int cxh_batch_launch_plain(cxh_batch b, void* stream);
+/* Introspection: the generated source of a batch kernel over these tapes
+ * (NUL-terminated into buf, truncated to cap; full length in *len).         */
+int cxh_batch_source(int32_t n_fun, const cxh_tape* tapes, char* buf, int64_t cap, int64_t* len);
 
 /* Batched sparse LDL^T on CasADi's symbolic factorisation.
  * sp_a, sp_lt: CasADi compact sparsity [nrow, ncol, colind..., row...] of A
@@ -251,8 +268,13 @@ int cxs_function_jacobian(cxs_fun f, cxs_fun* out);
 int cxs_function_forward(cxs_fun f, int64_t nfwd, cxs_fun* out);
 int cxs_function_reverse(cxs_fun f, int64_t nadj, cxs_fun* out);
 /* Jacobian / Hessian plan for the numeric sweeps: jsp (compact pattern),
- * direction seeds CSR and scatter sources; symmetric selects star colouring. */
-int cxs_function_jac_plan(cxs_fun f, int32_t symmetric, int32_t* forward, cxs_buf* jsp,
+ * direction seeds CSR and scatter sources; symmetric selects star colouring
+ * (XFunction::jac / get_partition, casadi/core/x_function.hpp:442-840,
+ * casadi/core/function_internal.cpp:2068-2165).  first_block != 0: the
+ * Jacobian of output 0 w.r.t. input 0 (the other inputs are data, as the
+ * free symbols of SX::jacobian), else of all outputs w.r.t. all inputs
+ * flattened (Function::jacobian).                                           */
+int cxs_function_jac_plan(cxs_fun f, int32_t symmetric, int32_t first_block, int32_t* forward, cxs_buf* jsp,
                           cxs_buf* seed_ptr, cxs_buf* seed_idx, cxs_buf* src_dir,
                           cxs_buf* src_nz);
 /* Structural Jacobian pattern of output oind w.r.t. input iind             */

@@ -103,3 +103,14 @@ def test_engine_selection_is_validated(ca):
         t.set_engine("cuda")
     t.set_engine("jit")
     assert t.engine == "jit"
+
+
+def test_batch_kernel_source_compiles(ca):
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape, batch_source, jit_compile_check
+    fs = [W.cartpole_step(), W.quadrotor_step(), W.arm_dynamics()]
+    src = batch_source([HipTape(f) for f in fs])
+    # one fused body per Function and a dispatch case for each
+    for i in range(len(fs)):
+        assert f"cxj_f{i}(A, k)" in src and f"void cxj_f{i}(const CxjArgs& A" in src
+    jit_compile_check(src)

@@ -1,0 +1,118 @@
+"""The specialiser's generated code, executed on the host.
+
+The segment bodies the specialiser emits are plain C++ apart from a few
+qualifiers, so they are compiled here with g++ (the device qualifiers defined
+away, the register barrier empty) and run instance by instance on the CPU,
+driven through the same segment sequence the GPU driver launches: forward
+segments storing the live state, then reverse segments from the last to the
+first handing adjoints of the live slots backwards.  This checks the code
+generation itself -- liveness at segment and sub-block boundaries,
+recomputation, the adjoint hand-offs -- without a GPU, against the oracle's
+reverse sweep (same host libm: bit-exact up to the sign of zero).
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle_ref as O
+
+MAX_IO = 16
+
+
+class IOArg(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("s_nz", C.c_longlong), ("s_inst", C.c_longlong)]
+
+
+class IOTable(C.Structure):
+    _fields_ = [("a", IOArg * MAX_IO)]
+
+
+class CxjArgs(C.Structure):
+    _fields_ = [("arg", IOTable), ("res", IOTable), ("aseed", IOTable), ("asens", IOTable),
+                ("st_in", C.c_void_p), ("st_out", C.c_void_p), ("adj_in", C.c_void_p), ("adj_out", C.c_void_p),
+                ("n", C.c_longlong), ("lds", C.c_longlong)]
+
+
+def _table(arrs, n):
+    t = IOTable()
+    for i, a in enumerate(arrs):
+        if a is not None:
+            t.a[i].p = a.ctypes.data
+            t.a[i].s_nz = n
+            t.a[i].s_inst = 1
+    return t
+
+
+def _compile(sources, workdir):
+    """One shared library exposing run_<i>(const CxjArgs*, n) per source."""
+    libs = []
+    for i, src in enumerate(sources):
+        cut = src.index('extern "C" __global__')
+        code = ("#define __device__\n#define __forceinline__ inline\n#define CXJ_KEEP(x) ((void)0)\n#define CXJ_KEEP_PTR(x) ((void)0)\n#define CXJ_AFTER(p, x) ((void)0)\n" + src[:cut] +
+                f'extern "C" void run_{i}(const CxjArgs* A, long long n) {{\n'
+                "  for (long long k = 0; k < n; ++k) cxj_body(*A, k);\n}\n")
+        path = os.path.join(workdir, f"seg{i}.cpp")
+        with open(path, "w") as fh:
+            fh.write(code)
+        so = os.path.join(workdir, f"seg{i}.so")
+        subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w", path, "-o", so],
+                       check=True)
+        lib = C.CDLL(so)
+        fn = getattr(lib, f"run_{i}")
+        fn.argtypes = [C.POINTER(CxjArgs), C.c_longlong]
+        fn.restype = None
+        libs.append((lib, fn))
+    return libs
+
+
+@pytest.mark.parametrize("sub", ["0", "40"])
+def test_generated_segments_run_on_host(monkeypatch, sub):
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    monkeypatch.setenv("CXH_JIT_SUBBLOCK", sub)
+    f = W.cartpole_rollout(4)
+    every = 170
+    tape = HipTape(f, engine="jit")
+    nseg = (f.n_instructions() + every - 1) // every
+    fwd = [tape.jit_source(every, s, False)[0] for s in range(nseg)]
+    rev = [tape.jit_source(every, s, True)[0] for s in range(nseg)]
+    if sub != "0":
+        assert any("CXJ_SNAP(" in r for r in rev)       # sub-blocks were generated
+    rows = [len(re.findall(r"CXJ_ST_IN\(", r)) for r in rev]  # live slots at each segment start
+    n = 37
+    rng = np.random.default_rng(4)
+    x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (4, n))
+    seed = rng.standard_normal((1, n))
+    with tempfile.TemporaryDirectory() as d:
+        libs = _compile(fwd[:-1] + rev, d)
+        fk, rk = [fn for _, fn in libs[:nseg - 1]], [fn for _, fn in libs[nseg - 1:]]
+        state = [np.zeros((max(r, 1), n)) for r in rows] + [np.zeros((1, n))]
+        J = np.zeros((1, n))
+        gx, gu = np.zeros((4, n)), np.zeros((4, n))
+        A = CxjArgs()
+        A.n, A.lds = n, n
+        A.arg = _table([x0, us], n)
+        A.res = _table([J], n)
+        for s in range(nseg - 1):
+            A.st_in = state[s].ctypes.data
+            A.st_out = state[s + 1].ctypes.data
+            fk[s](C.byref(A), n)
+        A.aseed = _table([seed], n)
+        A.asens = _table([gx, gu], n)
+        adj = [np.zeros((max(rows + [1]), n)) for _ in range(2)]
+        snap = np.zeros((4096, n))          # sub-block boundary states
+        A.st_out = snap.ctypes.data
+        for s in range(nseg - 1, -1, -1):
+            A.res = _table([J] if s == nseg - 1 else [None], n)
+            A.st_in = state[s].ctypes.data
+            A.adj_in = adj[(s + 1) % 2].ctypes.data
+            A.adj_out = adj[s % 2].ctypes.data
+            rk[s](C.byref(A), n)
+    (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
+    assert np.array_equal(J, Jo.T)
+    assert np.array_equal(gx, ox.T) and np.array_equal(gu, ou.T)
