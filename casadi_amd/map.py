@@ -2,7 +2,9 @@
 
 ``serial`` / ``openmp`` / ``thread`` run the host VM (Map::eval_gen,
 casadi/core/map.cpp:141-157 and OmpMap::eval, casadi/core/map.cpp:340-386);
-``hip`` runs the gfx950 tape interpreter.  Inputs follow Map's signature:
+``hip`` runs the gfx950 kernels: by default the straight-line kernels
+specialised from the tape (opts {"engine": "jit"}), or the tape interpreter
+(opts {"engine": "interp"}), which needs no compilation.  Inputs follow Map's signature:
 input i is ``repmat(sparsity_in(i), 1, n)`` -- for a column-vector input an
 ``(nrow, n)`` array -- and a non-repeated input is broadcast to all instances.
 """
@@ -38,7 +40,8 @@ class MapFunction:
         if self._tape is None:
             from .hip import HipTape
             self._tape = HipTape(self.f, block=self.opts.get("block", 64),
-                                 w_in_lds=self.opts.get("w_in_lds", True))
+                                 w_in_lds=self.opts.get("w_in_lds", True),
+                                 engine=self.opts.get("engine", "jit"))
         return self._tape
 
     def _aos(self, i, a):

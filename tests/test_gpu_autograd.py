@@ -46,25 +46,36 @@ def test_autograd_rollout_gradient(rollout, cuda):
     from casadi_amd.hip import HipTape
     (Ji,), (ix, iu) = HipTape(f).fwd_adj([X0.detach(), U.detach()], [w], n, checkpoint_every=every)
     assert np.array_equal(gx, ix.cpu().numpy()) and np.array_equal(gu, iu.cpu().numpy())
-    # CasADi reverse(1) on 64 sampled instances (CPU oracle VM); the rollout is
-    # chaotic, so errors are per instance scaled by its gradient norm, the
-    # median held to 1e-13 and the maximum to the oracle's 1-ulp envelope
+    # CasADi reverse(1) on 64 sampled instances (CPU oracle VM).  The rollout is
+    # chaotic: a 1-ulp difference in one sin/cos (OCML vs glibc) grows along
+    # the trajectory, so errors are per instance, scaled by its gradient norm.
+    # Bars: the median meets 1e-13; the maximum stays inside the rounding
+    # envelope -- how far a CPU evaluation moves when every transcendental
+    # result is nudged by one ulp (env for the numeric sweep, env_r for the
+    # symbolic reverse(1) tape).  GPU and CPU each round independently of the
+    # exact value, so their distance is bounded by the sum of two envelopes:
+    # 2 env against the same numeric sweep, env + env_r against reverse(1).
     idx = np.random.default_rng(5).choice(n, 64, replace=False)
     wv = w.cpu().numpy()[:, idx]
     R = f.reverse(1)
-    rx, ru = O.map_eval(R, 64, [x0[:, idx].T, us[:, idx].T, None, wv.T])
+    ins = [x0[:, idx].T, us[:, idx].T]
+    rx, ru = O.map_eval(R, 64, ins + [None, wv.T])
     gr = np.concatenate([rx, ru], axis=1)
     g = np.concatenate([gx[:, idx], gu[:, idx]]).T
     scale = np.maximum(np.max(np.abs(gr), axis=1, keepdims=True), 1.0)
-    per = np.max(np.abs(g - gr) / scale, axis=1)
-    (Jo,), (ox, ou) = O.adj_map(f, 64, [x0[:, idx].T, us[:, idx].T], [wv.T])
-    with O.perturbed_transcendentals():
-        (Jp,), (px, pu) = O.adj_map(f, 64, [x0[:, idx].T, us[:, idx].T], [wv.T])
+    (Jo,), (ox, ou) = O.adj_map(f, 64, ins, [wv.T])
     go = np.concatenate([ox, ou], axis=1)
+    with O.perturbed_transcendentals():
+        (Jp,), (px, pu) = O.adj_map(f, 64, ins, [wv.T])
+        qx, qu = O.map_eval(R, 64, ins + [None, wv.T])
     env = float(np.max(np.abs(np.concatenate([px, pu], axis=1) - go) / scale))
-    print(f"[autograd] vs reverse(1): median {np.median(per):.2e} max {per.max():.2e} envelope {env:.2e}",
-          flush=True)
-    assert np.median(per) <= TOL and per.max() <= max(TOL, env)
+    env_r = float(np.max(np.abs(np.concatenate([qx, qu], axis=1) - gr) / scale))
+    per_num = np.max(np.abs(g - go) / scale, axis=1)
+    per = np.max(np.abs(g - gr) / scale, axis=1)
+    print(f"[autograd] vs numeric sweep: median {np.median(per_num):.2e} max {per_num.max():.2e} (2 env {2 * env:.2e}); "
+          f"vs reverse(1): median {np.median(per):.2e} max {per.max():.2e} (env {env + env_r:.2e})", flush=True)
+    assert np.median(per_num) <= TOL and per_num.max() <= max(TOL, 2 * env)
+    assert np.median(per) <= TOL and per.max() <= max(TOL, env + env_r)
     # central finite differences of the weighted cost
     h = 1e-6
     base = np.concatenate([x0[:, idx], us[:, idx]]).T

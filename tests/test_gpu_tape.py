@@ -151,8 +151,12 @@ def test_rollout_adjoint_checkpointed(ca, cuda, rollout):
     # sin/cos (OCML vs glibc, both correct to < 1 ulp) grows by ~1e4 along the
     # trajectory.  Errors are therefore measured per instance, scaled by the
     # instance's gradient norm; the median must meet 1e-13 and the maximum
-    # must stay inside the oracle's own envelope: how far the CPU oracle moves
-    # when every transcendental result is nudged by one ulp.
+    # must stay inside the rounding envelope: how far a CPU evaluation moves
+    # when every transcendental result is nudged by one ulp (env for the
+    # numeric sweep, env_r for the symbolic reverse(1) tape).  GPU and CPU
+    # round independently of the exact value, so their distance is bounded by
+    # two envelopes: 2 env against the same sweep, env + env_r against
+    # reverse(1).
     (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
     go = np.concatenate([ox, ou], axis=1)
     scale = np.maximum(np.max(np.abs(go), axis=1, keepdims=True), 1.0)
@@ -168,10 +172,13 @@ def test_rollout_adjoint_checkpointed(ca, cuda, rollout):
     rx, ru = O.map_eval(R, n, [x0.T, us.T, None, seed.T])
     gr = np.concatenate([rx, ru], axis=1)
     per_sym = np.max(np.abs(g - gr) / scale, axis=1)
+    with O.perturbed_transcendentals():
+        qx, qu = O.map_eval(R, n, [x0.T, us.T, None, seed.T])
+    env_r = float(np.max(np.abs(np.concatenate([qx, qu], axis=1) - gr) / scale))
     print(f"[rollout] vs numeric sweep: max {err_num:.2e} median {med_num:.2e}; vs reverse(1) tape: "
           f"max {per_sym.max():.2e} median {np.median(per_sym):.2e}; 1-ulp envelope {env:.2e}", flush=True)
     assert med_num <= TOL and np.median(per_sym) <= TOL
-    assert err_num <= max(TOL, env) and per_sym.max() <= max(TOL, env)
+    assert err_num <= max(TOL, 2 * env) and per_sym.max() <= max(TOL, env + env_r)
     assert O.rel_err(J.cpu().numpy(), Jo.T) <= max(TOL, env_J)
     # 3) central finite differences (h = 1e-6) on 64 sampled instances
     idx = np.random.default_rng(5).choice(n, 64, replace=False)
