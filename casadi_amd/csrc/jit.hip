@@ -617,6 +617,237 @@ std::string jit_source(cxh_tape_s* t, int64_t every, int seg, bool rev) {
   return gen_segment(t, P, seg, rev);
 }
 
+// ------------------------------------------------------ tangent sweeps ---
+//
+// The compressed-Jacobian sweep of XFunction::jac
+// (casadi/core/x_function.hpp:442-840) as straight-line code: the primal and,
+// per seed direction, the tangent of every value (forward mode,
+// SXFunction::ad_forward numerically, casadi/core/sx_function.cpp:1194-1366),
+// with the tangents scattered into the Jacobian nonzeros at the OUTPUT
+// instructions.  Tangents that are structurally zero (no seeded input reaches
+// the value) are known at generation time and never computed, as CasADi's
+// symbolic sweep drops structural zeros.  Directions are processed in groups
+// (one set of kernels per group, each recomputing the primal) and long tapes in
+// segments whose state is the live values plus their nonzero tangents.
+
+struct JitJac {
+  int group = 1, n_group = 1, n_seg = 1;
+  std::vector<int> t0;                                         // segment boundaries
+  std::vector<std::vector<int>> live;                          // live slots per boundary
+  std::vector<std::vector<std::vector<std::pair<int, int>>>> tnz;  // [g][b]: nonzero (slot, dir)
+  int64_t max_rows = 0;
+  std::vector<std::vector<hipFunction_t>> fn;                  // [g][s]
+  DeviceBuffer st;
+};
+
+static int jac_dir_group() {
+  if (const char* e = std::getenv("CXH_JIT_DIRGROUP")) return std::max(1, std::atoi(e));
+  return 8;
+}
+
+static JitJac& jac_plan(cxh_jac_s* j) {
+  if (j->jit) return *j->jit;
+  const cxh_tape_s* t = j->tape;
+  auto J = std::make_shared<JitJac>();
+  const int nd = j->ndir;
+  const int64_t ni = static_cast<int64_t>(t->alg.size());
+  J->group = std::min(std::max(nd, 1), std::min(jac_dir_group(), 64));
+  J->n_group = std::max(1, (nd + J->group - 1) / J->group);
+  int64_t every = default_every(ni);
+  if (const char* e = std::getenv("CXH_JIT_TANSEG")) every = std::max(1, std::atoi(e));
+  every = std::max<int64_t>(1, std::min(every, ni));
+  J->n_seg = static_cast<int>((ni + every - 1) / every);
+  for (int s = 0; s <= J->n_seg; ++s) J->t0.push_back(static_cast<int>(std::min<int64_t>(ni, s * every)));
+  J->live = live_sets(t, 0, static_cast<int>(ni), {}, J->t0);
+  J->tnz.assign(J->n_group, std::vector<std::vector<std::pair<int, int>>>(J->n_seg + 1));
+  for (int g = 0; g < J->n_group; ++g) {
+    const int d0 = g * J->group, d1 = std::min(nd, d0 + J->group);
+    std::vector<uint64_t> mask(t->ws, 0);
+    int b = 1;
+    for (int64_t k = 0; k < ni; ++k) {
+      const Instr& e = t->alg[k];
+      switch (e.op) {
+        case OP_CONST: mask[e.i0] = 0; break;
+        case OP_INPUT: {
+          uint64_t m = 0;
+          const int64_t flat = j->in_off[e.i1] + e.i2;
+          for (int d = d0; d < d1; ++d)
+            if (j->seedval[flat * nd + d] != 0.0) m |= uint64_t(1) << (d - d0);
+          mask[e.i0] = m;
+        } break;
+        case OP_OUTPUT: break;
+        default:
+          mask[e.i0] = e.op == OP_IF_ELSE_ZERO ? mask[e.i2]
+                       : op_is_binary(e.op)    ? (mask[e.i1] | mask[e.i2])
+                                               : mask[e.i1];
+      }
+      while (b < J->n_seg && J->t0[b] == k + 1) {
+        for (int slot : J->live[b])
+          for (int d = d0; d < d1; ++d)
+            if (mask[slot] >> (d - d0) & 1) J->tnz[g][b].push_back({slot, d});
+        ++b;
+      }
+    }
+    for (int bb = 1; bb < J->n_seg; ++bb)
+      J->max_rows = std::max<int64_t>(J->max_rows, static_cast<int64_t>(J->live[bb].size() + J->tnz[g][bb].size()));
+  }
+  J->fn.assign(J->n_group, std::vector<hipFunction_t>(J->n_seg, nullptr));
+  j->jit = J;
+  return *J;
+}
+
+// Kernel of direction group g, segment s.  Block 0 of the asens table carries
+// the Jacobian nonzeros: nonzero q of instance k at p[q*s_nz + k*s_inst].
+static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s) {
+  const cxh_tape_s* t = j->tape;
+  const int nd = j->ndir;
+  const int d0 = g * J.group, d1 = std::min(nd, d0 + J.group), G = d1 - d0;
+  const int t0 = J.t0[s], t1 = J.t0[s + 1];
+  std::string o = kSxOpsSource;
+  o += kPrologue;
+  o.reserve(o.size() + static_cast<size_t>(t1 - t0) * (80 + 60 * G));
+  o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n";
+  std::vector<int64_t> cur(t->ws, 0);
+  std::vector<uint64_t> mask(t->ws, 0);
+  auto V = [&](int64_t c) { return c >= 0 ? "v" + std::to_string(c) : "w" + std::to_string(-c - 1); };
+  auto T = [&](int64_t c, int d) {
+    return (c >= 0 ? "u" + std::to_string(c) : "z" + std::to_string(-c - 1)) + "_" + std::to_string(d);
+  };
+  const std::vector<int>& lin = J.live[s];
+  for (size_t r = 0; r < lin.size(); ++r) {
+    cur[lin[r]] = -(lin[r] + 1);
+    o += "  const double w" + std::to_string(lin[r]) + " = " +
+         (s > 0 ? "CXJ_ST_IN(" + std::to_string(r) + ")" : std::string("0.0")) + ";\n";
+  }
+  if (s > 0) {
+    int64_t r = static_cast<int64_t>(lin.size());
+    for (const auto& sd : J.tnz[g][s]) {
+      mask[sd.first] |= uint64_t(1) << (sd.second - d0);
+      o += "  const double " + T(-(sd.first + 1), sd.second) + " = CXJ_ST_IN(" + std::to_string(r++) + ");\n";
+    }
+  }
+  char buf[256];
+  for (int k = t0; k < t1; ++k) {
+    const Instr& e = t->alg[k];
+    const std::string ks = std::to_string(k);
+    switch (e.op) {
+      case OP_CONST:
+        o += "  const double v" + ks + " = " + lit(e.d) + ";\n";
+        cur[e.i0] = k;
+        mask[e.i0] = 0;
+        break;
+      case OP_INPUT: {
+        std::snprintf(buf, sizeof buf, "  const double v%d = CXJ_IN(%d, %d);\n", k, e.i1, e.i2);
+        o += buf;
+        uint64_t m = 0;
+        const int64_t flat = j->in_off[e.i1] + e.i2;
+        for (int d = d0; d < d1; ++d) {
+          const double sv = j->seedval[flat * nd + d];
+          if (sv != 0.0) {
+            m |= uint64_t(1) << (d - d0);
+            o += "  const double " + T(k, d) + " = " + lit(sv) + ";\n";
+          }
+        }
+        cur[e.i0] = k;
+        mask[e.i0] = m;
+      } break;
+      case OP_OUTPUT: {
+        const int64_t c = cur[e.i1];
+        if (g == 0) {
+          std::snprintf(buf, sizeof buf, "  CXJ_OUT(%d, %d, %s);\n", e.i0, e.i2, V(c).c_str());
+          o += buf;
+        }
+        const int64_t flat = j->out_off[e.i0] + e.i2;
+        for (int d = d0; d < d1; ++d) {
+          const int2 tg = j->scatter[flat * nd + d];
+          const std::string v = (mask[e.i1] >> (d - d0) & 1) ? T(c, d) : std::string("0.0");
+          if (tg.x >= 0) o += "  CXJ_SENS(0, " + std::to_string(tg.x) + ", " + v + ");\n";
+          if (tg.y >= 0) o += "  CXJ_SENS(0, " + std::to_string(tg.y) + ", " + v + ");\n";
+        }
+      } break;
+      default: {
+        const int64_t x = cur[e.i1];
+        const bool bin = op_is_binary(e.op);
+        const int64_t y = bin ? cur[e.i2] : x;
+        const uint64_t mx = mask[e.i1], my = bin ? mask[e.i2] : 0;
+        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, %s, %s);\n", k, e.op,
+                      V(x).c_str(), V(y).c_str());
+        o += buf;
+        const uint64_t m = e.op == OP_IF_ELSE_ZERO ? my : (mx | my);
+        if (m) {
+          o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + V(x) + ", " + V(y) +
+               ", v" + ks + ", d" + ks + ");\n";
+          for (int d = d0; d < d1; ++d) {
+            const uint64_t bit = uint64_t(1) << (d - d0);
+            if (!(m & bit)) continue;
+            std::string rhs;
+            if (e.op == OP_IF_ELSE_ZERO) {
+              rhs = "sxh::sxh_if_else_zero(d" + ks + "[1], " + T(y, d) + ")";
+            } else if (bin && (mx & bit) && (my & bit)) {
+              rhs = "d" + ks + "[0] * " + T(x, d) + " + d" + ks + "[1] * " + T(y, d);
+            } else if (mx & bit) {
+              rhs = "d" + ks + "[0] * " + T(x, d);
+            } else {
+              rhs = "d" + ks + "[1] * " + T(y, d);
+            }
+            o += "  const double " + T(k, d) + " = " + rhs + ";\n";
+          }
+        }
+        cur[e.i0] = k;
+        mask[e.i0] = m;
+      }
+    }
+  }
+  if (s + 1 < J.n_seg) {
+    const std::vector<int>& lout = J.live[s + 1];
+    int64_t r = 0;
+    for (int slot : lout) o += "  CXJ_ST_OUT(" + std::to_string(r++) + ") = " + V(cur[slot]) + ";\n";
+    for (const auto& sd : J.tnz[g][s + 1]) {
+      const bool nzv = mask[sd.first] >> (sd.second - d0) & 1;
+      o += "  CXJ_ST_OUT(" + std::to_string(r++) + ") = " + (nzv ? T(cur[sd.first], sd.second) : "0.0") + ";\n";
+    }
+  }
+  o += "}\n"
+       "extern \"C\" __global__ void __launch_bounds__(256) cxj_tan(const CxjArgs A) {\n"
+       "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+       "  if (k < A.n) cxj_body(A, k);\n"
+       "}\n";
+  return o;
+}
+
+int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
+                int64_t jac_s_inst, hipStream_t s) {
+  JitJac& J = jac_plan(j);
+  const int64_t lds = round_up(n, 64);
+  if (J.n_seg > 1) J.st.reserve(2 * static_cast<size_t>(std::max<int64_t>(J.max_rows, 1)) * lds * sizeof(double));
+  double* st = J.st.as<double>();
+  CxjArgs A{};
+  A.arg = arg;
+  A.res = res;
+  A.asens.a[0].p = jac;
+  A.asens.a[0].s_nz = jac_s_nz;
+  A.asens.a[0].s_inst = jac_s_inst;
+  A.n = n;
+  A.lds = lds;
+  for (int g = 0; g < J.n_group; ++g)
+    for (int sg = 0; sg < J.n_seg; ++sg) {
+      hipFunction_t& f = J.fn[g][sg];
+      if (!f) f = jit_function(gen_tangent(j, J, g, sg), "cxj_tan");
+      A.st_in = st ? st + static_cast<int64_t>(sg % 2) * J.max_rows * lds : nullptr;
+      A.st_out = st ? st + static_cast<int64_t>((sg + 1) % 2) * J.max_rows * lds : nullptr;
+      launch(f, n, j->tape->block, A, s);
+    }
+  return 0;
+}
+
+std::string jit_tangent_source(cxh_jac_s* j, int g, int s, int* n_group, int* n_seg) {
+  JitJac& J = jac_plan(j);
+  if (n_group) *n_group = J.n_group;
+  if (n_seg) *n_seg = J.n_seg;
+  if (g < 0 || g >= J.n_group || s < 0 || s >= J.n_seg) throw std::runtime_error("group/segment out of range");
+  return gen_tangent(j, J, g, s);
+}
+
 // ----------------------------------------------------- heterogeneous batch ---
 
 // A tile is `block` consecutive instances of one Function.
@@ -944,6 +1175,24 @@ int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, i
     return 0;
   }
   const std::string src = jit_source(t, checkpoint_every, segment, reverse != 0);
+  if (len) *len = static_cast<int64_t>(src.size());
+  if (buf && cap > 0) {
+    const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, src.data(), m);
+    buf[m] = '\0';
+  }
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_jac_jit_source(cxh_jac j, int32_t group, int32_t segment, char* buf, int64_t cap, int64_t* len,
+                       int32_t* n_groups, int32_t* n_segments) {
+  CXH_TRY
+  if (!j) return cxh_fail("null jac plan");
+  int ng = 0, ns = 0;
+  const std::string src = jit_tangent_source(j, group, segment, &ng, &ns);
+  if (n_groups) *n_groups = ng;
+  if (n_segments) *n_segments = ns;
   if (len) *len = static_cast<int64_t>(src.size());
   if (buf && cap > 0) {
     const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));

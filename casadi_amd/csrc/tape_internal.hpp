@@ -88,10 +88,22 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
 int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every);
 }  // namespace sxh
 
+namespace sxh {
+struct JitJac;
+}
+
 struct cxh_jac_s {
   cxh_tape_s* tape = nullptr;
   int ndir = 0;
   int64_t nnz_jac = 0;
   std::vector<int> in_off, out_off;
+  std::vector<double> seedval;  // [flat_in_nz][ndir]
+  std::vector<int2> scatter;    // [flat_out_nz][ndir] -> up to two Jacobian nonzeros
   sxh::DeviceBuffer d_seed, d_scatter, d_in_off, d_out_off;
+  std::shared_ptr<sxh::JitJac> jit;  // specialised tangent kernels (jit.hip)
 };
+
+namespace sxh {
+int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
+                int64_t jac_s_inst, hipStream_t s);
+}

@@ -806,10 +806,8 @@ int cxh_jac_create(cxh_tape t, int32_t forward, int64_t ndir, const int64_t* see
     else if (tg.y < 0) tg.y = static_cast<int>(k);
     else throw std::runtime_error("cxh_jac_create: more than two targets per sensitivity");
   }
-  j->d_seed.upload(seedval.data(), seedval.size() * sizeof(double));
-  j->d_scatter.upload(scatter.data(), scatter.size() * sizeof(int2));
-  j->d_in_off.upload(j->in_off.data(), j->in_off.size() * sizeof(int));
-  j->d_out_off.upload(j->out_off.data(), j->out_off.size() * sizeof(int));
+  j->seedval = seedval;
+  j->scatter = scatter;
   *out = j.release();
   return 0;
   CXH_CATCH
@@ -830,6 +828,17 @@ int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* r
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
   if (n == 0 || t->alg.empty()) return 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t jac_s_nz = layout == CXH_SOA ? ld : 1;
+  const int64_t jac_s_inst = layout == CXH_SOA ? 1 : j->nnz_jac;
+  if (t->engine == CXH_ENGINE_JIT)
+    return jit_map_jac(j, n, make_io(arg, t->nnz_in, layout, ld), make_io(res, t->nnz_out, layout, ld), jac,
+                       jac_s_nz, jac_s_inst, s);
+  if (!j->d_seed.ptr) {  // device copies of the plan, at the first interpreter launch
+    j->d_seed.upload(j->seedval.data(), j->seedval.size() * sizeof(double));
+    j->d_scatter.upload(j->scatter.data(), j->scatter.size() * sizeof(int2));
+    j->d_in_off.upload(j->in_off.data(), j->in_off.size() * sizeof(int));
+    j->d_out_off.upload(j->out_off.data(), j->out_off.size() * sizeof(int));
+  }
   JacArgs A{};
   A.tape = device_tape(t);
   A.n_instr = static_cast<int>(t->alg.size());
@@ -842,13 +851,8 @@ int cxh_map_jac(cxh_jac j, int64_t n, const double* const* arg, double* const* r
   A.seedval = j->d_seed.as<double>();
   A.scatter = j->d_scatter.as<int2>();
   A.jac = jac;
-  if (layout == CXH_SOA) {
-    A.jac_s_nz = ld;
-    A.jac_s_inst = 1;
-  } else {
-    A.jac_s_nz = 1;
-    A.jac_s_inst = j->nnz_jac;
-  }
+  A.jac_s_nz = jac_s_nz;
+  A.jac_s_inst = jac_s_inst;
   A.n = n;
   const GridShape g = grid_shape(t, n);
   A.ipw = g.ipw;

@@ -187,9 +187,9 @@ class HipJacobian:
     (the other inputs are data).  ``symmetric`` selects star colouring
     (Hessians: the Jacobian of a gradient)."""
 
-    def __init__(self, f, symmetric=False, tape=None, first_block=False):
+    def __init__(self, f, symmetric=False, tape=None, first_block=False, engine="interp"):
         self.f = f
-        self.tape = tape if tape is not None else HipTape(f)
+        self.tape = tape if tape is not None else HipTape(f, engine=engine)
         self.plan = f.jac_plan(symmetric, first_block)
         p = self.plan
         if not p["forward"]:
@@ -208,6 +208,17 @@ class HipJacobian:
         if h is not None and L.cxh_jac_destroy is not None:
             L.cxh_jac_destroy(h)
             self._h = None
+
+    def jit_source(self, group=0, segment=0):
+        """Generated source of a specialised tangent kernel, with the number of
+        direction groups and segments."""
+        n, ng, ns = L.I64(), L.I32(), L.I32()
+        L.check(L.cxh_jac_jit_source(self._h, int(group), int(segment), None, 0, C.byref(n), C.byref(ng),
+                                     C.byref(ns)), "cxh_jac_jit_source")
+        buf = C.create_string_buffer(n.value + 1)
+        L.check(L.cxh_jac_jit_source(self._h, int(group), int(segment), buf, n.value + 1, C.byref(n), C.byref(ng),
+                                     C.byref(ns)), "cxh_jac_jit_source")
+        return buf.value.decode(), ng.value, ns.value
 
     def __call__(self, args, n=None, res=True, stream=None):
         torch = _torch()

@@ -158,3 +158,21 @@ def test_jit_rollout_segments_match_interpreter(ca, cuda):
     # forward-only evaluation through the forward segment kernels
     (Je,) = _np(tj.eval(_dev(x0, us)))
     assert np.array_equal(Je, Jj.cpu().numpy())
+
+
+def test_jit_jacobian_matches_interpreter(ca, cuda):
+    """Specialised tangent sweeps (5 forward colours) for the dense 4x5
+    Jacobian of the cart-pole step over 65536 instances."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipJacobian
+    f = W.cartpole_step()
+    n = 65536
+    rng = np.random.default_rng(seed_of(0))
+    x, u = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (1, n))
+    print(f"[case] jit jacobian n={n}", flush=True)
+    (yi,), ji = HipJacobian(f)(_dev(x, u))
+    (yj,), jj = HipJacobian(f, engine="jit")(_dev(x, u))
+    assert np.array_equal(yi.cpu().numpy(), yj.cpu().numpy())
+    assert np.array_equal(ji.cpu().numpy(), jj.cpu().numpy())
+    ref = np.concatenate(O.map_eval(f.jacobian(), n, [x.T, u.T, None]), axis=1).T
+    assert O.rel_err(jj.cpu().numpy(), ref) <= TOL

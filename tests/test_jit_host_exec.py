@@ -116,3 +116,70 @@ def test_generated_segments_run_on_host(monkeypatch, sub):
     (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
     assert np.array_equal(J, Jo.T)
     assert np.array_equal(gx, ox.T) and np.array_equal(gu, ou.T)
+
+
+def test_generated_tangent_sweeps_run_on_host(monkeypatch):
+    """Specialised tangent kernels (cart-pole Jacobian, 5 forward colours in
+    direction groups of 2, one segment; a 3-step rollout gradient's star-
+    coloured Hessian in segments) against the oracle evaluating the symbolic
+    jacobian() / hessian() Functions."""
+    import casadi_amd as ca
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipJacobian
+    monkeypatch.setenv("CXH_JIT_DIRGROUP", "2")
+    f = W.cartpole_step()
+    hj = HipJacobian(f, engine="jit")
+    n = 29
+    rng = np.random.default_rng(9)
+    x, u = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (1, n))
+    _, ng, ns = hj.jit_source(0, 0)
+    srcs = [hj.jit_source(g, 0)[0] for g in range(ng)]
+    assert ng == 3 and ns == 1
+    jac = np.full((hj.nnz_jac, n), np.nan)
+    y = np.zeros((4, n))
+    with tempfile.TemporaryDirectory() as d:
+        libs = _compile(srcs, d)
+        A = CxjArgs()
+        A.n, A.lds = n, n
+        A.arg = _table([x, u], n)
+        A.res = _table([y], n)
+        A.asens = _table([jac], n)
+        for _, fn in libs:
+            fn(C.byref(A), n)
+    ref = np.concatenate(O.map_eval(f.jacobian(), n, [x.T, u.T, None]), axis=1).T
+    assert O.rel_err(jac, ref) <= 1e-14
+    assert O.rel_err(y, O.map_eval(f, n, [x.T, u.T])[0].T) == 0.0
+    # star-coloured Hessian of a 2-step quadrotor OCP Lagrangian, 4 segments
+    monkeypatch.setenv("CXH_JIT_DIRGROUP", "4")
+    monkeypatch.setenv("CXH_JIT_TANSEG", "400")
+    w, lam, Lag = W.quadrotor_ocp_lagrangian(N=2)
+    G = ca.Function("grad_lag", [w, lam], [ca.gradient(Lag, w)])
+    H, _ = ca.hessian(Lag, w)
+    hh = HipJacobian(G, symmetric=True, first_block=True, engine="jit")
+    _, ng, ns = hh.jit_source(0, 0)
+    assert ns > 1 and ng > 1
+    wv, lv = rng.uniform(-1, 1, (w.nnz(), n)), rng.standard_normal((lam.nnz(), n))
+    hnz = np.full((hh.nnz_jac, n), np.nan)
+    gv = np.zeros((w.nnz(), n))
+    st = [np.zeros((4096, n)) for _ in range(2)]
+    with tempfile.TemporaryDirectory() as d:
+        for g in range(ng):
+            srcs = [hh.jit_source(g, s)[0] for s in range(ns)]
+            sub = os.path.join(d, f"g{g}")
+            os.makedirs(sub)
+            libs = _compile(srcs, sub)
+            A = CxjArgs()
+            A.n, A.lds = n, n
+            A.arg = _table([wv, lv], n)
+            A.res = _table([gv], n)
+            A.asens = _table([hnz], n)
+            for s, (_, fn) in enumerate(libs):
+                A.st_in = st[s % 2].ctypes.data
+                A.st_out = st[(s + 1) % 2].ctypes.data
+                fn(C.byref(A), n)
+    Hf = ca.Function("hess_lag", [w, lam], [H])
+    (href,) = O.map_eval(Hf, n, [wv.T, lv.T])
+    assert hh.jsp == H.sparsity
+    assert O.rel_err(hnz, href.T) <= 1e-13
+    (gref,) = O.map_eval(G, n, [wv.T, lv.T])
+    assert O.rel_err(gv, gref.T) == 0.0
