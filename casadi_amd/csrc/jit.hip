@@ -118,7 +118,7 @@ struct JitTape {
 // Instructions per sub-block of the reverse kernels (0: whole segment).
 static int sub_block_len() {
   if (const char* e = std::getenv("CXH_JIT_SUBBLOCK")) return std::max(0, std::atoi(e));
-  return 128;
+  return 256;
 }
 
 // Slots live at each of the boundaries bnd (ascending, inside [t0, t1]),
