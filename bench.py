@@ -132,6 +132,16 @@ def main():
     (Jr,) = O.map_eval(f, m, [x0[:, :m].T, us[:, :m].T])
     err_J = O.rel_err(J[:, :m].cpu().numpy(), Jr.T)
 
+    # HBM bytes per cxj_rev launch and its hardware FP64 work, from the
+    # rocprofv3 --pmc runs summarised under profiles/ (same workload)
+    traffic, hw_tflops = None, None
+    prof = os.path.join(ROOT, "profiles", "cxj_rev_rollout_traffic.json")
+    if os.path.exists(prof) and n == N_BATCH:
+        with open(prof) as fh:
+            pj = json.load(fh)
+        traffic = pj["hbm_bytes_per_launch"]
+        hw_tflops = pj["fp64_flops_per_launch_hw"] / (rev_launch_ms * 1e-3) / 1e12
+
     value = n / (ms_fused * 1e-3)
     plain_value = n / (ms_plain * 1e-3)
     bytes_per_inst = (4 + N_STEPS + 1 + 1 + 4 + N_STEPS) * 8   # x0, u, seed in; J, gradient out
@@ -175,7 +185,9 @@ def main():
         "err_grad_max": float(per.max()),
         "err_cost": err_J,
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "hbm_gbs_from_traffic": traffic / (rev_launch_ms * 1e-3) / 1e9 if traffic else None,
+                     "hw_fp64_tflops_from_counters": hw_tflops,
                      "kernel": "cxj_rev (reverse segment: recompute + adjoint sweep)",
                      "kernel_ms": rev_launch_ms, "flops_per_launch": flops_per_launch,
                      "hbm_algorithmic_gbs": bytes_per_inst * n / (ms_fused * 1e-3) / 1e9,
