@@ -160,6 +160,23 @@ class Function:
         return a.view(np.uint8).reshape(-1, 16)[:, 8:].copy().view("<f8").reshape(-1)
 
     # -- evaluation -----------------------------------------------------------
+    def _npar(self, args):
+        """Number of parallel evaluations implied by the argument shapes
+        (FunctionInternal::check_mat, casadi/core/function_internal.cpp:3649-3670):
+        an argument with n_rows == size1_in(i) and P*size2_in(i) columns asks
+        for P evaluations; all such arguments must agree."""
+        npar = 1
+        for i, a in enumerate(args):
+            if a is None:
+                continue
+            v = np.asarray(a, dtype=float)
+            nr, nc = self.sparsity_in(i).shape
+            if v.ndim != 2 or v.size <= 1 or v.shape == (nr, nc):
+                continue
+            if v.shape[0] == nr and v.shape[1] > nc and nc > 0 and v.shape[1] % (npar * nc) == 0:
+                npar *= v.shape[1] // (npar * nc)
+        return npar
+
     def _numeric_args(self, args):
         out = []
         for i, a in enumerate(args):
@@ -168,11 +185,18 @@ class Function:
                 out.append(None)
                 continue
             v = np.asarray(a, dtype=float)
+            nr, nc = sp.shape
             if v.ndim == 0 or v.size == 1:
                 dense = np.full(sp.shape, float(v.reshape(-1)[0]) if v.size else 0.0)
+            elif v.size == 0:
+                dense = np.zeros(sp.shape)
             else:
                 if v.ndim == 1:
-                    v = v.reshape(-1, 1) if sp.shape[1] == 1 else v.reshape(1, -1)
+                    v = v.reshape(-1, 1) if nc == 1 else v.reshape(1, -1)
+                if v.shape != sp.shape and min(v.shape) == 1 and v.shape[::-1] == sp.shape:
+                    v = v.T                          # a transposed vector
+                if v.shape != sp.shape and v.shape[0] == nr and 0 < v.shape[1] < nc and nc % v.shape[1] == 0:
+                    v = np.tile(v, (1, nc // v.shape[1]))   # repeated horizontally
                 if v.shape != sp.shape:
                     raise ValueError(f"{self.name()}: input {i} has shape {v.shape}, expected {sp.shape}")
                 dense = v
@@ -210,6 +234,10 @@ class Function:
         return res
 
     def _call_num(self, args):
+        npar = self._npar(args)
+        if npar > 1:   # evaluation with multiple arguments: a serial map
+            from .map import MapFunction
+            return MapFunction(self, npar, "serial").call(list(args))
         arg = self._numeric_args(args)
         res = [np.zeros(self.nnz_out(i)) for i in range(self.n_out())]
         pa = L.ptr_array([a.ctypes.data if a is not None and a.size else None for a in arg])

@@ -72,6 +72,20 @@ class MapFunction:
         res = self.call(list(args))
         return res[0] if len(res) == 1 else tuple(res)
 
+    # -- derivatives: the map of the derivative (Map::get_forward / get_reverse,
+    # casadi/core/map.cpp:219-325); a map's forward/reverse Function has the
+    # base Function's derivative signature with every argument stacked n times
+    def forward(self, nfwd):
+        return MapFunction(self.f.forward(nfwd), self.n, self.parallelization, self.n_threads, self.opts)
+
+    def reverse(self, nadj):
+        return MapFunction(self.f.reverse(nadj), self.n, self.parallelization, self.n_threads, self.opts)
+
+    def jacobian_blocks(self):
+        """Per-instance Jacobian of the base Function, mapped (the block-diagonal
+        nonzeros of the map's Jacobian)."""
+        return MapFunction(self.f.jacobian(), self.n, self.parallelization, self.n_threads, self.opts)
+
     def call(self, args):
         if len(args) != self.f.n_in():
             raise ValueError(f"{self.name()}: expected {self.f.n_in()} inputs")
