@@ -101,6 +101,11 @@ cxh_batch_launch = _opt("cxh_batch_launch", I32, P, P)
 cxh_batch_launch_direct = _opt("cxh_batch_launch_direct", I32, P, P)
 cxh_batch_info = _opt("cxh_batch_info", I32, P, C.POINTER(I32), PI64)
 cxh_batch_destroy = _opt("cxh_batch_destroy", I32, P)
+cxh_mapaccum_create = _fn("cxh_mapaccum_create", I32, P, I64, I32, PP)
+cxh_mapaccum_destroy = _fn("cxh_mapaccum_destroy", I32, P)
+cxh_mapaccum_eval = _fn("cxh_mapaccum_eval", I32, P, I64, PP, PP, I32, I64, P)
+cxh_mapaccum_fwd_adj = _fn("cxh_mapaccum_fwd_adj", I32, P, I64, PP, PP, PP, PP, I32, I64, I64, P)
+cxh_mapaccum_source = _fn("cxh_mapaccum_source", I32, P, I32, I64, C.c_char_p, I64, PI64)
 cxh_batch_source = _opt("cxh_batch_source", I32, I32, PP, C.c_char_p, I64, PI64)
 
 # --- host symbolic layer ------------------------------------------------------

@@ -9,6 +9,7 @@ c2     star-coloured Hessian of the 20-step quadrotor OCP Lagrangian, 16384
 c4     heterogeneous batch: 40000 cart-pole + 12000 quadrotor + 25000 arm,
        forward + adjoint, one hipGraph launch (autotuned) vs plain launches
 c5     batched KKT solve, n = 274, 8192 instances, ldl and qr
+c3macc the 100-step rollout cost + gradient as a mapaccum of one step, 32768
 
 Inputs are synthetic, from numpy.random.default_rng(20261202 + workload
 index).  Times are HIP-event times on torch's current stream, device-resident
@@ -129,6 +130,29 @@ def hetero_batch():
     _out(k="batch_fused_per_function", engine="jit", ms=ms)
 
 
+def rollout_mapaccum(n=32768, N=100, K=10):
+    """The 100-step rollout cost + gradient as a mapaccum of the cost-carrying
+    step (one compiled step body, loops), checkpoints every K steps."""
+    import torch
+    from . import workloads as W
+    from .hip import HipMapAccum
+    step = W.cartpole_cost_step()
+    t0 = time.time()
+    m = HipMapAccum(step, N)
+    rng = np.random.default_rng(20261204)
+    X0 = _dev(np.vstack([rng.uniform(-1, 1, (4, n)), np.zeros((1, n))]))
+    U = _dev(rng.uniform(-10, 10, (N, n)))
+    S = torch.zeros((5 * N, n), dtype=torch.float64, device="cuda")
+    S[5 * (N - 1) + 4] = 1.0      # d(cost carried to the end) -- the |x_N|^2 term is outside the chain
+    m.fwd_adj([X0, U], [S], n, checkpoint_steps=K)
+    torch.cuda.synchronize()
+    ct = time.time() - t0
+    ms = _timed(lambda: m.fwd_adj([X0, U], [S], n, checkpoint_steps=K), 20)
+    _out(k="rollout_mapaccum_fwd_adj", n=n, N=N, K=K, ms=ms, inst_per_s=n / ms * 1e3, compile_s=ct)
+    ms = _timed(lambda: m.eval([X0, U], n), 20)
+    _out(k="rollout_mapaccum_eval", n=n, N=N, ms=ms)
+
+
 def kkt(n=8192):
     from . import workloads as W
     from .linsol import Linsol
@@ -155,6 +179,8 @@ def main(argv):
         hetero_batch()
     if "c5" in which:
         kkt()
+    if "c3macc" in which:
+        rollout_mapaccum()
 
 
 if __name__ == "__main__":

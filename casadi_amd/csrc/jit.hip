@@ -848,6 +848,265 @@ std::string jit_tangent_source(cxh_jac_s* j, int g, int s, int* n_group, int* n_
   return gen_tangent(j, J, g, s);
 }
 
+// ------------------------------------------------------------ mapaccum ---
+//
+// Function::mapaccum (casadi/core/function.cpp:661-787): N chained calls of a
+// step Function, the first n_accum outputs fed back as the first n_accum
+// inputs; the other inputs and all outputs are stacked horizontally, step t's
+// block after step t-1's.  On the GPU one lane runs the whole chain: a loop
+// over steps around ONE specialised step body, the state in registers, so code
+// size and compile time do not grow with N.  The adjoint checkpoints the state
+// every K steps (HBM, [row][instance]), then for each K-step segment from the
+// last: recompute and store the states of its steps, and sweep the steps
+// backwards with a specialised reverse step body, the adjoint of the state
+// carried in registers.
+
+struct MaccPlan {
+  int n_in = 0, n_out = 0, n_accum = 0;
+  int64_t N = 0, ns = 0;                 // steps, state size (nonzeros)
+  std::vector<int64_t> soff;             // state offset of accumulated block i
+  std::vector<int64_t> acc_input_instr;  // per state entry: INPUT instruction index or -1
+};
+
+static MaccPlan macc_plan(const cxh_tape_s* t, int64_t N, int n_accum) {
+  MaccPlan M;
+  M.n_in = static_cast<int>(t->nnz_in.size());
+  M.n_out = static_cast<int>(t->nnz_out.size());
+  M.n_accum = n_accum;
+  M.N = N;
+  if (n_accum < 0 || n_accum > std::min(M.n_in, M.n_out)) throw std::runtime_error("mapaccum: bad n_accum");
+  for (int i = 0; i < n_accum; ++i) {
+    if (t->nnz_in[i] != t->nnz_out[i])
+      throw std::runtime_error("mapaccum: accumulated input and output " + std::to_string(i) + " differ in size");
+    M.soff.push_back(M.ns);
+    M.ns += t->nnz_in[i];
+  }
+  M.acc_input_instr.assign(M.ns, -1);
+  std::vector<int64_t> out_seen(M.ns, 0);
+  for (size_t k = 0; k < t->alg.size(); ++k) {
+    const Instr& e = t->alg[k];
+    if (e.op == OP_INPUT && e.i1 < n_accum) M.acc_input_instr[M.soff[e.i1] + e.i2] = static_cast<int64_t>(k);
+    if (e.op == OP_OUTPUT && e.i0 < n_accum) ++out_seen[M.soff[e.i0] + e.i2];
+  }
+  for (int64_t j = 0; j < M.ns; ++j)
+    if (out_seen[j] != 1) throw std::runtime_error("mapaccum: every accumulated output nonzero needs one OUTPUT");
+  return M;
+}
+
+// Forward step body: state s -> sn, per-step inputs/outputs at column block
+// `step`.  Outputs are written when `outs` is true.
+static std::string gen_macc_step_fwd(const cxh_tape_s* t, const MaccPlan& M) {
+  std::string o = "__device__ __forceinline__ void cxj_step_fwd(const CxjArgs& A, const long long k, const long long "
+                  "step, const double* s, double* sn, const bool outs) {\n";
+  std::vector<int64_t> cur(t->ws, 0);
+  char buf[320];
+  for (size_t kk = 0; kk < t->alg.size(); ++kk) {
+    const Instr& e = t->alg[kk];
+    const int k = static_cast<int>(kk);
+    switch (e.op) {
+      case OP_CONST:
+        o += "  const double v" + std::to_string(k) + " = " + lit(e.d) + ";\n";
+        cur[e.i0] = k;
+        break;
+      case OP_INPUT:
+        if (e.i1 < M.n_accum)
+          std::snprintf(buf, sizeof buf, "  const double v%d = s[%lld];\n", k,
+                        static_cast<long long>(M.soff[e.i1] + e.i2));
+        else
+          std::snprintf(buf, sizeof buf, "  const double v%d = CXJ_IN(%d, step * %lld + %d);\n", k, e.i1,
+                        static_cast<long long>(t->nnz_in[e.i1]), e.i2);
+        o += buf;
+        cur[e.i0] = k;
+        break;
+      case OP_OUTPUT:
+        std::snprintf(buf, sizeof buf, "  if (outs) CXJ_OUT(%d, step * %lld + %d, v%lld);\n", e.i0,
+                      static_cast<long long>(t->nnz_out[e.i0]), e.i2, static_cast<long long>(cur[e.i1]));
+        o += buf;
+        if (e.i0 < M.n_accum)
+          o += "  sn[" + std::to_string(M.soff[e.i0] + e.i2) + "] = v" + std::to_string(cur[e.i1]) + ";\n";
+        break;
+      default: {
+        const int64_t x = cur[e.i1], y = op_is_binary(e.op) ? cur[e.i2] : x;
+        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, v%lld, v%lld);\n", k, e.op,
+                      static_cast<long long>(x), static_cast<long long>(y));
+        o += buf;
+        cur[e.i0] = k;
+      }
+    }
+  }
+  o += "}\n";
+  return o;
+}
+
+// Reverse step body: recompute the step from s, then sweep back.  as holds
+// the adjoint of the step's output state on entry and of its input state on
+// exit; seeds of step t's outputs and sensitivities of its inputs are at
+// column block `step`.
+static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
+  const int ni = static_cast<int>(t->alg.size());
+  std::string o = "__device__ __forceinline__ void cxj_step_rev(const CxjArgs& A, const long long k, const long long "
+                  "step, const double* s, double* as) {\n";
+  std::vector<int64_t> cur(t->ws, 0), ox(ni, 0), oy(ni, 0);
+  char buf[320];
+  for (int k = 0; k < ni; ++k) {
+    const Instr& e = t->alg[k];
+    switch (e.op) {
+      case OP_CONST:
+        o += "  const double v" + std::to_string(k) + " = " + lit(e.d) + ";\n";
+        cur[e.i0] = k;
+        break;
+      case OP_INPUT:
+        if (e.i1 < M.n_accum)
+          std::snprintf(buf, sizeof buf, "  const double v%d = s[%lld];\n", k,
+                        static_cast<long long>(M.soff[e.i1] + e.i2));
+        else
+          std::snprintf(buf, sizeof buf, "  const double v%d = CXJ_IN(%d, step * %lld + %d);\n", k, e.i1,
+                        static_cast<long long>(t->nnz_in[e.i1]), e.i2);
+        o += buf;
+        cur[e.i0] = k;
+        break;
+      case OP_OUTPUT:
+        ox[k] = cur[e.i1];
+        break;
+      default:
+        ox[k] = cur[e.i1];
+        oy[k] = op_is_binary(e.op) ? cur[e.i2] : ox[k];
+        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, v%lld, v%lld);\n", k, e.op,
+                      static_cast<long long>(ox[k]), static_cast<long long>(oy[k]));
+        o += buf;
+        cur[e.i0] = k;
+    }
+  }
+  std::vector<char> decl(ni, 0);
+  auto acc = [&](int64_t c, const std::string& term) {
+    if (decl[c]) {
+      o += "  a" + std::to_string(c) + " += " + term + ";\n";
+    } else {
+      o += "  double a" + std::to_string(c) + " = " + term + ";\n";
+      decl[c] = 1;
+    }
+  };
+  for (int k = ni - 1; k >= 0; --k) {
+    const Instr& e = t->alg[k];
+    switch (e.op) {
+      case OP_CONST:
+        break;
+      case OP_OUTPUT:
+        std::snprintf(buf, sizeof buf, "CXJ_SEED(%d, step * %lld + %d)", e.i0, static_cast<long long>(t->nnz_out[e.i0]),
+                      e.i2);
+        acc(ox[k], buf);
+        if (e.i0 < M.n_accum) acc(ox[k], "as[" + std::to_string(M.soff[e.i0] + e.i2) + "]");
+        break;
+      case OP_INPUT:
+        if (e.i1 >= M.n_accum) {
+          std::snprintf(buf, sizeof buf, "  CXJ_SENS(%d, step * %lld + %d, %s);\n", e.i1,
+                        static_cast<long long>(t->nnz_in[e.i1]), e.i2,
+                        decl[k] ? ("a" + std::to_string(k)).c_str() : "0.0");
+          o += buf;
+        }
+        break;
+      default: {
+        if (!decl[k]) break;
+        const std::string ks = std::to_string(k);
+        o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", v" + std::to_string(ox[k]) +
+             ", v" + std::to_string(oy[k]) + ", v" + ks + ", d" + ks + ");\n";
+        if (e.op == OP_IF_ELSE_ZERO) {
+          acc(oy[k], "sxh::sxh_if_else_zero(d" + ks + "[1], a" + ks + ")");
+        } else if (op_is_binary(e.op)) {
+          acc(ox[k], "d" + ks + "[0] * a" + ks);
+          acc(oy[k], "d" + ks + "[1] * a" + ks);
+        } else {
+          acc(ox[k], "d" + ks + "[0] * a" + ks);
+        }
+      }
+    }
+  }
+  // adjoint of the input state (all reads of `as` above happen first)
+  for (int64_t j = 0; j < M.ns; ++j) {
+    const int64_t c = M.acc_input_instr[j];
+    o += "  as[" + std::to_string(j) + "] = " + (c >= 0 && decl[c] ? "a" + std::to_string(c) : std::string("0.0")) +
+         ";\n";
+  }
+  // unread non-accumulated inputs of this step
+  for (const int2& u : t->unused_in)
+    if (u.x >= M.n_accum) {
+      std::snprintf(buf, sizeof buf, "  CXJ_SENS(%d, step * %lld + %d, 0.0);\n", u.x,
+                    static_cast<long long>(t->nnz_in[u.x]), u.y);
+      o += buf;
+    }
+  o += "}\n";
+  return o;
+}
+
+// Kernels over n instances.  Scratch (A.st_out, [row][lds]): S*ns checkpoint
+// rows, then K*ns rows for the states of one segment's steps.
+static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, bool adjoint, int64_t K) {
+  std::string o = kSxOpsSource;
+  o += kPrologue;
+  o += gen_macc_step_fwd(t, M);
+  const std::string ns = std::to_string(std::max<int64_t>(M.ns, 1));
+  const std::string N = std::to_string(M.N);
+  std::string load_x0, store_sens;
+  for (int i = 0; i < M.n_accum; ++i)
+    for (int64_t j = 0; j < t->nnz_in[i]; ++j) {
+      const std::string sj = std::to_string(M.soff[i] + j);
+      load_x0 += "  s[" + sj + "] = CXJ_IN(" + std::to_string(i) + ", " + std::to_string(j) + ");\n";
+      store_sens += "  CXJ_SENS(" + std::to_string(i) + ", " + std::to_string(j) + ", as[" + sj + "]);\n";
+    }
+  if (!adjoint) {
+    o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
+         "  double s[" + ns + "], sn[" + ns + "];\n" + load_x0 +
+         "#pragma unroll 1\n"
+         "  for (long long step = 0; step < " + N + "; ++step) {\n"
+         "    cxj_step_fwd(A, k, step, s, sn, true);\n"
+         "    for (int j = 0; j < " + std::to_string(M.ns) + "; ++j) s[j] = sn[j];\n"
+         "  }\n}\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) cxj_macc_fwd(const CxjArgs A) {\n"
+         "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+         "  if (k < A.n) cxj_body(A, k);\n"
+         "}\n";
+    return o;
+  }
+  o += gen_macc_step_rev(t, M);
+  const int64_t S = (M.N + K - 1) / K;
+  const std::string Ks = std::to_string(K), Ss = std::to_string(S), nsi = std::to_string(M.ns);
+  o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
+       "  double s[" + ns + "], sn[" + ns + "], as[" + ns + "];\n" + load_x0 +
+       "#pragma unroll 1\n"
+       "  for (long long step = 0; step < " + N + "; ++step) {\n"
+       "    if (step % " + Ks + " == 0)\n"
+       "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n"
+       "    cxj_step_fwd(A, k, step, s, sn, true);\n"
+       "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+       "  }\n"
+       "  for (int j = 0; j < " + nsi + "; ++j) as[j] = 0.0;\n"
+       "#pragma unroll 1\n"
+       "  for (long long seg = " + Ss + " - 1; seg >= 0; --seg) {\n"
+       "    const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
+       "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, seg * " + nsi + " + j);\n"
+       "#pragma unroll 1\n"
+       "    for (long long step = t0; step < t1; ++step) {\n"
+       "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, " + std::to_string(S * M.ns) +
+       " + (step - t0) * " + nsi + " + j) = s[j];\n"
+       "      if (step + 1 < t1) {\n"
+       "        cxj_step_fwd(A, k, step, s, sn, false);\n"
+       "        for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+       "      }\n"
+       "    }\n"
+       "#pragma unroll 1\n"
+       "    for (long long step = t1 - 1; step >= t0; --step) {\n"
+       "      for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, " + std::to_string(S * M.ns) +
+       " + (step - t0) * " + nsi + " + j);\n"
+       "      cxj_step_rev(A, k, step, s, as);\n"
+       "    }\n"
+       "  }\n" + store_sens + "}\n"
+       "extern \"C\" __global__ void __launch_bounds__(256) cxj_macc_adj(const CxjArgs A) {\n"
+       "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+       "  if (k < A.n) cxj_body(A, k);\n"
+       "}\n";
+  return o;
+}
+
 // ----------------------------------------------------- heterogeneous batch ---
 
 // A tile is `block` consecutive instances of one Function.
@@ -890,6 +1149,15 @@ static std::string gen_batch(const std::vector<cxh_tape_s*>& tapes) {
 }  // namespace sxh
 
 using namespace sxh;
+
+struct cxh_mapaccum_s {
+  cxh_tape_s* tape = nullptr;
+  sxh::MaccPlan plan;
+  std::vector<int64_t> nnz_in, nnz_out;  // of the mapaccum Function
+  hipFunction_t fwd = nullptr;
+  std::map<int64_t, hipFunction_t> adj;  // by checkpoint interval
+  sxh::DeviceBuffer scratch;
+};
 
 struct cxh_batch_s {
   std::vector<cxh_tape_s*> tapes;
@@ -963,6 +1231,87 @@ static void batch_check_bound(const cxh_batch_s* b) {
 }  // namespace sxh
 
 extern "C" {
+
+int cxh_mapaccum_create(cxh_tape step, int64_t n_steps, int32_t n_accum, cxh_mapaccum* out) {
+  CXH_TRY
+  if (!step || !out) return cxh_fail("cxh_mapaccum_create: null argument");
+  if (n_steps <= 0) return cxh_fail("cxh_mapaccum_create: n_steps must be positive");
+  if (step->alg.empty()) return cxh_fail("cxh_mapaccum_create: empty tape");
+  auto m = std::make_unique<cxh_mapaccum_s>();
+  m->tape = step;
+  m->plan = macc_plan(step, n_steps, n_accum);
+  for (size_t i = 0; i < step->nnz_in.size(); ++i)
+    m->nnz_in.push_back(static_cast<int>(i) < n_accum ? step->nnz_in[i] : n_steps * step->nnz_in[i]);
+  for (size_t i = 0; i < step->nnz_out.size(); ++i) m->nnz_out.push_back(n_steps * step->nnz_out[i]);
+  *out = m.release();
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_mapaccum_destroy(cxh_mapaccum m) {
+  delete m;
+  return 0;
+}
+
+int cxh_mapaccum_source(cxh_mapaccum m, int32_t adjoint, int64_t checkpoint_steps, char* buf, int64_t cap,
+                        int64_t* len) {
+  CXH_TRY
+  if (!m) return cxh_fail("null mapaccum");
+  const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
+  const std::string src = gen_macc(m->tape, m->plan, adjoint != 0, K);
+  if (len) *len = static_cast<int64_t>(src.size());
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return 0;
+  CXH_CATCH
+}
+
+static int macc_launch(cxh_mapaccum m, bool adjoint, int64_t n, const double* const* arg,
+                       const double* const* adj_out, double* const* res, double* const* adj_in, int32_t layout,
+                       int64_t ld, int64_t checkpoint_steps, void* stream) {
+  if (!m) return cxh_fail("null mapaccum");
+  if (n < 0) return cxh_fail("negative n");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  if (n == 0) return 0;
+  const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
+  hipFunction_t& f = adjoint ? m->adj[K] : m->fwd;
+  if (!f) f = jit_function(gen_macc(m->tape, m->plan, adjoint, K), adjoint ? "cxj_macc_adj" : "cxj_macc_fwd");
+  const int64_t lds = round_up(n, 64);
+  CxjArgs A{};
+  A.arg = make_io(arg, m->nnz_in, layout, ld);
+  A.res = make_io(res, m->nnz_out, layout, ld);
+  A.n = n;
+  A.lds = lds;
+  if (adjoint) {
+    A.aseed = make_io(adj_out, m->nnz_out, layout, ld);
+    A.asens = make_io(adj_in, m->nnz_in, layout, ld);
+    const int64_t S = (m->plan.N + K - 1) / K;
+    const int64_t rows = std::max<int64_t>((S + K) * m->plan.ns, 1);
+    m->scratch.reserve(static_cast<size_t>(rows) * lds * sizeof(double));
+    A.st_out = m->scratch.as<double>();
+  }
+  launch(f, n, m->tape->block, A, static_cast<hipStream_t>(stream));
+  return 0;
+}
+
+int cxh_mapaccum_eval(cxh_mapaccum m, int64_t n, const double* const* arg, double* const* res, int32_t layout,
+                      int64_t ld, void* stream) {
+  CXH_TRY
+  return macc_launch(m, false, n, arg, nullptr, res, nullptr, layout, ld, 0, stream);
+  CXH_CATCH
+}
+
+int cxh_mapaccum_fwd_adj(cxh_mapaccum m, int64_t n, const double* const* arg, const double* const* adj_out,
+                         double* const* res, double* const* adj_in, int32_t layout, int64_t ld,
+                         int64_t checkpoint_steps, void* stream) {
+  CXH_TRY
+  return macc_launch(m, true, n, arg, adj_out, res, adj_in, layout, ld, checkpoint_steps, stream);
+  CXH_CATCH
+}
 
 int cxh_batch_create(int32_t n_fun, const cxh_tape* tapes, const int64_t* counts, cxh_batch* out) {
   CXH_TRY

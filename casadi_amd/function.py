@@ -314,6 +314,53 @@ class Function:
                                     ndir=len(sptr) - 1)
         return self._cache[key]
 
+    # -- chains -----------------------------------------------------------------
+    def mapaccum(self, name, N=None, n_accum=1):
+        """Function::mapaccum (casadi/core/function.cpp:669-754): N chained
+        calls, the first n_accum outputs fed back as the first n_accum inputs;
+        the other inputs and all outputs stacked horizontally.  Built here as
+        one SX Function (the chain unrolled); ``macc_base`` keeps the step
+        Function so that batched GPU evaluation can loop over one compiled step
+        (casadi_amd.hip.HipMapAccum).  ``mapaccum(N)`` names it mapaccum_<f>."""
+        if N is None:
+            name, N = "mapaccum_" + self.name(), name
+        N = int(N)
+        if N <= 0:
+            raise ValueError("mapaccum: N must be positive")
+        if n_accum > min(self.n_in(), self.n_out()):
+            raise ValueError("mapaccum: too many accumulators")
+        for i in range(n_accum):
+            if self.sparsity_in(i).shape != self.sparsity_out(i).shape:
+                raise ValueError(f"mapaccum: input {i} and output {i} differ in shape")
+        ins = [SX.sym(self.name_in(i), *self.sparsity_in(i).shape) if i < n_accum else
+               SX.sym(self.name_in(i), self.sparsity_in(i).size1(), N * self.sparsity_in(i).size2())
+               for i in range(self.n_in())]
+        state = ins[:n_accum]
+        outs = [[] for _ in range(self.n_out())]
+        for k in range(N):
+            args = list(state)
+            for i in range(n_accum, self.n_in()):
+                c = self.sparsity_in(i).size2()
+                args.append(ins[i][:, k * c:(k + 1) * c])
+            res = self.call(args)
+            for o in range(self.n_out()):
+                outs[o].append(res[o])
+            state = res[:n_accum]
+        from .sx import horzcat
+        F = Function(name, ins, [horzcat(*r) for r in outs], self.name_in(), self.name_out())
+        F.macc_base = (self, N, n_accum)
+        return F
+
+    def fold(self, N):
+        """Function::fold (casadi/core/function.cpp:661-667): mapaccum with only
+        the last step's block of output 0."""
+        base = self.mapaccum("mapaccum_" + self.name(), N)
+        ins = [SX.sym(base.name_in(i), *base.sparsity_in(i).shape) for i in range(base.n_in())]
+        out = base.call(ins)
+        c = self.sparsity_out(0).size2()
+        out[0] = out[0][:, (N - 1) * c:N * c]
+        return Function("fold_" + self.name(), ins, out, base.name_in(), base.name_out())
+
     # -- batching ---------------------------------------------------------------
     def to_torch(self, checkpoint_every=0, engine="jit"):
         """Batched GPU evaluation as a differentiable torch op whose backward

@@ -297,6 +297,66 @@ class HipBatch:
         L.check(fn(self._h, _stream_ptr(stream)), "cxh_batch_launch")
 
 
+class HipMapAccum:
+    """Batched mapaccum (cxh_mapaccum): every instance runs the N-step chain of
+    ``step`` in one lane, looping over one specialised step body.  Blocks follow
+    the mapaccum Function: inputs i < n_accum are (nnz_in(i), n), the others
+    and all outputs (N*nnz, n) with step t's nonzeros after step t-1's."""
+
+    def __init__(self, step, N, n_accum=1, tape=None):
+        self.step = step
+        self.N, self.n_accum = int(N), int(n_accum)
+        self.tape = tape if tape is not None else HipTape(step)
+        self.nnz_in = [step.nnz_in(i) if i < n_accum else self.N * step.nnz_in(i) for i in range(step.n_in())]
+        self.nnz_out = [self.N * step.nnz_out(i) for i in range(step.n_out())]
+        h = L.P()
+        L.check(L.cxh_mapaccum_create(self.tape._h, self.N, self.n_accum, C.byref(h)), "cxh_mapaccum_create")
+        self._h = h
+
+    @classmethod
+    def from_function(cls, F):
+        """From a Function built by Function.mapaccum."""
+        base, N, n_accum = F.macc_base
+        return cls(base, N, n_accum)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L.cxh_mapaccum_destroy is not None:
+            L.cxh_mapaccum_destroy(h)
+            self._h = None
+
+    def source(self, adjoint=False, checkpoint_steps=0):
+        n = L.I64()
+        L.check(L.cxh_mapaccum_source(self._h, int(adjoint), int(checkpoint_steps), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        L.check(L.cxh_mapaccum_source(self._h, int(adjoint), int(checkpoint_steps), buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def _alloc(self, nnz, n, like):
+        torch = _torch()
+        return [torch.empty((k, n), dtype=torch.float64, device=like.device) if k else None for k in nnz]
+
+    def eval(self, args, n=None, stream=None):
+        like = next(a for a in args if a is not None)
+        n = n if n is not None else like.shape[1]
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        out = self._alloc(self.nnz_out, n, like)
+        L.check(L.cxh_mapaccum_eval(self._h, n, _ptrs(args), _ptrs(out), CXH_SOA, ld, _stream_ptr(stream)),
+                "cxh_mapaccum_eval")
+        return out
+
+    def fwd_adj(self, args, seeds, n=None, checkpoint_steps=0, res=True, stream=None):
+        like = next(a for a in args if a is not None)
+        n = n if n is not None else like.shape[1]
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        _check_soa(seeds, self.nnz_out, n, "adj_out")
+        outs = self._alloc(self.nnz_out, n, like) if res else [None] * len(self.nnz_out)
+        adj = self._alloc(self.nnz_in, n, like)
+        L.check(L.cxh_mapaccum_fwd_adj(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adj), CXH_SOA, ld,
+                                       int(checkpoint_steps), _stream_ptr(stream)), "cxh_mapaccum_fwd_adj")
+        return outs, adj
+
+
 def batch_source(tapes):
     """Generated source of the batch kernel over these HipTapes."""
     arr = L.ptr_array([t._h.value for t in tapes])

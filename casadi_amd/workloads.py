@@ -44,6 +44,16 @@ def cartpole_step(dt=0.05):
     return Function("cartpole", [x, u], [rk4(cartpole_ode, x, u, dt)], ["x", "u"], ["xf"])
 
 
+def cartpole_cost_step(dt=0.05, u_weight=0.01):
+    """One RK4 step carrying the running rollout cost: ([x; c], u) ->
+    [x_next; c + |x|^2 + u_weight u^2].  Its mapaccum over N steps, plus
+    |x_N|^2, is cartpole_rollout(N)'s cost."""
+    x, c, u = SX.sym("x", 4), SX.sym("c"), SX.sym("u")
+    xn = rk4(cartpole_ode, x, u, dt)
+    cn = c + sumsqr(x) + u_weight * u * u
+    return Function("cost_step", [vertcat(x, c), u], [vertcat(xn, cn)], ["xc", "u"], ["xc_next"])
+
+
 def cartpole_rollout(n_steps=100, dt=0.05, u_weight=0.01):
     """J(x0, u[0:n]) = sum_t ||x_t||^2 + u_weight * u_t^2 over an RK4 rollout,
     x_t the state before step t plus the final state."""

@@ -38,6 +38,7 @@ const char* cxh_last_error(void);
 typedef struct cxh_tape_s* cxh_tape;
 typedef struct cxh_jac_s* cxh_jac;
 typedef struct cxh_batch_s* cxh_batch;
+typedef struct cxh_mapaccum_s* cxh_mapaccum;
 typedef struct cxh_ldl_s* cxh_ldl;
 typedef struct cxh_qr_s* cxh_qr;
 
@@ -186,6 +187,30 @@ int cxh_batch_destroy(cxh_batch b);
 /* Introspection: the generated source of a batch kernel over these tapes
  * (NUL-terminated into buf, truncated to cap; full length in *len).         */
 int cxh_batch_source(int32_t n_fun, const cxh_tape* tapes, char* buf, int64_t cap, int64_t* len);
+
+/* mapaccum over a batch: N chained evaluations of a step tape per instance,
+ * its first n_accum outputs fed back as its first n_accum inputs.
+ * Replaces: Function::mapaccum / fold (casadi/core/function.cpp:661-787)
+ *   evaluated by Map::eval per instance.
+ * The mapaccum Function's blocks: inputs i < n_accum are the initial state
+ *   (nnz_in(i)); the other inputs and every output are the N step blocks
+ *   stacked (N*nnz, step t's block after step t-1's), as CasADi's horzcat.
+ * One lane runs the chain: a loop around one specialised step body (hiprtc,
+ * compiled at first use), so code size does not grow with N.
+ * cxh_mapaccum_fwd_adj: outputs and the adjoint of every input for output
+ *   cotangents adj_out, checkpointing the state every checkpoint_steps steps
+ *   (0 = keep only the initial state and recompute).                        */
+int cxh_mapaccum_create(cxh_tape step, int64_t n_steps, int32_t n_accum, cxh_mapaccum* out);
+int cxh_mapaccum_destroy(cxh_mapaccum m);
+int cxh_mapaccum_eval(cxh_mapaccum m, int64_t n, const double* const* arg, double* const* res,
+                      int32_t layout, int64_t ld, void* stream);
+int cxh_mapaccum_fwd_adj(cxh_mapaccum m, int64_t n, const double* const* arg,
+                         const double* const* adj_out, double* const* res, double* const* adj_in,
+                         int32_t layout, int64_t ld, int64_t checkpoint_steps, void* stream);
+/* Introspection: generated source of the forward (adjoint = 0) or fused
+ * forward + adjoint kernel.                                                */
+int cxh_mapaccum_source(cxh_mapaccum m, int32_t adjoint, int64_t checkpoint_steps, char* buf,
+                        int64_t cap, int64_t* len);
 
 /* Batched sparse LDL^T on CasADi's symbolic factorisation.
  * sp_a, sp_lt: CasADi compact sparsity [nrow, ncol, colind..., row...] of A

@@ -120,9 +120,9 @@ def test_generated_segments_run_on_host(monkeypatch, sub):
 
 def test_generated_tangent_sweeps_run_on_host(monkeypatch):
     """Specialised tangent kernels (cart-pole Jacobian, 5 forward colours in
-    direction groups of 2, one segment; a 3-step rollout gradient's star-
-    coloured Hessian in segments) against the oracle evaluating the symbolic
-    jacobian() / hessian() Functions."""
+    direction groups of 2, one segment; the star-coloured Hessian of a 2-step
+    quadrotor OCP Lagrangian in direction groups and segments) against the
+    oracle evaluating the symbolic jacobian() / hessian() Functions."""
     import casadi_amd as ca
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipJacobian
@@ -183,3 +183,40 @@ def test_generated_tangent_sweeps_run_on_host(monkeypatch):
     assert O.rel_err(hnz, href.T) <= 1e-13
     (gref,) = O.map_eval(G, n, [wv.T, lv.T])
     assert O.rel_err(gv, gref.T) == 0.0
+
+
+@pytest.mark.parametrize("K", [0, 3, 4])
+def test_mapaccum_loop_kernels_run_on_host(K):
+    """The mapaccum loop kernels (one step body, checkpoints every K steps)
+    against the oracle on the unrolled chain."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipMapAccum
+    step = W.cartpole_cost_step()
+    N = 10
+    F = step.mapaccum("roll", N)
+    m = HipMapAccum.from_function(F)
+    n = 23
+    rng = np.random.default_rng(11)
+    x0 = np.vstack([rng.uniform(-1, 1, (4, n)), np.zeros((1, n))])
+    U = rng.uniform(-5, 5, (N, n))
+    seed = rng.standard_normal((5 * N, n))
+    out = np.zeros((5 * N, n))
+    g0, gU = np.zeros((5, n)), np.zeros((N, n))
+    y = np.zeros((5 * N, n))
+    with tempfile.TemporaryDirectory() as d:
+        (_, fwd), (_, adj) = _compile([m.source(False), m.source(True, K)], d)
+        scratch = np.zeros((4096, n))
+        A = CxjArgs()
+        A.n, A.lds = n, n
+        A.arg = _table([x0, U], n)
+        A.res = _table([y], n)
+        fwd(C.byref(A), n)
+        A.res = _table([out], n)
+        A.aseed = _table([seed], n)
+        A.asens = _table([g0, gU], n)
+        A.st_out = scratch.ctypes.data
+        adj(C.byref(A), n)
+    (ref,) = O.map_eval(F, n, [x0.T, U.T])
+    assert np.array_equal(y, ref.T) and np.array_equal(out, ref.T)
+    (_,), (r0, rU) = O.adj_map(F, n, [x0.T, U.T], [seed.T])
+    assert O.rel_err(g0, r0.T) <= 1e-13 and O.rel_err(gU, rU.T) <= 1e-13
