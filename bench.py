@@ -146,6 +146,9 @@ def main():
     per = np.max(np.abs(g - gr) / np.maximum(np.max(np.abs(gr), axis=1, keepdims=True), 1.0), axis=1)
     (Jr,) = O.map_eval(f, m, [x0[:, :m].T, us[:, :m].T])
     err_J = O.rel_err(J[:, :m].cpu().numpy(), Jr.T)
+    _, (mg0, mgu) = fused_macc()
+    gm = np.concatenate([mg0[:4, :m].cpu().numpy(), mgu[:, :m].cpu().numpy()]).T
+    per_m = np.max(np.abs(gm - gr) / np.maximum(np.max(np.abs(gr), axis=1, keepdims=True), 1.0), axis=1)
 
     # HBM bytes per cxj_rev launch and its hardware FP64 work, from the
     # rocprofv3 --pmc runs summarised under profiles/ (same workload)
@@ -205,6 +208,7 @@ def main():
         "fused_interpreter": {"value": n / (ms_fused_interp * 1e-3), "ms_per_step": ms_fused_interp},
         "fused_mapaccum": {"value": n / (ms_macc * 1e-3), "ms_per_step": ms_macc,
                            "speedup_vs_plain": ms_plain / ms_macc,
+                           "err_grad_median": float(np.median(per_m)), "err_grad_max": float(per_m.max()),
                            "note": "same cost and gradient as mapaccum(100) of the cost-carrying RK4 step: "
                                    "loop kernel around one specialised step, checkpoints every 10 steps"},
         "jit_compile_s": compile_s,
