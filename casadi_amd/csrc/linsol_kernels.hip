@@ -37,6 +37,57 @@ struct Col {
   __device__ double& operator[](int64_t i) const { return base[i * stride]; }
 };
 
+// v - sum over k in [b, e) of x[k] * w[row[k]], subtracted in increasing k
+// (the reference's order).  Four terms' loads are issued before their
+// subtractions so that the memory latencies of a column overlap; the
+// arithmetic is unchanged.
+__device__ __forceinline__ double sub_dot(double v, Col x, Col w, const int* __restrict__ row, int b, int e) {
+  int k = b;
+  for (; k + 4 <= e; k += 4) {
+    const int r0 = row[k], r1 = row[k + 1], r2 = row[k + 2], r3 = row[k + 3];
+    const double x0 = x[k], x1 = x[k + 1], x2 = x[k + 2], x3 = x[k + 3];
+    const double w0 = w[r0], w1 = w[r1], w2 = w[r2], w3 = w[r3];
+    v -= x0 * w0;
+    v -= x1 * w1;
+    v -= x2 * w2;
+    v -= x3 * w3;
+  }
+  for (; k < e; ++k) v -= x[k] * w[row[k]];
+  return v;
+}
+
+// s + sum over k in [b, e) of x[k] * w[row[k]], added in increasing k.
+__device__ __forceinline__ double add_dot(double s, Col x, Col w, const int* __restrict__ row, int b, int e) {
+  int k = b;
+  for (; k + 4 <= e; k += 4) {
+    const int r0 = row[k], r1 = row[k + 1], r2 = row[k + 2], r3 = row[k + 3];
+    const double x0 = x[k], x1 = x[k + 1], x2 = x[k + 2], x3 = x[k + 3];
+    const double w0 = w[r0], w1 = w[r1], w2 = w[r2], w3 = w[r3];
+    s += x0 * w0;
+    s += x1 * w1;
+    s += x2 * w2;
+    s += x3 * w3;
+  }
+  for (; k < e; ++k) s += x[k] * w[row[k]];
+  return s;
+}
+
+// w[row[k]] -= a * x[k] for k in [b, e); the rows of one column are
+// distinct, so the updates are independent and their loads are batched.
+__device__ __forceinline__ void sub_axpy(double a, Col x, Col w, const int* __restrict__ row, int b, int e) {
+  int k = b;
+  for (; k + 4 <= e; k += 4) {
+    const int r0 = row[k], r1 = row[k + 1], r2 = row[k + 2], r3 = row[k + 3];
+    const double x0 = x[k], x1 = x[k + 1], x2 = x[k + 2], x3 = x[k + 3];
+    const double w0 = w[r0], w1 = w[r1], w2 = w[r2], w3 = w[r3];
+    w[r0] = w0 - a * x0;
+    w[r1] = w1 - a * x1;
+    w[r2] = w2 - a * x2;
+    w[r3] = w3 - a * x3;
+  }
+  for (; k < e; ++k) w[row[k]] -= a * x[k];
+}
+
 struct LdlArgs {
   int n;
   const int* a_colind;
@@ -81,8 +132,8 @@ __global__ void __launch_bounds__(64) ldl_kernel(LdlArgs A) {
     double dc = d[c];
     for (int k = A.lt_colind[c]; k < A.lt_colind[c + 1]; ++k) {
       const int r = A.lt_row[k];
-      double v = lt[k];
-      for (int k2 = A.lt_colind[r]; k2 < A.lt_colind[r + 1]; ++k2) v -= lt[k2] * w[A.lt_row[k2]];
+      const double v0 = lt[k];
+      double v = sub_dot(v0, lt, w, A.lt_row, A.lt_colind[r], A.lt_colind[r + 1]);
       w[r] = v;
       v /= d[r];
       lt[k] = v;
@@ -99,15 +150,11 @@ __global__ void __launch_bounds__(64) ldl_kernel(LdlArgs A) {
   for (int q = 0; q < A.nrhs; ++q) {
     double* x = A.x + jj * A.x_t + static_cast<int64_t>(q) * n * A.x_s;
     for (int i = 0; i < n; ++i) w[i] = x[A.p[i] * A.x_s];
-    for (int c = 0; c < n; ++c) {
-      double v = w[c];
-      for (int k = A.lt_colind[c]; k < A.lt_colind[c + 1]; ++k) v -= lt[k] * w[A.lt_row[k]];
-      w[c] = v;
-    }
+    for (int c = 0; c < n; ++c) w[c] = sub_dot(w[c], lt, w, A.lt_row, A.lt_colind[c], A.lt_colind[c + 1]);
     for (int i = 0; i < n; ++i) w[i] = w[i] / d[i];
     for (int c = n - 1; c >= 0; --c) {
       const double wc = w[c];
-      for (int k = A.lt_colind[c + 1] - 1; k >= A.lt_colind[c]; --k) w[A.lt_row[k]] -= lt[k] * wc;
+      sub_axpy(wc, lt, w, A.lt_row, A.lt_colind[c], A.lt_colind[c + 1]);
     }
     if (active)
       for (int i = 0; i < n; ++i) x[A.p[i] * A.x_s] = w[i];
@@ -168,10 +215,9 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
     for (int k = A.a_colind[pcc]; k < A.a_colind[pcc + 1]; ++k) x[A.prinv[A.a_row[k]]] = a[k * A.a_s];
     int rr;
     for (int k = A.r_colind[c]; k < A.r_colind[c + 1] && (rr = A.r_row[k]) < c; ++k) {
-      double alpha = 0;
-      for (int k1 = A.v_colind[rr]; k1 < A.v_colind[rr + 1]; ++k1) alpha += v[k1] * x[A.v_row[k1]];
+      double alpha = add_dot(0.0, v, x, A.v_row, A.v_colind[rr], A.v_colind[rr + 1]);
       alpha *= beta[rr];
-      for (int k1 = A.v_colind[rr]; k1 < A.v_colind[rr + 1]; ++k1) x[A.v_row[k1]] -= alpha * v[k1];
+      sub_axpy(alpha, v, x, A.v_row, A.v_colind[rr], A.v_colind[rr + 1]);
       r[nz_r++] = x[rr];
       x[rr] = 0;
     }
@@ -202,10 +248,9 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
       }
       for (int c1 = 0; c1 < ncol; ++c1) {
         const int c = ncol - 1 - c1;
-        double alpha = 0;
-        for (int k = A.v_colind[c]; k < A.v_colind[c + 1]; ++k) alpha += v[k] * w[A.v_row[k]];
+        double alpha = add_dot(0.0, v, w, A.v_row, A.v_colind[c], A.v_colind[c + 1]);
         alpha *= beta[c];
-        for (int k = A.v_colind[c]; k < A.v_colind[c + 1]; ++k) w[A.v_row[k]] -= alpha * v[k];
+        sub_axpy(alpha, v, w, A.v_row, A.v_colind[c], A.v_colind[c + 1]);
       }
       if (active)
         for (int c = 0; c < ncol; ++c) xs[c * A.x_s] = w[A.prinv[c]];
@@ -213,10 +258,9 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
       for (int c = 0; c < nrow; ++c) w[c] = 0;
       for (int c = 0; c < ncol; ++c) w[A.prinv[c]] = xs[c * A.x_s];
       for (int c = 0; c < ncol; ++c) {
-        double alpha = 0;
-        for (int k = A.v_colind[c]; k < A.v_colind[c + 1]; ++k) alpha += v[k] * w[A.v_row[k]];
+        double alpha = add_dot(0.0, v, w, A.v_row, A.v_colind[c], A.v_colind[c + 1]);
         alpha *= beta[c];
-        for (int k = A.v_colind[c]; k < A.v_colind[c + 1]; ++k) w[A.v_row[k]] -= alpha * v[k];
+        sub_axpy(alpha, v, w, A.v_row, A.v_colind[c], A.v_colind[c + 1]);
       }
       for (int c = ncol - 1; c >= 0; --c) {
         for (int k = A.r_colind[c + 1] - 1; k >= A.r_colind[c]; --k) {
