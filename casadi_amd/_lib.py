@@ -106,6 +106,9 @@ cxh_mapaccum_destroy = _fn("cxh_mapaccum_destroy", I32, P)
 cxh_mapaccum_eval = _fn("cxh_mapaccum_eval", I32, P, I64, PP, PP, I32, I64, P)
 cxh_mapaccum_fwd_adj = _fn("cxh_mapaccum_fwd_adj", I32, P, I64, PP, PP, PP, PP, I32, I64, I64, P)
 cxh_mapaccum_source = _fn("cxh_mapaccum_source", I32, P, I32, I64, C.c_char_p, I64, PI64)
+cxh_mapaccum_ckpt_size = _fn("cxh_mapaccum_ckpt_size", I32, P, I64, I64, PI64)
+cxh_mapaccum_fwd_ckpt = _fn("cxh_mapaccum_fwd_ckpt", I32, P, I64, PP, PP, I32, I64, I64, P, P)
+cxh_mapaccum_rev_ckpt = _fn("cxh_mapaccum_rev_ckpt", I32, P, I64, PP, PP, PP, I32, I64, I64, P, P)
 cxh_batch_source = _opt("cxh_batch_source", I32, I32, PP, C.c_char_p, I64, PI64)
 
 # --- host symbolic layer ------------------------------------------------------

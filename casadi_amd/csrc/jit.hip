@@ -1038,14 +1038,20 @@ static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
   return o;
 }
 
-// Kernels over n instances.  Scratch (A.st_out, [row][lds]): S*ns checkpoint
-// rows, then K*ns rows for the states of one segment's steps.
-static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, bool adjoint, int64_t K) {
+// Kernels over n instances.  mode 0: forward chain; 1: fused forward +
+// adjoint; 2: forward storing the checkpoints; 3: adjoint from stored
+// checkpoints.  Checkpoints (S = ceil(N/K) rows of the state, [row][lds]) at
+// A.st_in; the states of one segment's steps (K rows) at A.st_out.
+static const char* const kMaccKernel[] = {"cxj_macc_fwd", "cxj_macc_adj", "cxj_macc_fwd_ckpt", "cxj_macc_rev_ckpt"};
+
+static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, int64_t K) {
   std::string o = kSxOpsSource;
   o += kPrologue;
   o += gen_macc_step_fwd(t, M);
-  const std::string ns = std::to_string(std::max<int64_t>(M.ns, 1));
-  const std::string N = std::to_string(M.N);
+  if (mode == 1 || mode == 3) o += gen_macc_step_rev(t, M);
+  const std::string ns = std::to_string(std::max<int64_t>(M.ns, 1)), nsi = std::to_string(M.ns);
+  const std::string N = std::to_string(M.N), Ks = std::to_string(K);
+  const int64_t S = (M.N + K - 1) / K;
   std::string load_x0, store_sens;
   for (int i = 0; i < M.n_accum; ++i)
     for (int64_t j = 0; j < t->nnz_in[i]; ++j) {
@@ -1053,54 +1059,43 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, bool adjoint
       load_x0 += "  s[" + sj + "] = CXJ_IN(" + std::to_string(i) + ", " + std::to_string(j) + ");\n";
       store_sens += "  CXJ_SENS(" + std::to_string(i) + ", " + std::to_string(j) + ", as[" + sj + "]);\n";
     }
-  if (!adjoint) {
-    o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
-         "  double s[" + ns + "], sn[" + ns + "];\n" + load_x0 +
-         "#pragma unroll 1\n"
-         "  for (long long step = 0; step < " + N + "; ++step) {\n"
-         "    cxj_step_fwd(A, k, step, s, sn, true);\n"
-         "    for (int j = 0; j < " + std::to_string(M.ns) + "; ++j) s[j] = sn[j];\n"
-         "  }\n}\n"
-         "extern \"C\" __global__ void __launch_bounds__(256) cxj_macc_fwd(const CxjArgs A) {\n"
-         "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-         "  if (k < A.n) cxj_body(A, k);\n"
-         "}\n";
-    return o;
-  }
-  o += gen_macc_step_rev(t, M);
-  const int64_t S = (M.N + K - 1) / K;
-  const std::string Ks = std::to_string(K), Ss = std::to_string(S), nsi = std::to_string(M.ns);
   o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
-       "  double s[" + ns + "], sn[" + ns + "], as[" + ns + "];\n" + load_x0 +
-       "#pragma unroll 1\n"
-       "  for (long long step = 0; step < " + N + "; ++step) {\n"
-       "    if (step % " + Ks + " == 0)\n"
-       "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n"
-       "    cxj_step_fwd(A, k, step, s, sn, true);\n"
-       "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-       "  }\n"
-       "  for (int j = 0; j < " + nsi + "; ++j) as[j] = 0.0;\n"
-       "#pragma unroll 1\n"
-       "  for (long long seg = " + Ss + " - 1; seg >= 0; --seg) {\n"
-       "    const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
-       "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, seg * " + nsi + " + j);\n"
-       "#pragma unroll 1\n"
-       "    for (long long step = t0; step < t1; ++step) {\n"
-       "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, " + std::to_string(S * M.ns) +
-       " + (step - t0) * " + nsi + " + j) = s[j];\n"
-       "      if (step + 1 < t1) {\n"
-       "        cxj_step_fwd(A, k, step, s, sn, false);\n"
-       "        for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-       "      }\n"
-       "    }\n"
-       "#pragma unroll 1\n"
-       "    for (long long step = t1 - 1; step >= t0; --step) {\n"
-       "      for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, " + std::to_string(S * M.ns) +
-       " + (step - t0) * " + nsi + " + j);\n"
-       "      cxj_step_rev(A, k, step, s, as);\n"
-       "    }\n"
-       "  }\n" + store_sens + "}\n"
-       "extern \"C\" __global__ void __launch_bounds__(256) cxj_macc_adj(const CxjArgs A) {\n"
+       "  double s[" + ns + "], sn[" + ns + "], as[" + ns + "];\n"
+       "  double* ck = const_cast<double*>(A.st_in);\n";
+  if (mode != 3) {  // forward chain
+    o += load_x0 +
+         "#pragma unroll 1\n"
+         "  for (long long step = 0; step < " + N + "; ++step) {\n";
+    if (mode != 0)
+      o += "    if (step % " + Ks + " == 0)\n"
+           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n";
+    o += "    cxj_step_fwd(A, k, step, s, sn, true);\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+         "  }\n";
+  }
+  if (mode == 1 || mode == 3) {  // reverse over the segments
+    o += "  for (int j = 0; j < " + nsi + "; ++j) as[j] = 0.0;\n"
+         "#pragma unroll 1\n"
+         "  for (long long seg = " + std::to_string(S) + " - 1; seg >= 0; --seg) {\n"
+         "    const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(ck, seg * " + nsi + " + j);\n"
+         "#pragma unroll 1\n"
+         "    for (long long step = t0; step < t1; ++step) {\n"
+         "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j) = s[j];\n"
+         "      if (step + 1 < t1) {\n"
+         "        cxj_step_fwd(A, k, step, s, sn, false);\n"
+         "        for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+         "      }\n"
+         "    }\n"
+         "#pragma unroll 1\n"
+         "    for (long long step = t1 - 1; step >= t0; --step) {\n"
+         "      for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j);\n"
+         "      cxj_step_rev(A, k, step, s, as);\n"
+         "    }\n"
+         "  }\n" + store_sens;
+  }
+  o += std::string("}\nextern \"C\" __global__ void __launch_bounds__(256) ") + kMaccKernel[mode] +
+       "(const CxjArgs A) {\n"
        "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
        "  if (k < A.n) cxj_body(A, k);\n"
        "}\n";
@@ -1154,8 +1149,7 @@ struct cxh_mapaccum_s {
   cxh_tape_s* tape = nullptr;
   sxh::MaccPlan plan;
   std::vector<int64_t> nnz_in, nnz_out;  // of the mapaccum Function
-  hipFunction_t fwd = nullptr;
-  std::map<int64_t, hipFunction_t> adj;  // by checkpoint interval
+  std::map<std::pair<int, int64_t>, hipFunction_t> fn;  // by (mode, checkpoint interval)
   sxh::DeviceBuffer scratch;
 };
 
@@ -1258,7 +1252,8 @@ int cxh_mapaccum_source(cxh_mapaccum m, int32_t adjoint, int64_t checkpoint_step
   CXH_TRY
   if (!m) return cxh_fail("null mapaccum");
   const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
-  const std::string src = gen_macc(m->tape, m->plan, adjoint != 0, K);
+  if (adjoint < 0 || adjoint > 3) return cxh_fail("cxh_mapaccum_source: mode must be 0..3");
+  const std::string src = gen_macc(m->tape, m->plan, adjoint, K);
   if (len) *len = static_cast<int64_t>(src.size());
   if (buf && cap > 0) {
     const size_t n = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
@@ -1269,30 +1264,33 @@ int cxh_mapaccum_source(cxh_mapaccum m, int32_t adjoint, int64_t checkpoint_step
   CXH_CATCH
 }
 
-static int macc_launch(cxh_mapaccum m, bool adjoint, int64_t n, const double* const* arg,
+static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const* arg,
                        const double* const* adj_out, double* const* res, double* const* adj_in, int32_t layout,
-                       int64_t ld, int64_t checkpoint_steps, void* stream) {
+                       int64_t ld, int64_t checkpoint_steps, double* ckpt_ext, void* stream) {
   if (!m) return cxh_fail("null mapaccum");
   if (n < 0) return cxh_fail("negative n");
   if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
   if (n == 0) return 0;
   const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
-  hipFunction_t& f = adjoint ? m->adj[K] : m->fwd;
-  if (!f) f = jit_function(gen_macc(m->tape, m->plan, adjoint, K), adjoint ? "cxj_macc_adj" : "cxj_macc_fwd");
+  hipFunction_t& f = m->fn[{mode, mode == 0 ? 0 : K}];
+  if (!f) f = jit_function(gen_macc(m->tape, m->plan, mode, K), kMaccKernel[mode]);
   const int64_t lds = round_up(n, 64);
+  const int64_t S = (m->plan.N + K - 1) / K;
+  const int64_t ns = std::max<int64_t>(m->plan.ns, 1);
   CxjArgs A{};
   A.arg = make_io(arg, m->nnz_in, layout, ld);
   A.res = make_io(res, m->nnz_out, layout, ld);
+  A.aseed = make_io(adj_out, m->nnz_out, layout, ld);
+  A.asens = make_io(adj_in, m->nnz_in, layout, ld);
   A.n = n;
   A.lds = lds;
-  if (adjoint) {
-    A.aseed = make_io(adj_out, m->nnz_out, layout, ld);
-    A.asens = make_io(adj_in, m->nnz_in, layout, ld);
-    const int64_t S = (m->plan.N + K - 1) / K;
-    const int64_t rows = std::max<int64_t>((S + K) * m->plan.ns, 1);
-    m->scratch.reserve(static_cast<size_t>(rows) * lds * sizeof(double));
-    A.st_out = m->scratch.as<double>();
+  if (mode != 0) {
+    const int64_t rows = (ckpt_ext ? 0 : S * ns) + (mode == 2 ? 0 : K * ns);
+    m->scratch.reserve(static_cast<size_t>(std::max<int64_t>(rows, 1)) * lds * sizeof(double));
+    double* base = m->scratch.as<double>();
+    A.st_in = ckpt_ext ? ckpt_ext : base;
+    A.st_out = ckpt_ext ? base : base + S * ns * lds;
   }
   launch(f, n, m->tape->block, A, static_cast<hipStream_t>(stream));
   return 0;
@@ -1301,7 +1299,7 @@ static int macc_launch(cxh_mapaccum m, bool adjoint, int64_t n, const double* co
 int cxh_mapaccum_eval(cxh_mapaccum m, int64_t n, const double* const* arg, double* const* res, int32_t layout,
                       int64_t ld, void* stream) {
   CXH_TRY
-  return macc_launch(m, false, n, arg, nullptr, res, nullptr, layout, ld, 0, stream);
+  return macc_launch(m, 0, n, arg, nullptr, res, nullptr, layout, ld, 0, nullptr, stream);
   CXH_CATCH
 }
 
@@ -1309,7 +1307,32 @@ int cxh_mapaccum_fwd_adj(cxh_mapaccum m, int64_t n, const double* const* arg, co
                          double* const* res, double* const* adj_in, int32_t layout, int64_t ld,
                          int64_t checkpoint_steps, void* stream) {
   CXH_TRY
-  return macc_launch(m, true, n, arg, adj_out, res, adj_in, layout, ld, checkpoint_steps, stream);
+  return macc_launch(m, 1, n, arg, adj_out, res, adj_in, layout, ld, checkpoint_steps, nullptr, stream);
+  CXH_CATCH
+}
+
+int cxh_mapaccum_ckpt_size(cxh_mapaccum m, int64_t n, int64_t checkpoint_steps, int64_t* n_doubles) {
+  if (!m || !n_doubles) return cxh_fail("null argument");
+  const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
+  *n_doubles = (m->plan.N + K - 1) / K * std::max<int64_t>(m->plan.ns, 1) * round_up(std::max<int64_t>(n, 1), 64);
+  return 0;
+}
+
+int cxh_mapaccum_fwd_ckpt(cxh_mapaccum m, int64_t n, const double* const* arg, double* const* res,
+                          int32_t layout, int64_t ld, int64_t checkpoint_steps, double* ckpt, void* stream) {
+  CXH_TRY
+  if (!ckpt) return cxh_fail("null checkpoint buffer");
+  return macc_launch(m, 2, n, arg, nullptr, res, nullptr, layout, ld, checkpoint_steps, ckpt, stream);
+  CXH_CATCH
+}
+
+int cxh_mapaccum_rev_ckpt(cxh_mapaccum m, int64_t n, const double* const* arg, const double* const* adj_out,
+                          double* const* adj_in, int32_t layout, int64_t ld, int64_t checkpoint_steps,
+                          const double* ckpt, void* stream) {
+  CXH_TRY
+  if (!ckpt) return cxh_fail("null checkpoint buffer");
+  return macc_launch(m, 3, n, arg, adj_out, nullptr, adj_in, layout, ld, checkpoint_steps, const_cast<double*>(ckpt),
+                     stream);
   CXH_CATCH
 }
 

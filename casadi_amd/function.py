@@ -364,8 +364,13 @@ class Function:
     # -- batching ---------------------------------------------------------------
     def to_torch(self, checkpoint_every=0, engine="jit"):
         """Batched GPU evaluation as a differentiable torch op whose backward
-        is the checkpointed reverse sweep (casadi_amd.autograd.TorchFunction)."""
+        is the checkpointed reverse sweep (casadi_amd.autograd.TorchFunction).
+        checkpoint_every counts tape instructions, or steps for a Function
+        built by mapaccum (which then runs as loop kernels)."""
         from .autograd import TorchFunction
+        if getattr(self, "macc_base", None) is not None and engine == "jit":
+            from .hip import HipMapAccum
+            return TorchFunction(self, checkpoint_every, engine, tape=HipMapAccum.from_function(self))
         return TorchFunction(self, checkpoint_every, engine)
 
     def map(self, n, parallelization="serial", max_num_threads=None, opts=None):

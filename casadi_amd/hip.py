@@ -325,12 +325,40 @@ class HipMapAccum:
             L.cxh_mapaccum_destroy(h)
             self._h = None
 
-    def source(self, adjoint=False, checkpoint_steps=0):
+    def source(self, mode=0, checkpoint_steps=0):
+        """Generated kernel source: mode 0 forward chain, 1 fused forward +
+        adjoint, 2 forward storing checkpoints, 3 adjoint from them."""
         n = L.I64()
-        L.check(L.cxh_mapaccum_source(self._h, int(adjoint), int(checkpoint_steps), None, 0, C.byref(n)))
+        L.check(L.cxh_mapaccum_source(self._h, int(mode), int(checkpoint_steps), None, 0, C.byref(n)))
         buf = C.create_string_buffer(n.value + 1)
-        L.check(L.cxh_mapaccum_source(self._h, int(adjoint), int(checkpoint_steps), buf, n.value + 1, C.byref(n)))
+        L.check(L.cxh_mapaccum_source(self._h, int(mode), int(checkpoint_steps), buf, n.value + 1, C.byref(n)))
         return buf.value.decode()
+
+    def ckpt_size(self, n, checkpoint_steps):
+        v = L.I64()
+        L.check(L.cxh_mapaccum_ckpt_size(self._h, n, int(checkpoint_steps), C.byref(v)))
+        return v.value
+
+    def fwd_ckpt(self, args, n, checkpoint_steps, stream=None):
+        """Outputs, and the states every checkpoint_steps steps (for rev_ckpt)."""
+        torch = _torch()
+        like = next(a for a in args if a is not None)
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        outs = self._alloc(self.nnz_out, n, like)
+        ckpt = torch.empty(max(self.ckpt_size(n, checkpoint_steps), 1), dtype=torch.float64, device=like.device)
+        L.check(L.cxh_mapaccum_fwd_ckpt(self._h, n, _ptrs(args), _ptrs(outs), CXH_SOA, ld, int(checkpoint_steps),
+                                        ckpt.data_ptr(), _stream_ptr(stream)), "cxh_mapaccum_fwd_ckpt")
+        return outs, ckpt
+
+    def rev_ckpt(self, args, seeds, n, checkpoint_steps, ckpt, stream=None):
+        like = next(a for a in args if a is not None)
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        _check_soa(seeds, self.nnz_out, n, "adj_out")
+        adj = self._alloc(self.nnz_in, n, like)
+        L.check(L.cxh_mapaccum_rev_ckpt(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(adj), CXH_SOA, ld,
+                                        int(checkpoint_steps), ckpt.data_ptr(), _stream_ptr(stream)),
+                "cxh_mapaccum_rev_ckpt")
+        return adj
 
     def _alloc(self, nnz, n, like):
         torch = _torch()

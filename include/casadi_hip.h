@@ -207,9 +207,19 @@ int cxh_mapaccum_eval(cxh_mapaccum m, int64_t n, const double* const* arg, doubl
 int cxh_mapaccum_fwd_adj(cxh_mapaccum m, int64_t n, const double* const* arg,
                          const double* const* adj_out, double* const* res, double* const* adj_in,
                          int32_t layout, int64_t ld, int64_t checkpoint_steps, void* stream);
-/* Introspection: generated source of the forward (adjoint = 0) or fused
- * forward + adjoint kernel.                                                */
-int cxh_mapaccum_source(cxh_mapaccum m, int32_t adjoint, int64_t checkpoint_steps, char* buf,
+/* Split adjoint (for autograd): the forward chain storing the state every
+ * checkpoint_steps steps into a caller-owned device buffer of
+ * cxh_mapaccum_ckpt_size doubles, then the adjoint restarting from it; same
+ * results as cxh_mapaccum_fwd_adj.                                          */
+int cxh_mapaccum_ckpt_size(cxh_mapaccum m, int64_t n, int64_t checkpoint_steps, int64_t* n_doubles);
+int cxh_mapaccum_fwd_ckpt(cxh_mapaccum m, int64_t n, const double* const* arg, double* const* res,
+                          int32_t layout, int64_t ld, int64_t checkpoint_steps, double* ckpt, void* stream);
+int cxh_mapaccum_rev_ckpt(cxh_mapaccum m, int64_t n, const double* const* arg,
+                          const double* const* adj_out, double* const* adj_in, int32_t layout,
+                          int64_t ld, int64_t checkpoint_steps, const double* ckpt, void* stream);
+/* Introspection: generated source of a kernel: mode 0 forward chain, 1 fused
+ * forward + adjoint, 2 forward storing checkpoints, 3 adjoint from them.    */
+int cxh_mapaccum_source(cxh_mapaccum m, int32_t mode, int64_t checkpoint_steps, char* buf,
                         int64_t cap, int64_t* len);
 
 /* Batched sparse LDL^T on CasADi's symbolic factorisation.

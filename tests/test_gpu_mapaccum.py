@@ -72,3 +72,29 @@ def test_mapaccum_rollout_matches_flat_tape(cuda):
     per = np.max(np.abs(g - gf) / np.maximum(np.max(np.abs(gf), axis=1, keepdims=True), 1.0), axis=1)
     print(f"[mapaccum] vs flat tape: median {np.median(per):.2e} max {per.max():.2e}", flush=True)
     assert np.median(per) <= TOL
+
+
+def test_mapaccum_behind_autograd(cuda):
+    """Function.mapaccum(...).to_torch: forward stores the state every 10
+    steps, backward runs the loop adjoint from those checkpoints; equal to the
+    fused launch bit for bit."""
+    import torch
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipMapAccum
+    N, n = 100, 4096
+    F = W.cartpole_cost_step().mapaccum("roll", N)
+    T = F.to_torch(checkpoint_every=10)
+    rng = np.random.default_rng(seed_of(2))
+    XC0 = torch.tensor(np.vstack([rng.uniform(-1, 1, (4, n)), np.zeros((1, n))]), dtype=torch.float64,
+                       device="cuda", requires_grad=True)
+    U = torch.tensor(rng.uniform(-10, 10, (N, n)), dtype=torch.float64, device="cuda", requires_grad=True)
+    Y = T(XC0, U)
+    last = Y[5 * (N - 1):]
+    cost = last[4] + (last[:4] ** 2).sum(0)          # running cost + |x_N|^2
+    cost.sum().backward()
+    S = torch.zeros((5 * N, n), dtype=torch.float64, device="cuda")
+    S[5 * (N - 1):5 * (N - 1) + 4] = 2 * last[:4].detach()
+    S[5 * (N - 1) + 4] = 1.0
+    (Yf,), (g0, gU) = HipMapAccum.from_function(F).fwd_adj([XC0.detach(), U.detach()], [S], n, checkpoint_steps=10)
+    assert torch.equal(Y.detach(), Yf)
+    assert torch.equal(XC0.grad, g0) and torch.equal(U.grad, gU)

@@ -203,9 +203,12 @@ def test_mapaccum_loop_kernels_run_on_host(K):
     out = np.zeros((5 * N, n))
     g0, gU = np.zeros((5, n)), np.zeros((N, n))
     y = np.zeros((5 * N, n))
+    out2 = np.zeros((5 * N, n))
+    h0, hU = np.zeros((5, n)), np.zeros((N, n))
     with tempfile.TemporaryDirectory() as d:
-        (_, fwd), (_, adj) = _compile([m.source(False), m.source(True, K)], d)
-        scratch = np.zeros((4096, n))
+        (_, fwd), (_, adj), (_, fck), (_, rck) = _compile([m.source(0), m.source(1, K), m.source(2, K),
+                                                           m.source(3, K)], d)
+        ckpt, steps = np.zeros((512, n)), np.zeros((512, n))
         A = CxjArgs()
         A.n, A.lds = n, n
         A.arg = _table([x0, U], n)
@@ -214,9 +217,22 @@ def test_mapaccum_loop_kernels_run_on_host(K):
         A.res = _table([out], n)
         A.aseed = _table([seed], n)
         A.asens = _table([g0, gU], n)
-        A.st_out = scratch.ctypes.data
+        A.st_in, A.st_out = ckpt.ctypes.data, steps.ctypes.data
         adj(C.byref(A), n)
+        # split: forward storing checkpoints, then the adjoint from them
+        ckpt2 = np.zeros((512, n))
+        B = CxjArgs()
+        B.n, B.lds = n, n
+        B.arg = _table([x0, U], n)
+        B.res = _table([out2], n)
+        B.st_in, B.st_out = ckpt2.ctypes.data, steps.ctypes.data
+        fck(C.byref(B), n)
+        B.res = _table([], n)
+        B.aseed = _table([seed], n)
+        B.asens = _table([h0, hU], n)
+        rck(C.byref(B), n)
     (ref,) = O.map_eval(F, n, [x0.T, U.T])
-    assert np.array_equal(y, ref.T) and np.array_equal(out, ref.T)
+    assert np.array_equal(y, ref.T) and np.array_equal(out, ref.T) and np.array_equal(out2, ref.T)
     (_,), (r0, rU) = O.adj_map(F, n, [x0.T, U.T], [seed.T])
     assert O.rel_err(g0, r0.T) <= 1e-13 and O.rel_err(gU, rU.T) <= 1e-13
+    assert np.array_equal(h0, g0) and np.array_equal(hU, gU)
