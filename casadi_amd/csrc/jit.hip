@@ -284,11 +284,45 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   auto Adj = [&](int64_t c) { return adj_name(c, ws); };
   std::vector<int64_t> ox(len, 0), oy(len, 0), vc(len, 0);
   char buf[256];
-  auto emit_fwd = [&](int a, int b, int pass, bool outputs) {
+  // sin and cos of one value: one sincos call, its names by operand code
+  std::unordered_map<int64_t, std::pair<std::string, std::string>> trig;
+  auto trig_pairs = [&](int a, int b, int pass) {
+    std::vector<int64_t> sim = cur;
+    std::unordered_map<int64_t, int> seen;
+    for (int k = a; k < b; ++k) {
+      const Instr& e = t->alg[k];
+      if (e.op == OP_OUTPUT) continue;
+      if (e.op == OP_SIN) seen[sim[e.i1]] |= 1;
+      if (e.op == OP_COS) seen[sim[e.i1]] |= 2;
+      sim[e.i0] = 4 * static_cast<int64_t>(k) + pass;
+    }
+    std::unordered_set<int64_t> both;
+    for (auto& kv : seen)
+      if (kv.second == 3) both.insert(kv.first);
+    return both;
+  };
+  // all_trig: every SIN / COS through sincos (the reverse sweep of this range
+  // reads the complementary value as its partial); else only pairs
+  auto emit_fwd = [&](int a, int b, int pass, bool outputs, bool all_trig = false) {
+    const std::unordered_set<int64_t> pairs = all_trig ? std::unordered_set<int64_t>{} : trig_pairs(a, b, pass);
     for (int k = a; k < b; ++k) {
       const Instr& e = t->alg[k];
       const int64_t code = 4 * static_cast<int64_t>(k) + pass;
       const std::string nm = V(code);
+      if ((e.op == OP_SIN || e.op == OP_COS) && (all_trig || pairs.count(cur[e.i1]))) {
+        const int64_t x = cur[e.i1];
+        auto it = trig.find(x);
+        if (it == trig.end()) {
+          const std::string id = std::to_string(code);
+          it = trig.emplace(x, std::make_pair("ts" + id, "tc" + id)).first;
+          o += "  double ts" + id + ", tc" + id + ";\n  sxh::sxh_sincos(" + V(x) + ", &ts" + id + ", &tc" + id +
+               ");\n";
+        }
+        o += "  const double " + nm + " = " + (e.op == OP_SIN ? it->second.first : it->second.second) + ";\n";
+        ox[k - t0] = oy[k - t0] = x;
+        cur[e.i0] = vc[k - t0] = code;
+        continue;
+      }
       switch (e.op) {
         case OP_CONST:
           o += "  const double " + nm + " = " + lit(e.d) + ";\n";
@@ -382,7 +416,7 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
       }
     }
     in_code[j] = cur;
-    emit_fwd(c[j], c[j + 1], last ? 0 : 1, last);
+    emit_fwd(c[j], c[j + 1], last ? 0 : 1, last, true);
     if (last) {
       if (s + 1 < P.n_seg)
         for (size_t r = 0; r < lout.size(); ++r) {
@@ -420,8 +454,14 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
           const int64_t v = vc[k - t0];
           if (!decl.count(v)) break;  // no contribution reaches this value
           const std::string ks = std::to_string(k), an = Adj(v);
-          o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + V(ox[k - t0]) +
-               ", " + V(oy[k - t0]) + ", " + V(v) + ", d" + ks + ");\n";
+          auto tg = (e.op == OP_SIN || e.op == OP_COS) ? trig.find(ox[k - t0]) : trig.end();
+          if (tg != trig.end()) {  // d sin = cos, d cos = -sin, from the same sincos
+            o += "  const double d" + ks + "[2] = {" +
+                 (e.op == OP_SIN ? tg->second.second : "-" + tg->second.first) + ", 0.0};\n";
+          } else {
+            o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + V(ox[k - t0]) +
+                 ", " + V(oy[k - t0]) + ", " + V(v) + ", d" + ks + ");\n";
+          }
           if (e.op == OP_IF_ELSE_ZERO) {
             acc(oy[k - t0], "sxh::sxh_if_else_zero(d" + ks + "[1], " + an + ")");
           } else if (op_is_binary(e.op)) {
@@ -617,6 +657,42 @@ std::string jit_source(cxh_tape_s* t, int64_t every, int seg, bool rev) {
   return gen_segment(t, P, seg, rev);
 }
 
+// Operand codes (instruction indices) of the values whose sin and cos are
+// both taken in the tape: computed by one sincos call.
+static std::unordered_set<int64_t> step_trig_pairs(const cxh_tape_s* t) {
+  std::vector<int64_t> cur(t->ws, 0);
+  std::unordered_map<int64_t, int> seen;
+  for (size_t k = 0; k < t->alg.size(); ++k) {
+    const Instr& e = t->alg[k];
+    if (e.op == OP_OUTPUT) continue;
+    if (e.op == OP_SIN) seen[cur[e.i1]] |= 1;
+    if (e.op == OP_COS) seen[cur[e.i1]] |= 2;
+    cur[e.i0] = static_cast<int64_t>(k);
+  }
+  std::unordered_set<int64_t> both;
+  for (auto& kv : seen)
+    if (kv.second == 3) both.insert(kv.first);
+  return both;
+}
+
+// Emits `const double v<k> = sin/cos(v<x>)` through sincos (one call per
+// operand, names remembered in trig) and returns true, for SIN / COS when
+// all_trig or the operand is in pairs.
+static bool emit_trig(std::string& o, const Instr& e, int64_t k, int64_t x, const std::string& xname, bool all_trig,
+                      const std::unordered_set<int64_t>& pairs,
+                      std::unordered_map<int64_t, std::pair<std::string, std::string>>& trig) {
+  if ((e.op != OP_SIN && e.op != OP_COS) || !(all_trig || pairs.count(x))) return false;
+  auto it = trig.find(x);
+  if (it == trig.end()) {
+    const std::string id = std::to_string(k);
+    it = trig.emplace(x, std::make_pair("ts" + id, "tc" + id)).first;
+    o += "  double ts" + id + ", tc" + id + ";\n  sxh::sxh_sincos(" + xname + ", &ts" + id + ", &tc" + id + ");\n";
+  }
+  o += "  const double v" + std::to_string(k) + " = " + (e.op == OP_SIN ? it->second.first : it->second.second) +
+       ";\n";
+  return true;
+}
+
 // ------------------------------------------------------ tangent sweeps ---
 //
 // The compressed-Jacobian sweep of XFunction::jac
@@ -770,10 +846,10 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
         const bool bin = op_is_binary(e.op);
         const int64_t y = bin ? cur[e.i2] : x;
         const uint64_t mx = mask[e.i1], my = bin ? mask[e.i2] : 0;
+        const uint64_t m = e.op == OP_IF_ELSE_ZERO ? my : (mx | my);
         std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, %s, %s);\n", k, e.op,
                       V(x).c_str(), V(y).c_str());
         o += buf;
-        const uint64_t m = e.op == OP_IF_ELSE_ZERO ? my : (mx | my);
         if (m) {
           o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + V(x) + ", " + V(y) +
                ", v" + ks + ", d" + ks + ");\n";
@@ -900,9 +976,16 @@ static std::string gen_macc_step_fwd(const cxh_tape_s* t, const MaccPlan& M) {
                   "step, const double* s, double* sn, const bool outs) {\n";
   std::vector<int64_t> cur(t->ws, 0);
   char buf[320];
+  const std::unordered_set<int64_t> pairs = step_trig_pairs(t);
+  std::unordered_map<int64_t, std::pair<std::string, std::string>> trig;
   for (size_t kk = 0; kk < t->alg.size(); ++kk) {
     const Instr& e = t->alg[kk];
     const int k = static_cast<int>(kk);
+    const bool is_trig = e.op == OP_SIN || e.op == OP_COS;
+    if (is_trig && emit_trig(o, e, k, cur[e.i1], "v" + std::to_string(cur[e.i1]), false, pairs, trig)) {
+      cur[e.i0] = k;
+      continue;
+    }
     switch (e.op) {
       case OP_CONST:
         o += "  const double v" + std::to_string(k) + " = " + lit(e.d) + ";\n";
@@ -948,8 +1031,15 @@ static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
                   "step, const double* s, double* as) {\n";
   std::vector<int64_t> cur(t->ws, 0), ox(ni, 0), oy(ni, 0);
   char buf[320];
+  std::unordered_map<int64_t, std::pair<std::string, std::string>> trig;
   for (int k = 0; k < ni; ++k) {
     const Instr& e = t->alg[k];
+    const bool is_trig = e.op == OP_SIN || e.op == OP_COS;
+    if (is_trig && emit_trig(o, e, k, cur[e.i1], "v" + std::to_string(cur[e.i1]), true, {}, trig)) {
+      ox[k] = oy[k] = cur[e.i1];
+      cur[e.i0] = k;
+      continue;
+    }
     switch (e.op) {
       case OP_CONST:
         o += "  const double v" + std::to_string(k) + " = " + lit(e.d) + ";\n";
@@ -1008,8 +1098,14 @@ static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
       default: {
         if (!decl[k]) break;
         const std::string ks = std::to_string(k);
-        o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", v" + std::to_string(ox[k]) +
-             ", v" + std::to_string(oy[k]) + ", v" + ks + ", d" + ks + ");\n";
+        auto tg = (e.op == OP_SIN || e.op == OP_COS) ? trig.find(ox[k]) : trig.end();
+        if (tg != trig.end()) {
+          o += "  const double d" + ks + "[2] = {" +
+               (e.op == OP_SIN ? tg->second.second : "-" + tg->second.first) + ", 0.0};\n";
+        } else {
+          o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", v" + std::to_string(ox[k]) +
+               ", v" + std::to_string(oy[k]) + ", v" + ks + ", d" + ks + ");\n";
+        }
         if (e.op == OP_IF_ELSE_ZERO) {
           acc(oy[k], "sxh::sxh_if_else_zero(d" + ks + "[1], a" + ks + ")");
         } else if (op_is_binary(e.op)) {

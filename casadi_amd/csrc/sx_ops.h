@@ -139,6 +139,17 @@ SXH_HD double sxh_erfinv(double x) {
   return y;
 }
 
+// sin(x) and cos(x) from one argument reduction on the device (the values
+// equal sin(x) and cos(x) computed separately); two calls on the host.
+SXH_HD void sxh_sincos(double x, double* s, double* c) {
+#if defined(__HIPCC_RTC__) || defined(__HIP_DEVICE_COMPILE__)
+  sincos(x, s, c);
+#else
+  *s = sin(x);
+  *c = cos(x);
+#endif
+}
+
 SXH_HD double op_fun_full(int op, double x, double y);
 SXH_HD void op_der_full(int op, double x, double y, double f, double* d);
 
