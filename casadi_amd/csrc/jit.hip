@@ -35,12 +35,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -66,9 +68,10 @@ struct CxjArgs {
   const double* adj_in;
   double* adj_out;
   long long n, lds;
+  double* spill;
 };
 static_assert(sizeof(IOArg) == 24, "IOArg layout");
-static_assert(sizeof(CxjArgs) == 4 * CXH_MAX_IO * 24 + 6 * 8, "CxjArgs layout");
+static_assert(sizeof(CxjArgs) == 4 * CXH_MAX_IO * 24 + 7 * 8, "CxjArgs layout");
 
 static const char kPrologue[] = R"CXJ(
 struct IOArg { const double* p; long long s_nz; long long s_inst; };
@@ -77,6 +80,7 @@ struct CxjArgs {
   IOTable arg, res, aseed, asens;
   const double* st_in; double* st_out; const double* adj_in; double* adj_out;
   long long n, lds;
+  double* spill;
 };
 #define CXJ_AT(t, i, j) ((t).a[i].p + (long long)(j) * (t).a[i].s_nz + k * (t).a[i].s_inst)
 #define CXJ_IN(i, j) ((A.arg.a[i].p) ? *CXJ_AT(A.arg, i, j) : 0.0)
@@ -92,6 +96,9 @@ struct CxjArgs {
 #define CXJ_KEEP(x) asm volatile("" : "+v"(x))
 #define CXJ_KEEP_PTR(x) asm volatile("" : "+s"(x))
 #define CXJ_AFTER(p, x) asm volatile("" : "+s"(p) : "v"(x))
+#define CXJ_GLOBAL __attribute__((address_space(1)))
+#define CXJ_OPAQUE(p) asm volatile("" : "+v"(p))
+#define CXJ_OPAQUE_AFTER(p, x) asm volatile("" : "+v"(p) : "v"(x))
 #endif
 )CXJ";
 
@@ -702,18 +709,21 @@ static bool emit_trig(std::string& o, const Instr& e, int64_t k, int64_t x, cons
 // with the tangents scattered into the Jacobian nonzeros at the OUTPUT
 // instructions.  Tangents that are structurally zero (no seeded input reaches
 // the value) are known at generation time and never computed, as CasADi's
-// symbolic sweep drops structural zeros.  Directions are processed in groups
-// (one set of kernels per group, each recomputing the primal) and long tapes in
-// segments whose state is the live values plus their nonzero tangents.
+// symbolic sweep drops structural zeros.  Directions are processed in groups,
+// each group a body of its own that recomputes the primal; the groups of one
+// segment share a kernel (blockIdx.y picks the body) and run side by side.
+// Long tapes are cut into segments whose state is the live values plus their
+// nonzero tangents.
 
 struct JitJac {
   int group = 1, n_group = 1, n_seg = 1;
   std::vector<int> t0;                                         // segment boundaries
   std::vector<std::vector<int>> live;                          // live slots per boundary
   std::vector<std::vector<std::vector<std::pair<int, int>>>> tnz;  // [g][b]: nonzero (slot, dir)
-  int64_t max_rows = 0;
-  std::vector<std::vector<hipFunction_t>> fn;                  // [g][s]
-  DeviceBuffer st;
+  int64_t max_rows = 0;                                        // boundary state rows
+  int64_t spill_rows = 0;                                      // spill rows (max over kernels)
+  std::vector<hipFunction_t> fn;                               // [s], all groups
+  DeviceBuffer st, spill, zero;
 };
 
 static int jac_dir_group() {
@@ -767,126 +777,488 @@ static JitJac& jac_plan(cxh_jac_s* j) {
     for (int bb = 1; bb < J->n_seg; ++bb)
       J->max_rows = std::max<int64_t>(J->max_rows, static_cast<int64_t>(J->live[bb].size() + J->tnz[g][bb].size()));
   }
-  J->fn.assign(J->n_group, std::vector<hipFunction_t>(J->n_seg, nullptr));
+  J->fn.assign(J->n_seg, nullptr);
   j->jit = J;
   return *J;
 }
 
+// Register budget of one tangent kernel, in doubles held at once.
+static int jac_reg_budget() {
+  if (const char* e = std::getenv("CXH_JIT_TANREGS")) return std::max(8, std::atoi(e));
+  return 96;
+}
+
+// Reloads are issued this many instructions ahead of their use.
+static int jac_prefetch() {
+  if (const char* e = std::getenv("CXH_JIT_PREFETCH")) return std::max(0, std::atoi(e));
+  return 8;
+}
+
 // Kernel of direction group g, segment s.  Block 0 of the asens table carries
 // the Jacobian nonzeros: nonzero q of instance k at p[q*s_nz + k*s_inst].
-static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s) {
+//
+// The straight-line body of a long gradient tape keeps hundreds of values
+// live (a reverse sweep reads forward values from far back), more than the
+// register file holds; left to the compiler they spill to scratch at one wave
+// per SIMD.  The generator therefore allocates registers itself: it walks the
+// segment with at most jac_reg_budget() values resident and, when over budget,
+// evicts the value whose next use is furthest away (Belady), storing it once
+// to a spill row (or straight to its boundary row when it is live out) and
+// reloading it just before its next use.  Segment live-ins and inputs are
+// loaded next to their first use instead of at the top.  Reloads go through a
+// pointer made opaque after the most recently computed value, so the compiler
+// can neither fold them into the store nor hoist them back up.
+//
+// State and spill rows are tiled [instance/64][row][64]: a row's offset is a
+// compile-time constant and accesses of a wave are 512 B contiguous.
+static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s, int* spill_rows) {
   const cxh_tape_s* t = j->tape;
   const int nd = j->ndir;
-  const int d0 = g * J.group, d1 = std::min(nd, d0 + J.group), G = d1 - d0;
+  const int d0 = g * J.group, d1 = std::min(nd, d0 + J.group);
   const int t0 = J.t0[s], t1 = J.t0[s + 1];
-  std::string o = kSxOpsSource;
-  o += kPrologue;
-  o.reserve(o.size() + static_cast<size_t>(t1 - t0) * (80 + 60 * G));
-  o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n";
-  std::vector<int64_t> cur(t->ws, 0);
-  std::vector<uint64_t> mask(t->ws, 0);
-  auto V = [&](int64_t c) { return c >= 0 ? "v" + std::to_string(c) : "w" + std::to_string(-c - 1); };
-  auto T = [&](int64_t c, int d) {
+  const bool last_seg = s + 1 >= J.n_seg;
+
+  // ---- values and the reads of every instruction
+  struct Val {
+    std::string base, name;  // SSA base name, current name (literal text for literals)
+    int kind = 0;            // 0 computed, 1 live-in (state row), 2 input, 3 literal
+    int in_row = -1, ai = 0, aj = 0;
+    int home = -1;           // spill row holding the value once stored
+    int out_row = -1;        // boundary row if live out (its home when stored)
+    std::vector<int> uses;
+    size_t nu = 0;
+    bool resident = false, dirty = false, stored = false;
+    int reloads = 0;
+  };
+  std::vector<Val> val;
+  std::unordered_map<int64_t, int> vid;
+  auto key = [](int64_t code, int d) { return (code + (int64_t(1) << 40)) * 128 + (d + 1); };
+  auto mk = [&](int kind, std::string nm) {
+    Val v;
+    v.kind = kind;
+    v.base = v.name = std::move(nm);
+    val.push_back(std::move(v));
+    return static_cast<int>(val.size()) - 1;
+  };
+  auto add = [&](int64_t code, int d, int kind, std::string nm) {
+    const int id = mk(kind, std::move(nm));
+    vid[key(code, d)] = id;
+    return id;
+  };
+  auto Vn = [](int64_t c) { return c >= 0 ? "v" + std::to_string(c) : "w" + std::to_string(-c - 1); };
+  auto Tn = [](int64_t c, int d) {
     return (c >= 0 ? "u" + std::to_string(c) : "z" + std::to_string(-c - 1)) + "_" + std::to_string(d);
   };
+  struct Op {
+    int k;                    // instruction; t1 marks the segment end
+    std::vector<int> reads;   // deduplicated
+    std::vector<int> defs;    // values defined (with uses or not)
+    int x = -1, y = -1;
+    int trig = 0, s_id = -1, c_id = -1;  // 1: sincos here, 2: value of an earlier sincos
+    std::vector<int> tx, ty, td;  // per direction of the group
+  };
+  std::vector<Op> ops;
+  const int G = d1 - d0;
+  std::vector<int64_t> cur(t->ws, 0);
+  std::vector<uint64_t> mask(t->ws, 0);
+  auto vof = [&](int64_t c, int d) { return vid.at(key(c, d)); };
   const std::vector<int>& lin = J.live[s];
   for (size_t r = 0; r < lin.size(); ++r) {
-    cur[lin[r]] = -(lin[r] + 1);
-    o += "  const double w" + std::to_string(lin[r]) + " = " +
-         (s > 0 ? "CXJ_ST_IN(" + std::to_string(r) + ")" : std::string("0.0")) + ";\n";
+    const int64_t c = -(lin[r] + 1);
+    cur[lin[r]] = c;
+    const int v = add(c, -1, s > 0 ? 1 : 3, s > 0 ? Vn(c) : std::string("0.0"));
+    val[v].in_row = static_cast<int>(r);
   }
   if (s > 0) {
-    int64_t r = static_cast<int64_t>(lin.size());
+    int r = static_cast<int>(lin.size());
     for (const auto& sd : J.tnz[g][s]) {
       mask[sd.first] |= uint64_t(1) << (sd.second - d0);
-      o += "  const double " + T(-(sd.first + 1), sd.second) + " = CXJ_ST_IN(" + std::to_string(r++) + ");\n";
+      const int v = add(-(sd.first + 1), sd.second, 1, Tn(-(sd.first + 1), sd.second));
+      val[v].in_row = r++;
     }
   }
-  char buf[256];
+  // sin and cos of one operand in this segment: one sincos gives both
+  std::unordered_set<int64_t> pair_ops;
+  {
+    std::vector<int64_t> c2(cur);
+    std::unordered_map<int64_t, int> seen;
+    for (int k = t0; k < t1; ++k) {
+      const Instr& e = t->alg[k];
+      if (e.op == OP_OUTPUT) continue;
+      if (e.op == OP_SIN) seen[c2[e.i1]] |= 1;
+      if (e.op == OP_COS) seen[c2[e.i1]] |= 2;
+      c2[e.i0] = k;
+    }
+    for (const auto& kv : seen)
+      if (kv.second == 3) pair_ops.insert(kv.first);
+  }
+  std::unordered_map<int64_t, std::pair<int, int>> sib;  // operand -> (sin, cos) values
   for (int k = t0; k < t1; ++k) {
     const Instr& e = t->alg[k];
-    const std::string ks = std::to_string(k);
     switch (e.op) {
       case OP_CONST:
-        o += "  const double v" + ks + " = " + lit(e.d) + ";\n";
+        add(k, -1, 3, lit(e.d));
         cur[e.i0] = k;
         mask[e.i0] = 0;
         break;
       case OP_INPUT: {
-        std::snprintf(buf, sizeof buf, "  const double v%d = CXJ_IN(%d, %d);\n", k, e.i1, e.i2);
-        o += buf;
+        const int v = add(k, -1, 2, Vn(k));
+        val[v].ai = e.i1;
+        val[v].aj = e.i2;
         uint64_t m = 0;
         const int64_t flat = j->in_off[e.i1] + e.i2;
         for (int d = d0; d < d1; ++d) {
           const double sv = j->seedval[flat * nd + d];
           if (sv != 0.0) {
             m |= uint64_t(1) << (d - d0);
-            o += "  const double " + T(k, d) + " = " + lit(sv) + ";\n";
+            add(k, d, 3, lit(sv));
           }
         }
         cur[e.i0] = k;
         mask[e.i0] = m;
       } break;
       case OP_OUTPUT: {
+        Op op;
+        op.k = k;
         const int64_t c = cur[e.i1];
+        op.x = vof(c, -1);
+        if (g == 0) op.reads.push_back(op.x);
+        op.tx.assign(G, -1);
+        const int64_t flat = j->out_off[e.i0] + e.i2;
+        for (int d = d0; d < d1; ++d) {
+          const int2 tg = j->scatter[flat * nd + d];
+          if ((tg.x >= 0 || tg.y >= 0) && (mask[e.i1] >> (d - d0) & 1)) {
+            op.tx[d - d0] = vof(c, d);
+            op.reads.push_back(op.tx[d - d0]);
+          }
+        }
+        ops.push_back(std::move(op));
+      } break;
+      default: {
+        Op op;
+        op.k = k;
+        const bool bin = op_is_binary(e.op);
+        const int64_t x = cur[e.i1], y = bin ? cur[e.i2] : x;
+        const uint64_t mx = mask[e.i1], my = bin ? mask[e.i2] : 0;
+        const uint64_t m = e.op == OP_IF_ELSE_ZERO ? my : (mx | my);
+        op.x = vof(x, -1);
+        op.y = vof(y, -1);
+        const bool trig = e.op == OP_SIN || e.op == OP_COS;
+        const auto sit = trig ? sib.find(x) : sib.end();
+        if (sit != sib.end()) {  // the value (and partial) come from an earlier sincos
+          op.trig = 2;
+          op.s_id = sit->second.first;
+          op.c_id = sit->second.second;
+          vid[key(k, -1)] = e.op == OP_SIN ? op.s_id : op.c_id;
+          if (m) op.reads.push_back(e.op == OP_SIN ? op.c_id : op.s_id);
+        } else if (trig && (m || pair_ops.count(x))) {
+          op.trig = 1;
+          op.reads.push_back(op.x);
+          op.s_id = mk(0, "ts" + std::to_string(k));
+          op.c_id = mk(0, "tc" + std::to_string(k));
+          vid[key(k, -1)] = e.op == OP_SIN ? op.s_id : op.c_id;
+          if (pair_ops.count(x)) sib[x] = {op.s_id, op.c_id};
+          op.defs = {op.s_id, op.c_id};
+        } else {
+          op.reads.push_back(op.x);
+          if (op.y != op.x) op.reads.push_back(op.y);
+          op.defs.push_back(add(k, -1, 0, Vn(k)));
+        }
+        op.tx.assign(G, -1);
+        op.ty.assign(G, -1);
+        op.td.assign(G, -1);
+        for (int d = d0; d < d1; ++d) {
+          const uint64_t bit = uint64_t(1) << (d - d0);
+          if (!(m & bit)) continue;
+          if (e.op != OP_IF_ELSE_ZERO && (mx & bit)) op.tx[d - d0] = vof(x, d);
+          if ((e.op == OP_IF_ELSE_ZERO || bin) && (my & bit)) op.ty[d - d0] = vof(y, d);
+          for (int q : {op.tx[d - d0], op.ty[d - d0]})
+            if (q >= 0 && std::find(op.reads.begin(), op.reads.end(), q) == op.reads.end()) op.reads.push_back(q);
+          op.td[d - d0] = add(k, d, 0, Tn(k, d));
+          op.defs.push_back(op.td[d - d0]);
+        }
+        cur[e.i0] = k;
+        mask[e.i0] = m;
+        ops.push_back(std::move(op));
+      }
+    }
+  }
+  // the end: live-out values (state rows of the next segment)
+  std::vector<std::pair<int, int>> outs;  // (row, value id or -1 for a structural zero)
+  if (!last_seg) {
+    Op op;
+    op.k = t1;
+    int r = 0;
+    for (int slot : J.live[s + 1]) outs.push_back({r++, vof(cur[slot], -1)});
+    for (const auto& sd : J.tnz[g][s + 1]) {
+      const bool nzv = mask[sd.first] >> (sd.second - d0) & 1;
+      outs.push_back({r++, nzv ? vof(cur[sd.first], sd.second) : -1});
+    }
+    for (auto& ro : outs)
+      if (ro.second >= 0 && val[ro.second].kind != 3) {
+        if (val[ro.second].out_row < 0) val[ro.second].out_row = ro.first;
+        if (std::find(op.reads.begin(), op.reads.end(), ro.second) == op.reads.end()) op.reads.push_back(ro.second);
+      }
+    ops.push_back(std::move(op));
+  }
+  for (size_t i = 0; i < ops.size(); ++i)
+    for (int v : ops[i].reads)
+      if (val[v].kind != 3) val[v].uses.push_back(static_cast<int>(i));
+
+  // ---- allocation and emission
+  const int R = jac_reg_budget();
+  const int LA = jac_prefetch();
+  std::string b;  // body
+  b.reserve(static_cast<size_t>(t1 - t0) * (90 + 70 * G));
+  std::set<std::pair<int, int>> res;  // (next use, value) of resident values
+  std::vector<int> free_rows;
+  int n_spill = 0;
+  std::string prev;  // most recently computed value, orders the reloads after it
+  auto next_use = [&](const Val& v) { return v.nu < v.uses.size() ? v.uses[v.nu] : INT32_MAX; };
+  int n_ptr = 0;
+  bool use_si = false, use_so = false, use_ss = false;
+  std::set<int> use_arg;
+  // Every group of accesses goes through its own opaque copy of the
+  // per-instance base pointer: the row offset is then an immediate or a
+  // short-lived add, never an index shared by (and kept live between) all
+  // accesses of one row.
+  auto opaque = [&](const char* base, bool is_const, const std::string& after) {
+    const std::string p = "cxj_p" + std::to_string(n_ptr++);
+    b += std::string("  CXJ_GLOBAL ") + (is_const ? "const " : "") + "double* " + p + " = " + base + ";\n";
+    b += after.empty() ? "  CXJ_OPAQUE(" + p + ");\n" : "  CXJ_OPAQUE_AFTER(" + p + ", " + after + ");\n";
+    return p;
+  };
+  std::vector<std::pair<int, std::string>> st_rows, sp_rows;  // pending stores
+  auto flush = [&]() {
+    if (!st_rows.empty()) {
+      use_so = true;
+      const std::string p = opaque("cxj_so", false, "");
+      for (const auto& x : st_rows) b += "  " + p + "[" + std::to_string(64 * x.first) + "] = " + x.second + ";\n";
+      st_rows.clear();
+    }
+    if (!sp_rows.empty()) {
+      use_ss = true;
+      const std::string p = opaque("cxj_ss", false, "");
+      for (const auto& x : sp_rows) b += "  " + p + "[" + std::to_string(64 * x.first) + "] = " + x.second + ";\n";
+      sp_rows.clear();
+    }
+  };
+  auto evict = [&](int id) {
+    Val& v = val[id];
+    res.erase({next_use(v), id});
+    v.resident = false;
+    if (!v.dirty) return;
+    v.dirty = false;
+    v.stored = true;
+    if (v.out_row >= 0) {
+      st_rows.push_back({v.out_row, v.name});
+    } else {
+      if (free_rows.empty()) free_rows.push_back(n_spill++);
+      v.home = free_rows.back();
+      free_rows.pop_back();
+      sp_rows.push_back({v.home, v.name});
+    }
+  };
+  auto over_budget = [&](int not_before) {
+    while (static_cast<int>(res.size()) > R) {
+      const auto it = std::prev(res.end());
+      if (it->first <= not_before) break;
+      evict(it->second);
+    }
+    flush();
+  };
+  // bring the values of `ids` into registers, loads ordered after `prev`
+  auto reload = [&](const std::vector<int>& ids) {
+    std::string p_st, p_sp, p_in;
+    std::map<int, std::string> p_arg;
+    auto ptr = [&](std::string& p, const std::string& base) {
+      if (p.empty()) p = opaque(base.c_str(), true, prev);
+      return p;
+    };
+    for (int id : ids) {
+      Val& v = val[id];
+      if (v.kind == 3 || v.resident) continue;
+      const std::string nm = v.base + "_" + std::to_string(++v.reloads);
+      std::string load;
+      if (v.stored && v.out_row >= 0) {
+        use_so = true;
+        load = ptr(p_st, "cxj_so") + "[" + std::to_string(64 * v.out_row) + "]";
+      } else if (v.stored) {
+        use_ss = true;
+        load = ptr(p_sp, "cxj_ss") + "[" + std::to_string(64 * v.home) + "]";
+      } else if (v.kind == 1) {
+        use_si = true;
+        load = ptr(p_in, "cxj_si") + "[" + std::to_string(64 * v.in_row) + "]";
+      } else if (v.kind == 2) {
+        use_arg.insert(v.ai);
+        load = ptr(p_arg[v.ai], "cxj_a" + std::to_string(v.ai)) + "[" + std::to_string(v.aj) + " * A.arg.a[" +
+               std::to_string(v.ai) + "].s_nz]";
+      } else {
+        throw std::logic_error("tangent codegen: value neither resident nor stored");
+      }
+      b += "  const double " + nm + " = " + load + ";\n";
+      v.name = nm;
+      v.resident = true;
+      res.insert({next_use(v), id});
+    }
+  };
+  char buf[256];
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const Op& op = ops[i];
+    const int ii = static_cast<int>(i);
+    if (op.k == t1) {  // values already in their boundary row stay there
+      std::vector<int> need;
+      for (const auto& ro : outs)
+        if (ro.second >= 0 && !(val[ro.second].stored && val[ro.second].out_row == ro.first)) need.push_back(ro.second);
+      reload(need);
+    } else {
+      // this instruction's operands, and those of the one LA ahead (the
+      // reload is then in flight while the instructions between execute)
+      std::vector<int> need(op.reads);
+      const size_t ahead = i + static_cast<size_t>(LA);
+      if (LA > 0 && ahead < ops.size() && ops[ahead].k != t1)
+        for (int id : ops[ahead].reads)  // only what memory already holds
+          if (val[id].kind == 1 || val[id].kind == 2 || val[id].stored) need.push_back(id);
+      reload(need);
+    }
+    over_budget(ii + LA);
+    if (op.k == t1) {  // segment end: boundary rows
+      for (const auto& ro : outs) {
+        const std::string rr = std::to_string(ro.first);
+        if (ro.second < 0) {
+          st_rows.push_back({ro.first, "0.0"});
+          continue;
+        }
+        const Val& v = val[ro.second];
+        if (v.stored && v.out_row == ro.first) continue;  // already there
+        st_rows.push_back({ro.first, v.name});
+      }
+      flush();
+    } else {
+      const Instr& e = t->alg[op.k];
+      const std::string ks = std::to_string(op.k);
+      if (e.op == OP_OUTPUT) {
         if (g == 0) {
-          std::snprintf(buf, sizeof buf, "  CXJ_OUT(%d, %d, %s);\n", e.i0, e.i2, V(c).c_str());
-          o += buf;
+          std::snprintf(buf, sizeof buf, "  CXJ_OUT(%d, %d, %s);\n", e.i0, e.i2, val[op.x].name.c_str());
+          b += buf;
         }
         const int64_t flat = j->out_off[e.i0] + e.i2;
         for (int d = d0; d < d1; ++d) {
           const int2 tg = j->scatter[flat * nd + d];
-          const std::string v = (mask[e.i1] >> (d - d0) & 1) ? T(c, d) : std::string("0.0");
-          if (tg.x >= 0) o += "  CXJ_SENS(0, " + std::to_string(tg.x) + ", " + v + ");\n";
-          if (tg.y >= 0) o += "  CXJ_SENS(0, " + std::to_string(tg.y) + ", " + v + ");\n";
+          const std::string v = op.tx[d - d0] >= 0 ? val[op.tx[d - d0]].name : std::string("0.0");
+          if (tg.x >= 0) b += "  CXJ_SENS(0, " + std::to_string(tg.x) + ", " + v + ");\n";
+          if (tg.y >= 0) b += "  CXJ_SENS(0, " + std::to_string(tg.y) + ", " + v + ");\n";
         }
-      } break;
-      default: {
-        const int64_t x = cur[e.i1];
-        const bool bin = op_is_binary(e.op);
-        const int64_t y = bin ? cur[e.i2] : x;
-        const uint64_t mx = mask[e.i1], my = bin ? mask[e.i2] : 0;
-        const uint64_t m = e.op == OP_IF_ELSE_ZERO ? my : (mx | my);
-        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, %s, %s);\n", k, e.op,
-                      V(x).c_str(), V(y).c_str());
-        o += buf;
-        if (m) {
-          o += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + V(x) + ", " + V(y) +
-               ", v" + ks + ", d" + ks + ");\n";
+      } else {
+        const std::string& X = val[op.x].name;
+        const std::string& Y = val[op.y].name;
+        if (op.trig == 1) {
+          b += "  double ts" + ks + ", tc" + ks + ";\n  sxh::sxh_sincos(" + X + ", &ts" + ks + ", &tc" + ks + ");\n";
+        } else if (op.trig == 0) {
+          b += "  const double v" + ks + " = sxh::op_fun_full(" + std::to_string(e.op) + ", " + X + ", " + Y + ");\n";
+        }
+        bool any = false;
+        for (int q : op.td) any |= q >= 0;
+        if (any && op.trig) {  // d sin = cos, d cos = -sin, from the sincos
+          b += "  const double d" + ks + "[2] = {" +
+               (e.op == OP_SIN ? val[op.c_id].name : "-" + val[op.s_id].name) + ", 0.0};\n";
+        } else if (any) {
+          b += "  double d" + ks + "[2];\n  sxh::op_der_full(" + std::to_string(e.op) + ", " + X + ", " + Y + ", v" +
+               ks + ", d" + ks + ");\n";
+        }
+        if (any) {
           for (int d = d0; d < d1; ++d) {
-            const uint64_t bit = uint64_t(1) << (d - d0);
-            if (!(m & bit)) continue;
+            const int q = op.td[d - d0];
+            if (q < 0) continue;
+            const int a = op.tx[d - d0], c = op.ty[d - d0];
             std::string rhs;
             if (e.op == OP_IF_ELSE_ZERO) {
-              rhs = "sxh::sxh_if_else_zero(d" + ks + "[1], " + T(y, d) + ")";
-            } else if (bin && (mx & bit) && (my & bit)) {
-              rhs = "d" + ks + "[0] * " + T(x, d) + " + d" + ks + "[1] * " + T(y, d);
-            } else if (mx & bit) {
-              rhs = "d" + ks + "[0] * " + T(x, d);
+              rhs = "sxh::sxh_if_else_zero(d" + ks + "[1], " + val[c].name + ")";
+            } else if (a >= 0 && c >= 0) {
+              rhs = "d" + ks + "[0] * " + val[a].name + " + d" + ks + "[1] * " + val[c].name;
+            } else if (a >= 0) {
+              rhs = "d" + ks + "[0] * " + val[a].name;
             } else {
-              rhs = "d" + ks + "[1] * " + T(y, d);
+              rhs = "d" + ks + "[1] * " + val[c].name;
             }
-            o += "  const double " + T(k, d) + " = " + rhs + ";\n";
+            b += "  const double " + val[q].name + " = " + rhs + ";\n";
           }
         }
-        cur[e.i0] = k;
-        mask[e.i0] = m;
       }
     }
-  }
-  if (s + 1 < J.n_seg) {
-    const std::vector<int>& lout = J.live[s + 1];
-    int64_t r = 0;
-    for (int slot : lout) o += "  CXJ_ST_OUT(" + std::to_string(r++) + ") = " + V(cur[slot]) + ";\n";
-    for (const auto& sd : J.tnz[g][s + 1]) {
-      const bool nzv = mask[sd.first] >> (sd.second - d0) & 1;
-      o += "  CXJ_ST_OUT(" + std::to_string(r++) + ") = " + (nzv ? T(cur[sd.first], sd.second) : "0.0") + ";\n";
+    // reads advance; dead values leave the registers and free their rows
+    for (int id : op.reads) {
+      Val& v = val[id];
+      if (v.kind == 3) continue;
+      res.erase({next_use(v), id});
+      ++v.nu;
+      if (v.nu < v.uses.size()) {
+        res.insert({next_use(v), id});
+      } else {
+        v.resident = false;
+        if (v.home >= 0) free_rows.push_back(v.home);
+      }
     }
+    // definitions enter the registers
+    {
+      for (int id : op.defs) {
+        Val& v = val[id];
+        if (v.uses.empty()) continue;
+        prev = v.name;
+        v.resident = true;
+        v.dirty = true;
+        res.insert({next_use(v), id});
+      }
+    }
+    over_budget(ii);
   }
-  o += "}\n"
+  if (spill_rows) *spill_rows = n_spill;
+  // group g's rows follow those of groups 0..g-1: [group][instance/64][row][64]
+  const std::string gs = std::to_string(g);
+  std::string o;
+  o.reserve(b.size() + 1024);
+  o += "__device__ __forceinline__ void cxj_body_g" + gs + "(const CxjArgs& A, const long long k) {\n"
+       "  const long long cxj_tb = (k >> 6) * (CXJ_TROWS * 64LL) + (k & 63) + " + gs + "LL * A.lds * CXJ_TROWS;\n"
+       "  const long long cxj_sb = (k >> 6) * (CXJ_SROWS * 64LL) + (k & 63) + " + gs + "LL * A.lds * CXJ_SROWS;\n";
+  if (use_si) o += "  CXJ_GLOBAL const double* const cxj_si = (CXJ_GLOBAL const double*)A.st_in + cxj_tb;\n";
+  if (use_so) o += "  CXJ_GLOBAL double* const cxj_so = (CXJ_GLOBAL double*)A.st_out + cxj_tb;\n";
+  if (use_ss) o += "  CXJ_GLOBAL double* const cxj_ss = (CXJ_GLOBAL double*)A.spill + cxj_sb;\n";
+  for (int i : use_arg) {
+    const std::string is = std::to_string(i);
+    o += "  CXJ_GLOBAL const double* const cxj_a" + is + " = (CXJ_GLOBAL const double*)A.arg.a[" + is +
+         "].p + k * A.arg.a[" + is + "].s_inst;\n";
+  }
+  o += "  (void)cxj_tb;\n  (void)cxj_sb;\n";
+  o += b;
+  o += "}\n";
+  return o;
+}
+
+// Kernel of segment s: the bodies of all direction groups, blockIdx.y picking
+// the group, so that the groups run side by side (one wave per 64 instances
+// and group) rather than one after another.
+static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s, int* spill_rows) {
+  std::string bodies;
+  int sp = 0;
+  for (int g = 0; g < J.n_group; ++g) {
+    int ns = 0;
+    bodies += gen_tangent(j, J, g, s, &ns);
+    sp = std::max(sp, ns);
+  }
+  if (spill_rows) *spill_rows = sp;
+  std::string o = kSxOpsSource;
+  o += kPrologue;
+  o.reserve(o.size() + bodies.size() + 1024);
+  o += "#define CXJ_TROWS " + std::to_string(J.max_rows) + "\n#define CXJ_SROWS " + std::to_string(sp) + "\n";
+  o += bodies;
+  o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k, const int grp) {\n"
+       "  switch (grp) {\n";
+  for (int g = 0; g < J.n_group; ++g)
+    o += "    case " + std::to_string(g) + ": cxj_body_g" + std::to_string(g) + "(A, k); break;\n";
+  o += "  }\n}\n"
        "extern \"C\" __global__ void __launch_bounds__(256) cxj_tan(const CxjArgs A) {\n"
        "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-       "  if (k < A.n) cxj_body(A, k);\n"
+       "  if (k < A.n) cxj_body(A, k, (int)blockIdx.y);\n"
        "}\n";
   return o;
 }
@@ -894,34 +1266,58 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
 int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
                 int64_t jac_s_inst, hipStream_t s) {
   JitJac& J = jac_plan(j);
-  const int64_t lds = round_up(n, 64);
-  if (J.n_seg > 1) J.st.reserve(2 * static_cast<size_t>(std::max<int64_t>(J.max_rows, 1)) * lds * sizeof(double));
+  for (int sg = 0; sg < J.n_seg; ++sg) {
+    hipFunction_t& f = J.fn[sg];
+    if (!f) {
+      int ns = 0;
+      const std::string src = gen_tangent_kernel(j, J, sg, &ns);
+      f = jit_function(src, "cxj_tan");
+      J.spill_rows = std::max<int64_t>(J.spill_rows, ns);
+    }
+  }
+  const int64_t lds = round_up(n, 64);  // instances, padded to whole tiles of 64
+  const int64_t half = static_cast<int64_t>(J.n_group) * J.max_rows * lds;
+  if (J.n_seg > 1) J.st.reserve(2 * static_cast<size_t>(std::max<int64_t>(half, 1)) * sizeof(double));
+  if (J.spill_rows > 0) J.spill.reserve(static_cast<size_t>(J.n_group) * J.spill_rows * lds * sizeof(double));
   double* st = J.st.as<double>();
   CxjArgs A{};
   A.arg = arg;
+  for (auto& a : A.arg.a)  // an absent input reads zeros: one device zero, strides 0
+    if (!a.p) {
+      if (!J.zero.ptr) {
+        static const double zeros[8] = {};
+        J.zero.upload(zeros, sizeof zeros);
+      }
+      a.p = J.zero.as<double>();
+      a.s_nz = 0;
+      a.s_inst = 0;
+    }
   A.res = res;
   A.asens.a[0].p = jac;
   A.asens.a[0].s_nz = jac_s_nz;
   A.asens.a[0].s_inst = jac_s_inst;
   A.n = n;
   A.lds = lds;
-  for (int g = 0; g < J.n_group; ++g)
-    for (int sg = 0; sg < J.n_seg; ++sg) {
-      hipFunction_t& f = J.fn[g][sg];
-      if (!f) f = jit_function(gen_tangent(j, J, g, sg), "cxj_tan");
-      A.st_in = st ? st + static_cast<int64_t>(sg % 2) * J.max_rows * lds : nullptr;
-      A.st_out = st ? st + static_cast<int64_t>((sg + 1) % 2) * J.max_rows * lds : nullptr;
-      launch(f, n, j->tape->block, A, s);
-    }
+  A.spill = J.spill_rows > 0 ? J.spill.as<double>() : nullptr;
+  const int block = j->tape->block;
+  const unsigned grid = static_cast<unsigned>((n + block - 1) / block);
+  for (int sg = 0; sg < J.n_seg; ++sg) {
+    A.st_in = st ? st + static_cast<int64_t>(sg % 2) * half : nullptr;
+    A.st_out = st ? st + static_cast<int64_t>((sg + 1) % 2) * half : nullptr;
+    void* params[] = {&A};
+    hip_check(hipModuleLaunchKernel(J.fn[sg], grid, static_cast<unsigned>(J.n_group), 1, block, 1, 1, 0, s, params,
+                                    nullptr),
+              "jit tangent kernel launch");
+  }
   return 0;
 }
 
-std::string jit_tangent_source(cxh_jac_s* j, int g, int s, int* n_group, int* n_seg) {
+std::string jit_tangent_source(cxh_jac_s* j, int s, int* n_group, int* n_seg) {
   JitJac& J = jac_plan(j);
   if (n_group) *n_group = J.n_group;
   if (n_seg) *n_seg = J.n_seg;
-  if (g < 0 || g >= J.n_group || s < 0 || s >= J.n_seg) throw std::runtime_error("group/segment out of range");
-  return gen_tangent(j, J, g, s);
+  if (s < 0 || s >= J.n_seg) throw std::runtime_error("segment out of range");
+  return gen_tangent_kernel(j, J, s, nullptr);
 }
 
 // ------------------------------------------------------------ mapaccum ---
@@ -1653,12 +2049,12 @@ int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, i
   CXH_CATCH
 }
 
-int cxh_jac_jit_source(cxh_jac j, int32_t group, int32_t segment, char* buf, int64_t cap, int64_t* len,
-                       int32_t* n_groups, int32_t* n_segments) {
+int cxh_jac_jit_source(cxh_jac j, int32_t segment, char* buf, int64_t cap, int64_t* len, int32_t* n_groups,
+                       int32_t* n_segments) {
   CXH_TRY
   if (!j) return cxh_fail("null jac plan");
   int ng = 0, ns = 0;
-  const std::string src = jit_tangent_source(j, group, segment, &ng, &ns);
+  const std::string src = jit_tangent_source(j, segment, &ng, &ns);
   if (n_groups) *n_groups = ng;
   if (n_segments) *n_segments = ns;
   if (len) *len = static_cast<int64_t>(src.size());

@@ -209,14 +209,14 @@ class HipJacobian:
             L.cxh_jac_destroy(h)
             self._h = None
 
-    def jit_source(self, group=0, segment=0):
-        """Generated source of a specialised tangent kernel, with the number of
-        direction groups and segments."""
+    def jit_source(self, segment=0):
+        """Generated source of the specialised tangent kernel of one segment
+        (all direction groups), with the number of groups and segments."""
         n, ng, ns = L.I64(), L.I32(), L.I32()
-        L.check(L.cxh_jac_jit_source(self._h, int(group), int(segment), None, 0, C.byref(n), C.byref(ng),
-                                     C.byref(ns)), "cxh_jac_jit_source")
+        L.check(L.cxh_jac_jit_source(self._h, int(segment), None, 0, C.byref(n), C.byref(ng), C.byref(ns)),
+                "cxh_jac_jit_source")
         buf = C.create_string_buffer(n.value + 1)
-        L.check(L.cxh_jac_jit_source(self._h, int(group), int(segment), buf, n.value + 1, C.byref(n), C.byref(ng),
+        L.check(L.cxh_jac_jit_source(self._h, int(segment), buf, n.value + 1, C.byref(n), C.byref(ng),
                                      C.byref(ns)), "cxh_jac_jit_source")
         return buf.value.decode(), ng.value, ns.value
 

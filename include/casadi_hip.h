@@ -89,11 +89,12 @@ int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, 
  * to cap), its full length in *len and the number of segments.               */
 int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, int32_t reverse,
                         char* buf, int64_t cap, int64_t* len, int32_t* n_segments);
-/* Introspection: generated source of the specialised tangent kernel of a
- * Jacobian plan (direction group, segment), with the group / segment counts.
- * cxh_map_jac runs these kernels when the plan's tape uses CXH_ENGINE_JIT.  */
-int cxh_jac_jit_source(cxh_jac j, int32_t group, int32_t segment, char* buf, int64_t cap,
-                       int64_t* len, int32_t* n_groups, int32_t* n_segments);
+/* Introspection: generated source of the specialised tangent kernel of one
+ * segment of a Jacobian plan (the bodies of all direction groups, selected by
+ * blockIdx.y), with the group / segment counts.  cxh_map_jac runs these
+ * kernels when the plan's tape uses CXH_ENGINE_JIT.                         */
+int cxh_jac_jit_source(cxh_jac j, int32_t segment, char* buf, int64_t cap, int64_t* len,
+                       int32_t* n_groups, int32_t* n_segments);
 /* Compile a generated source for arch (e.g. "gfx950") without loading it;
  * 0 when it compiles, the compiler log in cxh_last_error otherwise.        */
 int cxh_jit_compile_check(const char* src, const char* arch);

@@ -35,7 +35,7 @@ class IOTable(C.Structure):
 class CxjArgs(C.Structure):
     _fields_ = [("arg", IOTable), ("res", IOTable), ("aseed", IOTable), ("asens", IOTable),
                 ("st_in", C.c_void_p), ("st_out", C.c_void_p), ("adj_in", C.c_void_p), ("adj_out", C.c_void_p),
-                ("n", C.c_longlong), ("lds", C.c_longlong)]
+                ("n", C.c_longlong), ("lds", C.c_longlong), ("spill", C.c_void_p)]
 
 
 def _table(arrs, n):
@@ -48,14 +48,28 @@ def _table(arrs, n):
     return t
 
 
-def _compile(sources, workdir):
-    """One shared library exposing run_<i>(const CxjArgs*, n) per source."""
+def _tiled(srcs, n, groups):
+    """State (two halves) and spill buffers of the tangent kernels, tiled
+    [group][instance/64][row][64] with the row counts each source declares,
+    and the padded instance count the kernels take as lds."""
+    rows = max(int(re.search(r"#define CXJ_TROWS (\d+)", x).group(1)) for x in srcs)
+    spill = max(int(re.search(r"#define CXJ_SROWS (\d+)", x).group(1)) for x in srcs)
+    lds = (n + 63) // 64 * 64
+    return ([np.zeros(groups * lds * max(rows, 1)) for _ in range(2)], np.zeros(groups * lds * max(spill, 1)), spill,
+            lds)
+
+
+def _compile(sources, workdir, groups=None):
+    """One shared library exposing run_<i>(const CxjArgs*, n) per source; with
+    groups, the body runs for every direction group in turn."""
     libs = []
     for i, src in enumerate(sources):
         cut = src.index('extern "C" __global__')
-        code = ("#define __device__\n#define __forceinline__ inline\n#define CXJ_KEEP(x) ((void)0)\n#define CXJ_KEEP_PTR(x) ((void)0)\n#define CXJ_AFTER(p, x) ((void)0)\n" + src[:cut] +
-                f'extern "C" void run_{i}(const CxjArgs* A, long long n) {{\n'
-                "  for (long long k = 0; k < n; ++k) cxj_body(*A, k);\n}\n")
+        loop = ("  for (long long k = 0; k < n; ++k) cxj_body(*A, k);\n" if groups is None else
+                f"  for (int g = 0; g < {groups}; ++g)\n    for (long long k = 0; k < n; ++k) cxj_body(*A, k, g);\n")
+        code = ("#define __device__\n#define __forceinline__ inline\n#define CXJ_KEEP(x) ((void)0)\n#define CXJ_KEEP_PTR(x) ((void)0)\n#define CXJ_AFTER(p, x) ((void)0)\n"
+                "#define CXJ_GLOBAL\n#define CXJ_OPAQUE(p) ((void)0)\n#define CXJ_OPAQUE_AFTER(p, x) ((void)0)\n" + src[:cut] +
+                f'extern "C" void run_{i}(const CxjArgs* A, long long n) {{\n' + loop + "}\n")
         path = os.path.join(workdir, f"seg{i}.cpp")
         with open(path, "w") as fh:
             fh.write(code)
@@ -132,18 +146,20 @@ def test_generated_tangent_sweeps_run_on_host(monkeypatch):
     n = 29
     rng = np.random.default_rng(9)
     x, u = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (1, n))
-    _, ng, ns = hj.jit_source(0, 0)
-    srcs = [hj.jit_source(g, 0)[0] for g in range(ng)]
+    src, ng, ns = hj.jit_source(0)
+    srcs = [src]
     assert ng == 3 and ns == 1
     jac = np.full((hj.nnz_jac, n), np.nan)
     y = np.zeros((4, n))
+    _, sp, _, lds = _tiled(srcs, n, ng)
     with tempfile.TemporaryDirectory() as d:
-        libs = _compile(srcs, d)
+        libs = _compile(srcs, d, ng)
         A = CxjArgs()
-        A.n, A.lds = n, n
+        A.n, A.lds = n, lds
         A.arg = _table([x, u], n)
         A.res = _table([y], n)
         A.asens = _table([jac], n)
+        A.spill = sp.ctypes.data
         for _, fn in libs:
             fn(C.byref(A), n)
     ref = np.concatenate(O.map_eval(f.jacobian(), n, [x.T, u.T, None]), axis=1).T
@@ -152,31 +168,32 @@ def test_generated_tangent_sweeps_run_on_host(monkeypatch):
     # star-coloured Hessian of a 2-step quadrotor OCP Lagrangian, 4 segments
     monkeypatch.setenv("CXH_JIT_DIRGROUP", "4")
     monkeypatch.setenv("CXH_JIT_TANSEG", "400")
+    monkeypatch.setenv("CXH_JIT_TANREGS", "24")   # small register budget: spills and reloads
     w, lam, Lag = W.quadrotor_ocp_lagrangian(N=2)
     G = ca.Function("grad_lag", [w, lam], [ca.gradient(Lag, w)])
     H, _ = ca.hessian(Lag, w)
     hh = HipJacobian(G, symmetric=True, first_block=True, engine="jit")
-    _, ng, ns = hh.jit_source(0, 0)
+    _, ng, ns = hh.jit_source(0)
     assert ns > 1 and ng > 1
     wv, lv = rng.uniform(-1, 1, (w.nnz(), n)), rng.standard_normal((lam.nnz(), n))
     hnz = np.full((hh.nnz_jac, n), np.nan)
     gv = np.zeros((w.nnz(), n))
-    st = [np.zeros((4096, n)) for _ in range(2)]
+    srcs = [hh.jit_source(s)[0] for s in range(ns)]
+    assert all(f"cxj_body_g{g}(" in x for x in srcs for g in range(ng))
+    st, sp, n_spill, lds = _tiled(srcs, n, ng)
+    assert n_spill > 0 and any("cxj_ss" in x for x in srcs)
     with tempfile.TemporaryDirectory() as d:
-        for g in range(ng):
-            srcs = [hh.jit_source(g, s)[0] for s in range(ns)]
-            sub = os.path.join(d, f"g{g}")
-            os.makedirs(sub)
-            libs = _compile(srcs, sub)
-            A = CxjArgs()
-            A.n, A.lds = n, n
-            A.arg = _table([wv, lv], n)
-            A.res = _table([gv], n)
-            A.asens = _table([hnz], n)
-            for s, (_, fn) in enumerate(libs):
-                A.st_in = st[s % 2].ctypes.data
-                A.st_out = st[(s + 1) % 2].ctypes.data
-                fn(C.byref(A), n)
+        libs = _compile(srcs, d, ng)
+        A = CxjArgs()
+        A.n, A.lds = n, lds
+        A.arg = _table([wv, lv], n)
+        A.res = _table([gv], n)
+        A.asens = _table([hnz], n)
+        A.spill = sp.ctypes.data
+        for s, (_, fn) in enumerate(libs):
+            A.st_in = st[s % 2].ctypes.data
+            A.st_out = st[(s + 1) % 2].ctypes.data
+            fn(C.byref(A), n)
     Hf = ca.Function("hess_lag", [w, lam], [H])
     (href,) = O.map_eval(Hf, n, [wv.T, lv.T])
     assert hh.jsp == H.sparsity
