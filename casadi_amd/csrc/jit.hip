@@ -726,18 +726,31 @@ struct JitJac {
   DeviceBuffer st, spill, zero;
 };
 
-static int jac_dir_group() {
+// Directions per group.  Each group is a body of its own over all instances
+// (n/64 waves) and the kernels hold about one wave per SIMD (register-bound),
+// so by default the directions are split into as many groups as fill the
+// device's SIMDs once; CXH_JIT_DIRGROUP overrides.  n <= 0: no batch size
+// known (source introspection), 8 directions per group.
+static int jac_dir_group(int ndir, int64_t n) {
   if (const char* e = std::getenv("CXH_JIT_DIRGROUP")) return std::max(1, std::atoi(e));
-  return 8;
+  if (n <= 0) return 8;
+  int dev = 0, cus = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device CU count");
+  const int64_t simds = 4 * static_cast<int64_t>(std::max(cus, 1));
+  const int64_t waves = std::max<int64_t>(1, (n + 63) / 64);
+  const int64_t groups = std::min<int64_t>(std::max<int64_t>(1, (simds + waves / 2) / waves), std::max(ndir, 1));
+  return static_cast<int>((ndir + groups - 1) / groups);
 }
 
-static JitJac& jac_plan(cxh_jac_s* j) {
-  if (j->jit) return *j->jit;
+static JitJac& jac_plan(cxh_jac_s* j, int64_t n) {
+  const int want = std::min(std::max(j->ndir, 1), std::min(jac_dir_group(j->ndir, n), 64));
+  if (j->jit && j->jit->group == want) return *j->jit;
   const cxh_tape_s* t = j->tape;
   auto J = std::make_shared<JitJac>();
   const int nd = j->ndir;
   const int64_t ni = static_cast<int64_t>(t->alg.size());
-  J->group = std::min(std::max(nd, 1), std::min(jac_dir_group(), 64));
+  J->group = want;
   J->n_group = std::max(1, (nd + J->group - 1) / J->group);
   int64_t every = default_every(ni);
   if (const char* e = std::getenv("CXH_JIT_TANSEG")) every = std::max(1, std::atoi(e));
@@ -785,7 +798,7 @@ static JitJac& jac_plan(cxh_jac_s* j) {
 // Register budget of one tangent kernel, in doubles held at once.
 static int jac_reg_budget() {
   if (const char* e = std::getenv("CXH_JIT_TANREGS")) return std::max(8, std::atoi(e));
-  return 96;
+  return 200;
 }
 
 // Reloads are issued this many instructions ahead of their use.
@@ -1265,7 +1278,7 @@ static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s
 
 int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
                 int64_t jac_s_inst, hipStream_t s) {
-  JitJac& J = jac_plan(j);
+  JitJac& J = jac_plan(j, n);
   for (int sg = 0; sg < J.n_seg; ++sg) {
     hipFunction_t& f = J.fn[sg];
     if (!f) {
@@ -1313,7 +1326,7 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
 }
 
 std::string jit_tangent_source(cxh_jac_s* j, int s, int* n_group, int* n_seg) {
-  JitJac& J = jac_plan(j);
+  JitJac& J = jac_plan(j, 0);
   if (n_group) *n_group = J.n_group;
   if (n_seg) *n_seg = J.n_seg;
   if (s < 0 || s >= J.n_seg) throw std::runtime_error("segment out of range");
