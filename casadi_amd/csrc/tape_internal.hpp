@@ -66,6 +66,7 @@ struct cxh_tape_s {
   sxh::DeviceBuffer d_tape, d_unused;  // uploaded at first interpreter launch
   bool uploaded = false;
   sxh::DeviceBuffer scratch, trail, ckpt;
+  sxh::DeviceBuffer red, partial;  // summed blocks of a reduced map (mapsum.hip)
   int block = 64;
   int w_lds = 1;
   int ipw = 64;  // instances per wavefront (8, 16, 32, 64)
@@ -80,6 +81,12 @@ struct GridShape {
 };
 GridShape grid_shape(const cxh_tape_s* t, int64_t n);
 const Instr* device_tape(cxh_tape_s* t);
+
+// Evaluation and fused forward + adjoint over explicit I/O tables (any
+// per-block strides: s_inst = 0 broadcasts a block to every instance).
+int map_eval_io(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);
+int adj_io(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
+           const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);
 
 // Straight-line specialised kernels (jit.hip).
 int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);

@@ -600,10 +600,17 @@ int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const*
   if (n < 0) return cxh_fail("negative n");
   if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  return map_eval_io(t, n, make_io(arg, t->nnz_in, layout, ld), make_io(res, t->nnz_out, layout, ld),
+                     static_cast<hipStream_t>(stream));
+  CXH_CATCH
+}
+
+}  // extern "C"
+
+namespace sxh {
+
+int map_eval_io(cxh_tape t, int64_t n, const IOTable& d_arg, const IOTable& d_res, hipStream_t s) {
   if (n == 0 || t->alg.empty()) return 0;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const IOTable d_arg = make_io(arg, t->nnz_in, layout, ld);
-  const IOTable d_res = make_io(res, t->nnz_out, layout, ld);
   if (t->engine == CXH_ENGINE_JIT) return jit_map_eval(t, n, d_arg, d_res, s);
   const GridShape g = grid_shape(t, n);
   const size_t lds = static_cast<size_t>(t->ws) * g.cols * sizeof(double);
@@ -621,8 +628,11 @@ int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const*
   }
   hip_check(hipGetLastError(), "tape_fwd_kernel launch");
   return 0;
-  CXH_CATCH
 }
+
+}  // namespace sxh
+
+extern "C" {
 
 int cxh_map_eval_host(cxh_tape t, int64_t n, const double* const* arg, double* const* res) {
   CXH_TRY
@@ -678,19 +688,23 @@ static int launch_adj(cxh_tape t, int mode, int64_t n, const double* const* arg,
   if (n < 0) return cxh_fail("negative n");
   if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  return adj_io(t, mode, n, make_io(arg, t->nnz_in, layout, ld), make_io(aseed, t->nnz_out, layout, ld),
+                make_io(res, t->nnz_out, layout, ld), make_io(asens, t->nnz_in, layout, ld), every, ckpt_ext, s);
+}
+
+int adj_io(cxh_tape t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
+           const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s) {
   if (n == 0 || t->alg.empty()) return 0;
-  if (t->engine == CXH_ENGINE_JIT)
-    return jit_adj(t, mode, n, make_io(arg, t->nnz_in, layout, ld), make_io(aseed, t->nnz_out, layout, ld),
-                   make_io(res, t->nnz_out, layout, ld), make_io(asens, t->nnz_in, layout, ld), every, ckpt_ext, s);
+  if (t->engine == CXH_ENGINE_JIT) return jit_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
   AdjArgs A{};
   A.tape = device_tape(t);
   A.n_instr = static_cast<int>(t->alg.size());
   A.ws = static_cast<int>(t->ws);
   segments(t, every, A.seg_len, A.n_seg);
-  A.arg = make_io(arg, t->nnz_in, layout, ld);
-  A.res = make_io(res, t->nnz_out, layout, ld);
-  A.aseed = make_io(aseed, t->nnz_out, layout, ld);
-  A.asens = make_io(asens, t->nnz_in, layout, ld);
+  A.arg = arg;
+  A.res = res;
+  A.aseed = aseed;
+  A.asens = asens;
   A.unused_in = t->d_unused.as<int2>();
   A.n_unused = static_cast<int>(t->unused_in.size());
   A.n = n;

@@ -373,13 +373,47 @@ class Function:
             return TorchFunction(self, checkpoint_every, engine, tape=HipMapAccum.from_function(self))
         return TorchFunction(self, checkpoint_every, engine)
 
-    def map(self, n, parallelization="serial", max_num_threads=None, opts=None):
-        """Map::create (casadi/core/map.cpp:39-51)."""
+    def map(self, *args, **kwargs):
+        """Function::map overloads (casadi/core/function.cpp:789-896):
+
+        * ``map(n, parallelization="serial", max_num_threads=None, opts=None)``:
+          Map::create (casadi/core/map.cpp:39-51);
+        * ``map(n, reduce_in, reduce_out, opts=None)`` with lists of bools: the
+          serial MapSum (casadi/core/function.cpp:789-794);
+        * ``map(name, parallelization, n, reduce_in, reduce_out, opts=None)``
+          with lists of input / output indices or names: a map whose listed
+          inputs are shared and listed outputs summed
+          (casadi/core/function.cpp:796-826).
+        """
+        from .map import MapFunction, MapSumFunction
+        if args and isinstance(args[0], str):
+            name, parallelization, n = args[0], args[1], int(args[2])
+            rin = list(args[3]) if len(args) > 3 else list(kwargs.pop("reduce_in", []))
+            rout = list(args[4]) if len(args) > 4 else list(kwargs.pop("reduce_out", []))
+            opts = args[5] if len(args) > 5 else kwargs.pop("opts", None)
+            self._check_par(parallelization)
+            ri = [False] * self.n_in()
+            for i in rin:
+                ri[self.index_in(i) if isinstance(i, str) else int(i)] = True
+            ro = [False] * self.n_out()
+            for j in rout:
+                ro[self.index_out(j) if isinstance(j, str) else int(j)] = True
+            return MapSumFunction(self, n, parallelization, ri, ro, name=name, opts=opts or {})
+        if len(args) >= 3 and isinstance(args[1], (list, tuple)):
+            opts = args[3] if len(args) > 3 else kwargs.pop("opts", None)
+            return MapSumFunction(self, int(args[0]), "serial", args[1], args[2], opts=opts or {})
+        n = int(args[0])
+        parallelization = args[1] if len(args) > 1 else kwargs.pop("parallelization", "serial")
+        max_num_threads = args[2] if len(args) > 2 else kwargs.pop("max_num_threads", None)
+        opts = args[3] if len(args) > 3 else kwargs.pop("opts", None)
+        self._check_par(parallelization)
+        return MapFunction(self, n, parallelization, max_num_threads, opts or {})
+
+    @staticmethod
+    def _check_par(parallelization):
         if parallelization not in PARALLELIZATIONS:
             raise ValueError(f"Unknown parallelization '{parallelization}' for map: "
                              f"choose one of {PARALLELIZATIONS}")
-        from .map import MapFunction
-        return MapFunction(self, int(n), parallelization, max_num_threads, opts or {})
 
     def __repr__(self):
         ins = ", ".join(f"{self.name_in(i)}[{self.size_in(i)[0]}x{self.size_in(i)[1]}]" for i in range(self.n_in()))

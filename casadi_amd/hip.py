@@ -41,7 +41,7 @@ def _check_soa(tensors, nnz, n, what):
             raise TypeError(f"{what}[{i}] must be a float64 CUDA tensor")
         if t.dim() != 2 or t.shape[0] != nnz[i] or t.shape[1] < n:
             raise ValueError(f"{what}[{i}] has shape {tuple(t.shape)}, expected ({nnz[i]}, {n})")
-        if nnz[i] > 1 and t.stride(1) != 1:
+        if nnz[i] > 1 and t.shape[1] > 0 and t.stride(1) != 1:
             raise ValueError(f"{what}[{i}] must have unit stride along instances")
         s = t.stride(0) if nnz[i] > 1 else n
         if ld is None:
@@ -152,6 +152,77 @@ class HipTape:
         adj = self._alloc(self.nnz_in, n, like)
         L.check(L.cxh_map_fwd_adj(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adj), CXH_SOA, ld,
                                   int(checkpoint_every), _stream_ptr(stream)), "cxh_map_fwd_adj")
+        return outs, adj
+
+    # -- reduced maps (shared inputs, outputs summed over instances) ------------
+    @staticmethod
+    def _flags(flags, count, what):
+        if flags is None:
+            return [False] * count
+        flags = [bool(f) for f in flags]
+        if len(flags) != count:
+            raise ValueError(f"{what} has {len(flags)} entries, expected {count}")
+        return flags
+
+    @staticmethod
+    def _shared(t, nnz, what):
+        """A block shared by all instances (or summed over them): nnz doubles."""
+        torch = _torch()
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+            raise TypeError(f"{what} must be a float64 CUDA tensor")
+        if t.numel() != nnz or (nnz > 1 and t.reshape(-1).stride(0) != 1):
+            raise ValueError(f"{what} must hold the block's {nnz} nonzeros contiguously")
+        return t.reshape(-1)
+
+    def _red_args(self, tensors, nnz, flags, n, what):
+        """Per-instance blocks checked as SoA, shared blocks as nnz vectors."""
+        per = [None if (t is None or flags[i]) else t for i, t in enumerate(tensors)]
+        ld = _check_soa(per, nnz, n, what)
+        out = [None if t is None else (self._shared(t, nnz[i], f"{what}[{i}]") if flags[i] else t)
+               for i, t in enumerate(tensors)]
+        return out, ld
+
+    def _red_alloc(self, nnz, flags, n, ld, like):
+        torch = _torch()
+        return [None if not k else
+                torch.empty(k, dtype=torch.float64, device=like.device) if flags[i] else
+                torch.empty((k, ld), dtype=torch.float64, device=like.device)[:, :n]
+                for i, k in enumerate(nnz)]
+
+    def eval_sum(self, args, n, reduce_in=None, reduce_out=None, stream=None):
+        """Reduced map: inputs flagged in reduce_in are shared by all n
+        instances (a vector of the block's nonzeros), outputs flagged in
+        reduce_out are summed over the instances (a vector); the other blocks
+        are (nnz, n) as in eval()."""
+        ri = self._flags(reduce_in, len(self.nnz_in), "reduce_in")
+        ro = self._flags(reduce_out, len(self.nnz_out), "reduce_out")
+        like = self._like(args)
+        args, ld = self._red_args(args, self.nnz_in, ri, n, "arg")
+        out = self._red_alloc(self.nnz_out, ro, n, ld, like)
+        L.check(L.cxh_map_eval_sum(self._h, n, _ptrs(args), _ptrs(out), (L.I32 * len(ri))(*ri),
+                                   (L.I32 * len(ro))(*ro), ld, _stream_ptr(stream)), "cxh_map_eval_sum")
+        return out
+
+    def fwd_adj_sum(self, args, seeds, n, reduce_in=None, reduce_out=None, checkpoint_every=0, res=True,
+                    stream=None):
+        """Fused forward + adjoint of the reduced map: the seed of a reduced
+        output (a vector) applies to every instance, the sensitivity of a
+        reduced input is summed over the instances (a vector)."""
+        ri = self._flags(reduce_in, len(self.nnz_in), "reduce_in")
+        ro = self._flags(reduce_out, len(self.nnz_out), "reduce_out")
+        like = self._like(args)
+        args, ld = self._red_args(args, self.nnz_in, ri, n, "arg")
+        seeds, ld_s = self._red_args(seeds, self.nnz_out, ro, n, "adj_out")
+        per_in = any(k > 1 and not f for k, f in zip(self.nnz_in, ri))
+        per_out = any(k > 1 and not f and s is not None for k, f, s in zip(self.nnz_out, ro, seeds))
+        if per_in and per_out and ld_s != ld:
+            raise ValueError("inputs and seeds must share the leading dimension")
+        ld = ld if per_in else ld_s
+        outs = self._red_alloc(self.nnz_out, ro, n, ld, like) if res else [None] * len(self.nnz_out)
+        adj = self._red_alloc(self.nnz_in, ri, n, ld, like)
+        L.check(L.cxh_map_fwd_adj_sum(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adj),
+                                      (L.I32 * len(ri))(*ri), (L.I32 * len(ro))(*ro), ld, int(checkpoint_every),
+                                      _stream_ptr(stream)), "cxh_map_fwd_adj_sum")
         return outs, adj
 
     def ckpt_size(self, n, checkpoint_every):

@@ -106,3 +106,117 @@ class MapFunction:
             pr = L.ptr_array([r.ctypes.data if r.size else None for r in res])
             L.check(L.cxs_function_map_eval(self.f._h, self.n, pa, pr, self.n_threads), self.name())
         return [self._dense_out(i, r) for i, r in enumerate(res)]
+
+
+class MapSumFunction:
+    """A map whose reduced inputs are shared by all n instances and whose
+    reduced outputs are summed over them.
+
+    ``Function.map(n, reduce_in, reduce_out)`` is CasADi's serial MapSum
+    (MapSum::eval_gen, casadi/core/mapsum.cpp:155-186);
+    ``Function.map(name, parallelization, n, reduce_in, reduce_out)`` is the
+    reduced form of any map (casadi/core/function.cpp:796-817: repmat of the
+    reduced inputs, repsum of the reduced outputs).  With ``hip`` the shared
+    inputs are read in place by every instance and the sums run on the GPU in a
+    fixed order (``cxh_map_eval_sum``); the host parallelizations add instance
+    by instance, as MapSum does.  A reduced input takes the base Function's
+    input (its nonzeros as a vector on the device), a reduced output is the base
+    Function's output."""
+
+    def __init__(self, f, n, parallelization, reduce_in, reduce_out, name=None, max_num_threads=None,
+                 opts=None):
+        self.f = f
+        self.n = int(n)
+        self.parallelization = parallelization
+        self.reduce_in = [bool(x) for x in reduce_in]
+        self.reduce_out = [bool(x) for x in reduce_out]
+        if len(self.reduce_in) != f.n_in() or len(self.reduce_out) != f.n_out():
+            raise ValueError("Dimension mismatch")  # casadi/core/mapsum.cpp:45-46
+        self._name = name or f"mapsum_{self.n}_{f.name()}"
+        self.map = MapFunction(f, self.n, parallelization, max_num_threads, opts)
+
+    def name(self):
+        return self._name
+
+    def n_in(self):
+        return self.f.n_in()
+
+    def n_out(self):
+        return self.f.n_out()
+
+    @property
+    def tape(self):
+        return self.map.tape
+
+    def __call__(self, *args):
+        res = self.call(list(args))
+        return res[0] if len(res) == 1 else tuple(res)
+
+    def _device(self, args):
+        try:
+            import torch
+        except ImportError:
+            return False
+        return any(isinstance(a, torch.Tensor) and a.is_cuda for a in args)
+
+    def _nz(self, sp, a):
+        """Nonzeros of one (shared) block given densely, as a vector."""
+        v = np.asarray(a, dtype=float)
+        nr, nc = sp.shape
+        if v.size == 1 and sp.nnz() != 1:
+            v = np.full((nr, nc), float(v.reshape(-1)[0]))
+        v = v.reshape(nr, nc) if v.size == nr * nc else v
+        if v.shape != (nr, nc):
+            raise ValueError(f"{self.name()}: shared input has shape {v.shape}, expected {(nr, nc)}")
+        return np.ascontiguousarray(v[sp.row, sp.get_col()])
+
+    def _dense(self, sp, nz):
+        nr, nc = sp.shape
+        out = np.zeros((nr, nc))
+        out[sp.row, sp.get_col()] = nz
+        return out
+
+    def call(self, args):
+        if len(args) != self.f.n_in():
+            raise ValueError(f"{self.name()}: expected {self.f.n_in()} inputs")
+        if self._device(args):
+            if self.parallelization != "hip":
+                raise TypeError("device tensors require parallelization 'hip'")
+            return self.tape.eval_sum(list(args), self.n, self.reduce_in, self.reduce_out)
+        if self.parallelization == "hip":
+            import torch
+            dev = [None if a is None else
+                   torch.tensor(self._nz(self.f.sparsity_in(i), a), device="cuda") if self.reduce_in[i] else
+                   torch.tensor(self.map._aos(i, a).T.copy(), device="cuda")
+                   for i, a in enumerate(args)]
+            out = self.tape.eval_sum(dev, self.n, self.reduce_in, self.reduce_out)
+            return [self._dense(self.f.sparsity_out(j), o.cpu().numpy()) if self.reduce_out[j] else
+                    self.map._dense_out(j, o.cpu().numpy().T.reshape(-1))
+                    for j, o in enumerate(out)]
+        # host: every instance reads the shared blocks; sums in instance order
+        full = [None if a is None else
+                np.tile(self._dense(self.f.sparsity_in(i), self._nz(self.f.sparsity_in(i), a)), (1, self.n))
+                if self.reduce_in[i] else a for i, a in enumerate(args)]
+        res = self.map.call(full)
+        out = []
+        for j, r in enumerate(res):
+            if not self.reduce_out[j]:
+                out.append(r)
+                continue
+            nr, nc = self.f.sparsity_out(j).shape
+            blocks = r.reshape(nr, self.n, nc)
+            acc = np.zeros((nr, nc))
+            for k in range(self.n):
+                acc = acc + blocks[:, k, :]
+            out.append(acc)
+        return out
+
+    def fwd_adj(self, args, seeds, checkpoint_every=0, res=True):
+        """Device tensors: outputs and the reverse sweep of the reduced map in
+        one pass (the MapSum's reverse, casadi/core/mapsum.cpp:354-415): the
+        seed of a reduced output applies to every instance, the sensitivity of
+        a reduced input is summed over the instances."""
+        if self.parallelization != "hip":
+            raise TypeError("fwd_adj runs on the GPU: parallelization 'hip'")
+        return self.tape.fwd_adj_sum(list(args), list(seeds), self.n, self.reduce_in, self.reduce_out,
+                                     checkpoint_every=checkpoint_every, res=res)

@@ -125,6 +125,29 @@ int cxh_map_fwd_adj(cxh_tape t, int64_t n, const double* const* arg,
                     double* const* adj_in, int32_t layout, int64_t ld,
                     int64_t checkpoint_every, void* stream);
 
+/* Reduced map: the n instances share the inputs flagged in reduce_in (arg[i]
+ * is then nnz_in(i) doubles, read by every instance) and the outputs flagged
+ * in reduce_out are summed over the instances (res[j] receives nnz_out(j)
+ * doubles).  Other blocks are SoA [nz][ld] as in cxh_map_eval.  NULL flag
+ * arrays mean no reduction.  The sum runs in a fixed order that depends on n
+ * only (bitwise reproducible), not instance by instance.
+ * Replaces: Function::map(name, parallelization, n, reduce_in, reduce_out)
+ *   (casadi/core/function.cpp:796-817: repmat of the reduced inputs, repsum of
+ *   the reduced outputs around a parallel map) and MapSum::eval_gen
+ *   (casadi/core/mapsum.cpp:155-186) for the "hip" parallelization.         */
+int cxh_map_eval_sum(cxh_tape t, int64_t n, const double* const* arg, double* const* res,
+                     const int32_t* reduce_in, const int32_t* reduce_out, int64_t ld, void* stream);
+
+/* Fused forward + adjoint of a reduced map: the seed adj_out[j] of a reduced
+ * output is nnz_out(j) doubles applied to every instance, the sensitivity
+ * adj_in[i] of a reduced input is the sum over instances (nnz_in(i) doubles).
+ * Replaces: the reverse Function of a MapSum (MapSum::get_reverse,
+ *   casadi/core/mapsum.cpp:354-415) evaluated after the MapSum.              */
+int cxh_map_fwd_adj_sum(cxh_tape t, int64_t n, const double* const* arg,
+                        const double* const* adj_out, double* const* res, double* const* adj_in,
+                        const int32_t* reduce_in, const int32_t* reduce_out, int64_t ld,
+                        int64_t checkpoint_every, void* stream);
+
 /* Split adjoint for autograd: a forward pass that stores the work vector at
  * the start of every segment into a caller-owned device buffer `ckpt`
  * (cxh_ckpt_size doubles), then the reverse sweep restarting from those

@@ -109,6 +109,43 @@ def adj_map(f, n, args_aos, seeds_aos, n_threads=1):
     return res, sens
 
 
+def mapsum_eval(f, n, args, reduce_in, reduce_out):
+    """MapSum::eval_gen (casadi/core/mapsum.cpp:155-186): a reduced input is
+    one block read by every instance, a reduced output is the sum over the
+    instances, added instance by instance in order (casadi_add).
+    args[i]: (nnz,) for reduced inputs, (n, nnz) otherwise; returns (nnz,) for
+    reduced outputs, (n, nnz) otherwise."""
+    aos = [None if a is None else
+           np.tile(np.asarray(a, float).reshape(1, -1), (n, 1)) if reduce_in[i] else a
+           for i, a in enumerate(args)]
+    res = map_eval(f, n, aos)
+    return [_serial_sum(r) if reduce_out[j] else r for j, r in enumerate(res)]
+
+
+def mapsum_adj(f, n, args, seeds, reduce_in, reduce_out):
+    """The reverse of a MapSum (MapSum::get_reverse,
+    casadi/core/mapsum.cpp:354-415: the MapSum of f.reverse(1) whose seeds of
+    reduced outputs are shared and whose sensitivities of reduced inputs are
+    summed).  Returns (outputs, sensitivities) shaped as in mapsum_eval."""
+    aos = [None if a is None else
+           np.tile(np.asarray(a, float).reshape(1, -1), (n, 1)) if reduce_in[i] else a
+           for i, a in enumerate(args)]
+    sd = [None if a is None else
+          np.tile(np.asarray(a, float).reshape(1, -1), (n, 1)) if reduce_out[j] else a
+          for j, a in enumerate(seeds)]
+    res, sens = adj_map(f, n, aos, sd)
+    return ([_serial_sum(r) if reduce_out[j] else r for j, r in enumerate(res)],
+            [_serial_sum(g) if reduce_in[i] else g for i, g in enumerate(sens)])
+
+
+def _serial_sum(rows):
+    """sum_k rows[k], accumulated from zero in instance order."""
+    acc = np.zeros(rows.shape[1])
+    for k in range(rows.shape[0]):
+        acc = acc + rows[k]
+    return acc
+
+
 def ldl(sp_a, a, sp_lt, p):
     sp_a = np.ascontiguousarray(sp_a, np.int64)
     sp_lt = np.ascontiguousarray(sp_lt, np.int64)
