@@ -167,6 +167,36 @@ def kkt(n=8192):
              GBs=(sp.nnz() + 2 * sp.size1()) * 8 * n / ms / 1e6)
 
 
+def mapsum(n=1 << 20):
+    """Reduced map: the cart-pole step over n initial states with the control
+    shared by all of them (reduce_in), the next states summed (reduce_out);
+    forward alone and forward + adjoint (the gradient w.r.t. the shared
+    control summed over the instances).  Against the same results without the
+    reduction support: the shared blocks replicated per instance and the sums
+    taken by torch after a plain map."""
+    import torch
+    from . import workloads as W
+    from .hip import HipTape
+    f = W.cartpole_step()
+    rng = np.random.default_rng(20261206)
+    X = _dev(rng.uniform(-1, 1, (4, n)))
+    U = _dev(rng.uniform(-10, 10, (1,)))
+    S = _dev(rng.standard_normal(4))
+    t = HipTape(f, engine="jit")
+    ms = _timed(lambda: t.eval_sum([X, U], n, [False, True], [True]), 20)
+    _out(k="mapsum_eval", n=n, ms=ms, GBs=32 * n / ms / 1e6)
+    ms = _timed(lambda: t.fwd_adj_sum([X, U], [S], n, [False, True], [True]), 20)
+    _out(k="mapsum_fwd_adj", n=n, ms=ms, GBs=64 * n / ms / 1e6)
+
+    def unfused():
+        Ut = U.reshape(1, 1).expand(1, n).contiguous()
+        St = S.reshape(4, 1).expand(4, n).contiguous()
+        (y,), (gx, gu) = t.fwd_adj([X, Ut], [St], n)
+        return y.sum(dim=1), gx, gu.sum(dim=1)
+    ms = _timed(unfused, 20)
+    _out(k="mapsum_fwd_adj_unfused", n=n, ms=ms)
+
+
 def main(argv):
     which = argv or ["c1", "c2", "c4", "c5"]
     if "c1" in which:
@@ -181,6 +211,8 @@ def main(argv):
         kkt()
     if "c3macc" in which:
         rollout_mapaccum()
+    if "mapsum" in which:
+        mapsum()
 
 
 if __name__ == "__main__":

@@ -131,7 +131,7 @@ class MapSumFunction:
         self.reduce_in = [bool(x) for x in reduce_in]
         self.reduce_out = [bool(x) for x in reduce_out]
         if len(self.reduce_in) != f.n_in() or len(self.reduce_out) != f.n_out():
-            raise ValueError("Dimension mismatch")  # casadi/core/mapsum.cpp:45-46
+            raise ValueError("Dimension mismatch")  # casadi/core/mapsum.cpp:38-39
         self._name = name or f"mapsum_{self.n}_{f.name()}"
         self.map = MapFunction(f, self.n, parallelization, max_num_threads, opts)
 
