@@ -119,6 +119,7 @@ struct JitPlan {
 
 struct JitTape {
   std::map<int64_t, JitPlan> plans;  // keyed by segment length
+  int64_t eval_every = 0;            // segment length of forward-only evaluation (0: not chosen yet)
   DeviceBuffer st, adj, ck, snap;
 };
 
@@ -569,9 +570,37 @@ static hipFunction_t jit_function(const std::string& src, const char* name) {
   return fn;
 }
 
+// Bytes of scratch (private segment) per lane of a loaded kernel.
+static int scratch_bytes(hipFunction_t f) {
+  int v = 0;
+  hip_check(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f), "hipFuncGetAttribute");
+  return v;
+}
+
+// Generated kernels whose registers the compiler spills to scratch are not
+// launched.  Two such kernels gave wrong results or faulted on the GPU (a
+// Hessian tangent kernel spilling ~1.8 KB per lane, a rollout reverse kernel
+// with 384-instruction sub-blocks spilling 204 B per lane) while the same
+// generated source was correct when executed on the host, and the cause was
+// not found.  The generators keep every kernel in use free of scratch: the
+// tangent kernels lower their register budget until the compiler needs none,
+// the others raise this error with the setting to change.
+static hipFunction_t jit_function_checked(const std::string& src, const char* name, const char* remedy) {
+  hipFunction_t f = jit_function(src, name);
+  const int sb = scratch_bytes(f);
+  if (sb > 0)
+    throw std::runtime_error(std::string(name) + ": the generated kernel spills " + std::to_string(sb) +
+                             " bytes per lane to scratch and is not launched; " + remedy);
+  return f;
+}
+
 static hipFunction_t segment_kernel(cxh_tape_s* t, JitPlan& P, int s, bool rev) {
   hipFunction_t& f = rev ? P.rev[s] : P.fwd[s];
-  if (!f) f = jit_function(gen_segment(t, P, s, rev), rev ? "cxj_rev" : "cxj_fwd");
+  if (!f)
+    f = jit_function_checked(gen_segment(t, P, s, rev), rev ? "cxj_rev" : "cxj_fwd",
+                             rev ? "use sub-blocks of at most 256 instructions (CXH_JIT_SUBBLOCK), a shorter "
+                                   "checkpoint interval or the interpreter engine"
+                                 : "use a shorter checkpoint interval or the interpreter engine");
   return f;
 }
 
@@ -593,8 +622,32 @@ void jit_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
 
 int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every) { return plan_for(t, every).ck_rows; }
 
+// Forward-only evaluation chooses its own segment length: the default one,
+// halved until no forward segment kernel needs scratch.
+static JitPlan& eval_plan(cxh_tape_s* t) {
+  if (t->jit && t->jit->eval_every > 0) return plan_for(t, t->jit->eval_every);
+  for (int64_t every = default_every(static_cast<int64_t>(t->alg.size()));; every = (every + 1) / 2) {
+    JitPlan& P = plan_for(t, every);
+    bool ok = true;
+    for (int g = 0; g < P.n_seg && ok; ++g) {
+      if (P.fwd[g]) continue;
+      hipFunction_t f = jit_function(gen_segment(t, P, g, false), "cxj_fwd");
+      if (scratch_bytes(f) > 0) {
+        ok = false;
+      } else {
+        P.fwd[g] = f;
+      }
+    }
+    if (ok) {
+      t->jit->eval_every = every;
+      return P;
+    }
+    if (every <= 64) throw std::runtime_error("cxj_fwd: no segment length avoids scratch spills");
+  }
+}
+
 int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s) {
-  JitPlan& P = plan_for(t, default_every(static_cast<int64_t>(t->alg.size())));
+  JitPlan& P = eval_plan(t);
   const int64_t lds = round_up(n, 64);
   CxjArgs A{};
   A.arg = arg;
@@ -814,17 +867,19 @@ static int jac_prefetch() {
 // live (a reverse sweep reads forward values from far back), more than the
 // register file holds; left to the compiler they spill to scratch at one wave
 // per SIMD.  The generator therefore allocates registers itself: it walks the
-// segment with at most jac_reg_budget() values resident and, when over budget,
-// evicts the value whose next use is furthest away (Belady), storing it once
-// to a spill row (or straight to its boundary row when it is live out) and
-// reloading it just before its next use.  Segment live-ins and inputs are
-// loaded next to their first use instead of at the top.  Reloads go through a
-// pointer made opaque after the most recently computed value, so the compiler
-// can neither fold them into the store nor hoist them back up.
+// segment with at most R values resident and, when over budget, evicts the
+// value whose next use is furthest away (Belady), storing it once to a spill
+// row (or straight to its boundary row when it is live out) and reloading it
+// a few instructions before its next use.  Segment live-ins and inputs are
+// loaded next to their first use instead of at the top.  Each group of loads
+// or stores goes through its own opaque copy of the per-instance base pointer
+// (reloads: made after the most recently computed value), so the compiler can
+// neither fold a reload into its store nor hoist it back up, and no row offset
+// stays live across the kernel.
 //
-// State and spill rows are tiled [instance/64][row][64]: a row's offset is a
-// compile-time constant and accesses of a wave are 512 B contiguous.
-static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s, int* spill_rows) {
+// State and spill rows are tiled [group][instance/64][row][64]: a row's offset
+// is a compile-time constant and accesses of a wave are 512 B contiguous.
+static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s, int R, int* spill_rows) {
   const cxh_tape_s* t = j->tape;
   const int nd = j->ndir;
   const int d0 = g * J.group, d1 = std::min(nd, d0 + J.group);
@@ -1019,7 +1074,6 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
       if (val[v].kind != 3) val[v].uses.push_back(static_cast<int>(i));
 
   // ---- allocation and emission
-  const int R = jac_reg_budget();
   const int LA = jac_prefetch();
   std::string b;  // body
   b.reserve(static_cast<size_t>(t1 - t0) * (90 + 70 * G));
@@ -1250,12 +1304,12 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
 // Kernel of segment s: the bodies of all direction groups, blockIdx.y picking
 // the group, so that the groups run side by side (one wave per 64 instances
 // and group) rather than one after another.
-static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s, int* spill_rows) {
+static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s, int R, int* spill_rows) {
   std::string bodies;
   int sp = 0;
   for (int g = 0; g < J.n_group; ++g) {
     int ns = 0;
-    bodies += gen_tangent(j, J, g, s, &ns);
+    bodies += gen_tangent(j, J, g, s, R, &ns);
     sp = std::max(sp, ns);
   }
   if (spill_rows) *spill_rows = sp;
@@ -1281,10 +1335,14 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
   JitJac& J = jac_plan(j, n);
   for (int sg = 0; sg < J.n_seg; ++sg) {
     hipFunction_t& f = J.fn[sg];
-    if (!f) {
+    // the budget counts values, not the compiler's temporaries: where the
+    // compiler still needs scratch, regenerate with fewer values resident
+    for (int R = jac_reg_budget(); !f; R -= 24) {
+      if (R < 24) throw std::runtime_error("cxj_tan: no register budget avoids scratch spills");
       int ns = 0;
-      const std::string src = gen_tangent_kernel(j, J, sg, &ns);
-      f = jit_function(src, "cxj_tan");
+      hipFunction_t g = jit_function(gen_tangent_kernel(j, J, sg, R, &ns), "cxj_tan");
+      if (scratch_bytes(g) > 0) continue;
+      f = g;
       J.spill_rows = std::max<int64_t>(J.spill_rows, ns);
     }
   }
@@ -1330,7 +1388,7 @@ std::string jit_tangent_source(cxh_jac_s* j, int s, int* n_group, int* n_seg) {
   if (n_group) *n_group = J.n_group;
   if (n_seg) *n_seg = J.n_seg;
   if (s < 0 || s >= J.n_seg) throw std::runtime_error("segment out of range");
-  return gen_tangent_kernel(j, J, s, nullptr);
+  return gen_tangent_kernel(j, J, s, jac_reg_budget(), nullptr);
 }
 
 // ------------------------------------------------------------ mapaccum ---
@@ -1779,7 +1837,9 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
   if (n == 0) return 0;
   const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
   hipFunction_t& f = m->fn[{mode, mode == 0 ? 0 : K}];
-  if (!f) f = jit_function(gen_macc(m->tape, m->plan, mode, K), kMaccKernel[mode]);
+  if (!f)
+    f = jit_function_checked(gen_macc(m->tape, m->plan, mode, K), kMaccKernel[mode],
+                             "evaluate the unrolled chain with the interpreter engine instead");
   const int64_t lds = round_up(n, 64);
   const int64_t S = (m->plan.N + K - 1) / K;
   const int64_t ns = std::max<int64_t>(m->plan.ns, 1);
@@ -1854,7 +1914,8 @@ int cxh_batch_create(int32_t n_fun, const cxh_tape* tapes, const int64_t* counts
   }
   b->args.assign(n_fun, CxjArgs{});
   b->bound.assign(n_fun, 0);
-  b->fn = jit_function(gen_batch(b->tapes), "cxj_batch");
+  b->fn = jit_function_checked(gen_batch(b->tapes), "cxj_batch",
+                               "batch fewer or smaller Functions, or launch them one by one");
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   hip_check(hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
