@@ -160,11 +160,14 @@ def kkt(n=8192):
     rng = np.random.default_rng(20261206)
     _, A = W.ocp_kkt_batch(rng, n)
     At, Bt = _dev(A), _dev(rng.standard_normal((sp.size1(), n)))
-    for solver in ("ldl", "qr"):
-        Ls = Linsol("kkt", solver, sp)
+    for solver, engine in (("ldl", "jit"), ("ldl", "interp"), ("qr", None)):
+        t0 = time.time()
+        Ls = Linsol("kkt", solver, sp, {"engine": engine} if engine else None)
+        Ls.solve_batch(At, Bt)
+        ct = time.time() - t0
         ms = _timed(lambda: Ls.solve_batch(At, Bt), 5)
-        _out(k="kkt_" + solver, n=n, ms=ms, nnz=sp.nnz(), inst_per_s=n / ms * 1e3,
-             GBs=(sp.nnz() + 2 * sp.size1()) * 8 * n / ms / 1e6)
+        _out(k="kkt_" + solver, engine=engine, n=n, ms=ms, nnz=sp.nnz(), inst_per_s=n / ms * 1e3,
+             GBs=(sp.nnz() + 2 * sp.size1()) * 8 * n / ms / 1e6, first_call_s=ct)
 
 
 def mapsum(n=1 << 20):

@@ -797,7 +797,7 @@ static int jac_dir_group(int ndir, int64_t n) {
 }
 
 static JitJac& jac_plan(cxh_jac_s* j, int64_t n) {
-  const int want = std::min(std::max(j->ndir, 1), std::min(jac_dir_group(j->ndir, n), 64));
+  const int want = j->ndir == 0 ? 1 : std::min(std::max(j->ndir, 1), std::min(jac_dir_group(j->ndir, n), 64));
   if (j->jit && j->jit->group == want) return *j->jit;
   const cxh_tape_s* t = j->tape;
   auto J = std::make_shared<JitJac>();
@@ -805,7 +805,7 @@ static JitJac& jac_plan(cxh_jac_s* j, int64_t n) {
   const int64_t ni = static_cast<int64_t>(t->alg.size());
   J->group = want;
   J->n_group = std::max(1, (nd + J->group - 1) / J->group);
-  int64_t every = default_every(ni);
+  int64_t every = j->seg_len > 0 ? j->seg_len : default_every(ni);
   if (const char* e = std::getenv("CXH_JIT_TANSEG")) every = std::max(1, std::atoi(e));
   every = std::max<int64_t>(1, std::min(every, ni));
   J->n_seg = static_cast<int>((ni + every - 1) / every);
@@ -848,16 +848,18 @@ static JitJac& jac_plan(cxh_jac_s* j, int64_t n) {
   return *J;
 }
 
-// Register budget of one tangent kernel, in doubles held at once.
-static int jac_reg_budget() {
+// Starting register budget of a tangent kernel, in doubles held at once.
+static int jac_reg_budget(const cxh_jac_s* j) {
   if (const char* e = std::getenv("CXH_JIT_TANREGS")) return std::max(8, std::atoi(e));
-  return 200;
+  return j->reg_budget > 0 ? j->reg_budget : 200;
 }
 
-// Reloads are issued this many instructions ahead of their use.
-static int jac_prefetch() {
+// Reloads are issued this many instructions ahead of their use.  Kernels
+// with few wavefronts (nothing else to run while a reload is in flight) need
+// a longer lookahead than those that fill the device.
+static int jac_prefetch(const cxh_jac_s* j) {
   if (const char* e = std::getenv("CXH_JIT_PREFETCH")) return std::max(0, std::atoi(e));
-  return 8;
+  return j->prefetch >= 0 ? j->prefetch : 8;
 }
 
 // Kernel of direction group g, segment s.  Block 0 of the asens table carries
@@ -1074,7 +1076,7 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
       if (val[v].kind != 3) val[v].uses.push_back(static_cast<int>(i));
 
   // ---- allocation and emission
-  const int LA = jac_prefetch();
+  const int LA = jac_prefetch(j);
   std::string b;  // body
   b.reserve(static_cast<size_t>(t1 - t0) * (90 + 70 * G));
   std::set<std::pair<int, int>> res;  // (next use, value) of resident values
@@ -1337,7 +1339,7 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
     hipFunction_t& f = J.fn[sg];
     // the budget counts values, not the compiler's temporaries: where the
     // compiler still needs scratch, regenerate with fewer values resident
-    for (int R = jac_reg_budget(); !f; R -= 24) {
+    for (int R = jac_reg_budget(j); !f; R -= 24) {
       if (R < 24) throw std::runtime_error("cxj_tan: no register budget avoids scratch spills");
       int ns = 0;
       hipFunction_t g = jit_function(gen_tangent_kernel(j, J, sg, R, &ns), "cxj_tan");
@@ -1388,8 +1390,178 @@ std::string jit_tangent_source(cxh_jac_s* j, int s, int* n_group, int* n_seg) {
   if (n_group) *n_group = J.n_group;
   if (n_seg) *n_seg = J.n_seg;
   if (s < 0 || s >= J.n_seg) throw std::runtime_error("segment out of range");
-  return gen_tangent_kernel(j, J, s, jac_reg_budget(), nullptr);
+  return gen_tangent_kernel(j, J, s, jac_reg_budget(j), nullptr);
 }
+
+// ------------------------------------------------------ specialised LDL ---
+//
+// casadi_ldl + casadi_ldl_solve (casadi/core/runtime/casadi_ldl.hpp:24-109)
+// for one sparsity pattern, as a straight-line program over the instance's
+// own values: every operation of the batched LDL kernel (linsol_kernels.hip),
+// in its order, with the pattern's indices resolved at generation time --
+// including the products with entries of the work vector that are
+// structurally zero (v - l*0 is v except for the sign of a zero result, so
+// dropping them would not be bit-exact).  The program is evaluated by the
+// register-budgeted generator of the tangent sweeps with no seed directions,
+// as one kernel: the factor keeps most of L^T live until the solve, so a
+// segment boundary would carry all of it through memory.
+//
+// Inputs: block 0 = A's nonzeros, block 1 = the right-hand sides (nrhs blocks
+// of n).  Outputs: block 0 = the solutions (same layout as block 1; the same
+// memory may hold both), block 1 = the number of zero or non-finite pivots.
+
+struct LdlJit {
+  cxh_tape_s tape;
+  cxh_jac_s jac;
+  int nrhs = 0;
+  DeviceBuffer cnt;
+};
+
+namespace {
+
+struct TapeEmit {
+  std::vector<Instr> alg;
+  int ws = 0;
+  int op(int code, int x, int y) {
+    Instr e{};
+    e.op = code;
+    e.i0 = ws;
+    e.i1 = x;
+    e.i2 = y;
+    alg.push_back(e);
+    return ws++;
+  }
+  int konst(double d) {
+    Instr e{};
+    e.op = OP_CONST;
+    e.i0 = ws;
+    e.d = d;
+    alg.push_back(e);
+    return ws++;
+  }
+  int input(int blk, int nz) {
+    Instr e{};
+    e.op = OP_INPUT;
+    e.i0 = ws;
+    e.i1 = blk;
+    e.i2 = nz;
+    alg.push_back(e);
+    return ws++;
+  }
+  void output(int blk, int nz, int x) {
+    Instr e{};
+    e.op = OP_OUTPUT;
+    e.i0 = blk;
+    e.i1 = x;
+    e.i2 = nz;
+    alg.push_back(e);
+  }
+};
+
+__global__ void count_to_status(const double* __restrict__ cnt, int64_t n, int32_t* __restrict__ status) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) status[k] = static_cast<int32_t>(cnt[k]);
+}
+
+}  // namespace
+
+std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
+                                       const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
+                                       const std::vector<int>& p, int nrhs) {
+  TapeEmit e;
+  const int Z = e.konst(0.0);
+  const int nnz_a = a_colind[n], nnz_lt = lt_colind[n];
+  std::vector<int> w(n, Z), lt(nnz_lt, Z), d(n, Z);
+  // scatter the permuted A into L^T and D
+  for (int c = 0; c < n; ++c) {
+    const int c1 = p[c];
+    for (int k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = e.input(0, k);
+    for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) lt[k] = w[p[lt_row[k]]];
+    d[c] = w[p[c]];
+    for (int k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = Z;
+  }
+  // up-looking factorisation, column by column
+  for (int c = 0; c < n; ++c) {
+    int dc = d[c];
+    for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) {
+      const int r = lt_row[k];
+      int v = lt[k];
+      for (int k2 = lt_colind[r]; k2 < lt_colind[r + 1]; ++k2) v = e.op(OP_SUB, v, e.op(OP_MUL, lt[k2], w[lt_row[k2]]));
+      w[r] = v;
+      v = e.op(OP_DIV, v, d[r]);
+      lt[k] = v;
+      dc = e.op(OP_SUB, dc, e.op(OP_MUL, w[r], v));
+    }
+    d[c] = dc;
+    for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) w[lt_row[k]] = Z;
+  }
+  // zero or non-finite pivots: !(d != 0 && d - d == 0)
+  int cnt = Z;
+  for (int c = 0; c < n; ++c) {
+    const int nz = e.op(OP_NE, d[c], Z);
+    const int fin = e.op(OP_EQ, e.op(OP_SUB, d[c], d[c]), Z);
+    const int bad = e.op(OP_NOT, e.op(OP_AND, nz, fin), 0);
+    cnt = e.op(OP_ADD, cnt, bad);
+  }
+  e.output(1, 0, cnt);
+  // x = P' L'^-1 D^-1 L^-1 P b for every right-hand side
+  for (int q = 0; q < nrhs; ++q) {
+    for (int i = 0; i < n; ++i) w[i] = e.input(1, q * n + p[i]);
+    for (int c = 0; c < n; ++c) {
+      int v = w[c];
+      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) v = e.op(OP_SUB, v, e.op(OP_MUL, lt[k], w[lt_row[k]]));
+      w[c] = v;
+    }
+    for (int i = 0; i < n; ++i) w[i] = e.op(OP_DIV, w[i], d[i]);
+    for (int c = n - 1; c >= 0; --c) {
+      const int wc = w[c];
+      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k)
+        w[lt_row[k]] = e.op(OP_SUB, w[lt_row[k]], e.op(OP_MUL, wc, lt[k]));
+    }
+    for (int i = 0; i < n; ++i) e.output(0, q * n + p[i], w[i]);
+  }
+  for (Instr& x : e.alg)  // unary operations: second operand = first, as in the tapes
+    if (x.op == OP_NOT) x.i2 = x.i1;
+  auto L = std::make_shared<LdlJit>();
+  L->nrhs = nrhs;
+  L->tape.alg = std::move(e.alg);
+  L->tape.ws = e.ws;
+  L->tape.nnz_in = {nnz_a, static_cast<int64_t>(n) * nrhs};
+  L->tape.nnz_out = {static_cast<int64_t>(n) * nrhs, 1};
+  L->tape.engine = CXH_ENGINE_JIT;
+  L->jac.tape = &L->tape;
+  L->jac.ndir = 0;
+  L->jac.in_off = {0, nnz_a};
+  L->jac.out_off = {0, n * nrhs};
+  L->jac.seg_len = static_cast<int64_t>(L->tape.alg.size());
+  // measured on the 274 x 274 KKT batch (8192 instances = 128 wavefronts):
+  // budget 160 with reloads 64 instructions ahead, 0.34 ms against 0.80 ms
+  // at the tangent sweeps' 200 / 8 (whose kernel also needed scratch)
+  L->jac.reg_budget = 160;
+  L->jac.prefetch = 64;
+  return L;
+}
+
+int jit_ldl_run(LdlJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x, int64_t x_s,
+                int64_t x_t, int32_t* status, hipStream_t s) {
+  const int64_t ld = round_up(std::max<int64_t>(n_inst, 1), 64);
+  L.cnt.reserve(static_cast<size_t>(ld) * sizeof(double));
+  IOTable arg{}, res{};
+  arg.a[0] = {a, a_s, a_t};
+  arg.a[1] = {x, x_s, x_t};
+  res.a[0] = {x, x_s, x_t};
+  res.a[1] = {L.cnt.as<double>(), ld, 1};
+  const int rc = jit_map_jac(&L.jac, n_inst, arg, res, nullptr, 0, 0, s);
+  if (rc) return rc;
+  if (status) {
+    hipLaunchKernelGGL(count_to_status, dim3(static_cast<unsigned>((n_inst + 255) / 256)), dim3(256), 0, s,
+                       L.cnt.as<double>(), n_inst, status);
+    hip_check(hipGetLastError(), "count_to_status launch");
+  }
+  return 0;
+}
+
+std::string jit_ldl_source(LdlJit& L) { return jit_tangent_source(&L.jac, 0, nullptr, nullptr); }
 
 // ------------------------------------------------------------ mapaccum ---
 //

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -317,6 +319,9 @@ struct cxh_ldl_s {
   std::vector<int> p;
   DeviceBuffer d_a_colind, d_a_row, d_lt_colind, d_lt_row, d_p;
   DeviceBuffer lt_vals, d_vals, w;
+  bool uploaded = false;                               // pattern arrays on the device
+  int engine = CXH_ENGINE_JIT;                          // specialised program or the generic kernel
+  std::map<int, std::shared_ptr<sxh::LdlJit>> jit;     // specialised factor + solve per nrhs
 };
 
 struct cxh_qr_s {
@@ -346,11 +351,6 @@ int cxh_ldl_create(const int64_t* sp_a, const int64_t* sp_lt, const int64_t* p, 
     for (int k = l->lt.colind[c]; k < l->lt.colind[c + 1]; ++k)
       if (l->lt.row[k] >= c) return cxh_fail("cxh_ldl_create: L^T must be strictly upper triangular");
   l->p = to_perm(p, n, n);
-  upload_ints(l->d_a_colind, l->a.colind);
-  upload_ints(l->d_a_row, l->a.row);
-  upload_ints(l->d_lt_colind, l->lt.colind);
-  upload_ints(l->d_lt_row, l->lt.row);
-  upload_ints(l->d_p, l->p);
   *out = l.release();
   return 0;
   CXH_CATCH
@@ -371,6 +371,21 @@ int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, 
   if (!a || (!x && nrhs > 0)) return cxh_fail("null data pointer");
   const int n = l->lt.ncol;
   const int64_t nnz_a = l->a.colind.back(), nnz_lt = l->lt.colind.back();
+  if (l->engine == CXH_ENGINE_JIT) {
+    auto& J = l->jit[nrhs];
+    if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs);
+    const bool soa = layout == CXH_SOA;
+    return jit_ldl_run(*J, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x, soa ? n_inst : 1,
+                       soa ? 1 : static_cast<int64_t>(n) * nrhs, status, static_cast<hipStream_t>(stream));
+  }
+  if (!l->uploaded) {  // the generic kernel reads the pattern from device memory
+    upload_ints(l->d_a_colind, l->a.colind);
+    upload_ints(l->d_a_row, l->a.row);
+    upload_ints(l->d_lt_colind, l->lt.colind);
+    upload_ints(l->d_lt_row, l->lt.row);
+    upload_ints(l->d_p, l->p);
+    l->uploaded = true;
+  }
   const int bs = 64;
   const int64_t grid = (n_inst + bs - 1) / bs;
   LdlArgs A{};
@@ -406,6 +421,32 @@ int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, 
     hipLaunchKernelGGL(ldl_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
   }
   hip_check(hipGetLastError(), "ldl_kernel launch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_ldl_engine(cxh_ldl l, int32_t engine) {
+  CXH_TRY
+  if (!l) return cxh_fail("null ldl handle");
+  if (engine != CXH_ENGINE_INTERP && engine != CXH_ENGINE_JIT) return cxh_fail("cxh_ldl_engine: unknown engine");
+  l->engine = engine;
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_ldl_jit_source(cxh_ldl l, int32_t nrhs, char* buf, int64_t cap, int64_t* len) {
+  CXH_TRY
+  if (!l) return cxh_fail("null ldl handle");
+  if (nrhs < 0) return cxh_fail("negative nrhs");
+  auto& J = l->jit[nrhs];
+  if (!J) J = jit_ldl_create(l->lt.ncol, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs);
+  const std::string src = jit_ldl_source(*J);
+  if (len) *len = static_cast<int64_t>(src.size());
+  if (buf && cap > 0) {
+    const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, src.data(), m);
+    buf[m] = '\0';
+  }
   return 0;
   CXH_CATCH
 }

@@ -108,9 +108,21 @@ struct cxh_jac_s {
   std::vector<int2> scatter;    // [flat_out_nz][ndir] -> up to two Jacobian nonzeros
   sxh::DeviceBuffer d_seed, d_scatter, d_in_off, d_out_off;
   std::shared_ptr<sxh::JitJac> jit;  // specialised tangent kernels (jit.hip)
+  int64_t seg_len = 0;               // instructions per kernel of the tangent sweep (0: default)
+  int reg_budget = 0;                // starting register budget of its kernels (0: default)
+  int prefetch = -1;                 // reload lookahead in instructions (-1: default)
 };
 
 namespace sxh {
 int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
                 int64_t jac_s_inst, hipStream_t s);
+
+// Batched LDL factor + solve specialised to one sparsity pattern (jit.hip).
+struct LdlJit;
+std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
+                                       const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
+                                       const std::vector<int>& p, int nrhs);
+int jit_ldl_run(LdlJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x, int64_t x_s,
+                int64_t x_t, int32_t* status, hipStream_t s);
+std::string jit_ldl_source(LdlJit& L);
 }

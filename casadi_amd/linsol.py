@@ -43,6 +43,10 @@ class Linsol:
             h = L.P()
             L.check(L.cxh_ldl_create(spa.ctypes.data_as(L.PI64), splt.ctypes.data_as(L.PI64),
                                      p.ctypes.data_as(L.PI64), C.byref(h)), "cxh_ldl_create")
+            engine = opts.get("engine", "jit")
+            if engine not in ("jit", "interp"):
+                raise ValueError(f"unknown engine '{engine}' (jit, interp)")
+            L.check(L.cxh_ldl_engine(h, 1 if engine == "jit" else 0), "cxh_ldl_engine")
         else:
             self.eps = float(opts.get("eps", 1e-12))
             self.sp_v, self.sp_r, self.prinv, self.pc = sp.qr_sparse(amd=opts.get("amd", True))
@@ -61,6 +65,14 @@ class Linsol:
         if fn is not None:
             fn(h)
         self._h = None
+
+    def jit_source(self, nrhs=1):
+        """Generated source of the specialised LDL factor + solve kernel."""
+        n = L.I64()
+        L.check(L.cxh_ldl_jit_source(self._h, int(nrhs), None, 0, C.byref(n)), "cxh_ldl_jit_source")
+        buf = C.create_string_buffer(n.value + 1)
+        L.check(L.cxh_ldl_jit_source(self._h, int(nrhs), buf, n.value + 1, C.byref(n)), "cxh_ldl_jit_source")
+        return buf.value.decode()
 
     def name(self):
         return self._name

@@ -260,6 +260,15 @@ int cxh_ldl_destroy(cxh_ldl l);
  * Returns 0; use cxh_ldl_count_failures for the host-side count.           */
 int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, int32_t nrhs,
                          int32_t* status, int32_t layout, void* stream);
+/* Engine of cxh_ldl_factor_solve: CXH_ENGINE_JIT (default) compiles, at the
+ * first call per nrhs, the factor + solve of this pattern into one
+ * straight-line kernel (every operation of casadi_ldl / casadi_ldl_solve in
+ * its order, indices resolved: bit-identical results); CXH_ENGINE_INTERP
+ * runs the generic kernel that reads the pattern from memory.  Replaces
+ * nothing in the reference (a launch choice of the batched plugin).        */
+int cxh_ldl_engine(cxh_ldl l, int32_t engine);
+/* Introspection: generated source of the specialised kernel for nrhs.     */
+int cxh_ldl_jit_source(cxh_ldl l, int32_t nrhs, char* buf, int64_t cap, int64_t* len);
 
 /* Batched sparse Householder QR (casadi_qr / casadi_qr_solve,
  * casadi/core/runtime/casadi_qr.hpp:53-200, LinsolQr

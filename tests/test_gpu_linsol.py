@@ -37,6 +37,24 @@ def test_ldl_batch_bit_exact(cuda, batch):
         assert np.array_equal(X[:, k], xr), k
 
 
+def test_ldl_specialised_and_generic_engines_agree(cuda, batch):
+    """The specialised factor + solve (straight-line program of the pattern)
+    and the generic kernel give the same bits, zero-pivot counts included."""
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    n_inst = 4099
+    v = vals[:, :n_inst].copy()
+    v[:, 11] = 0.0
+    A = torch.tensor(v, dtype=torch.float64, device="cuda")
+    B = torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda")
+    print(f"[case] ldl engines instances={n_inst}", flush=True)
+    Xj, sj = Linsol("j", "ldl", sp, {"engine": "jit"}).solve_batch(A, B)
+    Xi, si = Linsol("i", "ldl", sp, {"engine": "interp"}).solve_batch(A, B)
+    assert torch.equal(sj, si) and int(sj[11]) == sp.nrow
+    assert np.array_equal(Xj.cpu().numpy(), Xi.cpu().numpy(), equal_nan=True)
+
+
 def test_qr_batch_bit_exact(cuda, batch):
     import torch
     from casadi_amd.linsol import Linsol

@@ -253,3 +253,39 @@ def test_mapaccum_loop_kernels_run_on_host(K):
     (_,), (r0, rU) = O.adj_map(F, n, [x0.T, U.T], [seed.T])
     assert O.rel_err(g0, r0.T) <= 1e-13 and O.rel_err(gU, rU.T) <= 1e-13
     assert np.array_equal(h0, g0) and np.array_equal(hU, gU)
+
+
+def test_specialised_ldl_runs_on_host():
+    """The straight-line LDL factor + solve generated for a KKT pattern (5
+    stages, two right-hand sides), executed on the host, against the oracle's
+    casadi_ldl / casadi_ldl_solve (itself bit-exact against the reference
+    runtime): bit for bit, a zero matrix included (every pivot counted)."""
+    from casadi_amd import workloads as W
+    from casadi_amd.linsol import Linsol
+    sp, _, _ = W.ocp_kkt_sparsity(N=5)
+    Lt, p = sp.ldl(amd=True)
+    src = Linsol("k", "ldl", sp).jit_source(2)
+    n, nn = 5, sp.size1()
+    a = np.zeros((sp.nnz(), n))
+    for k in range(n - 1):
+        K = W.ocp_kkt_values(np.random.default_rng(k), N=5)
+        a[:, k] = K[sp.row, sp.get_col()]
+    b = np.random.default_rng(9).standard_normal((2 * nn, n))
+    x = np.zeros((2 * nn, n))
+    cnt = np.zeros((1, n))
+    st, spill, _, lds = _tiled([src], n, 1)
+    with tempfile.TemporaryDirectory() as d:
+        (_, fn), = _compile([src], d, 1)
+        A = CxjArgs()
+        A.n, A.lds = n, lds
+        A.arg = _table([a, b], n)
+        A.res = _table([x, cnt], n)
+        A.spill = spill.ctypes.data
+        A.st_in, A.st_out = st[0].ctypes.data, st[1].ctypes.data
+        fn(C.byref(A), n)
+    for k in range(n):
+        lt, dd = O.ldl(sp.compressed(), a[:, k], Lt.compressed(), p)
+        xr = O.ldl_solve(b[:, k], Lt.compressed(), lt, dd, p)
+        assert np.array_equal(x[:, k], xr, equal_nan=True), k
+        assert cnt[0, k] == np.sum(~((dd != 0) & np.isfinite(dd)))
+    assert cnt[0, n - 1] == nn
