@@ -95,6 +95,8 @@ cxh_ldl_destroy = _opt("cxh_ldl_destroy", I32, P)
 cxh_ldl_factor_solve = _opt("cxh_ldl_factor_solve", I32, P, I64, P, P, I32, P, I32, P)
 cxh_ldl_engine = _opt("cxh_ldl_engine", I32, P, I32)
 cxh_ldl_jit_source = _opt("cxh_ldl_jit_source", I32, P, I32, C.c_char_p, I64, PI64)
+cxh_qr_engine = _opt("cxh_qr_engine", I32, P, I32)
+cxh_qr_jit_source = _opt("cxh_qr_jit_source", I32, P, I32, I32, I32, C.c_char_p, I64, PI64, C.POINTER(I32))
 cxh_qr_create = _opt("cxh_qr_create", I32, PI64, PI64, PI64, PI64, PI64, D, PP)
 cxh_qr_destroy = _opt("cxh_qr_destroy", I32, P)
 cxh_qr_factor_solve = _opt("cxh_qr_factor_solve", I32, P, I64, P, P, I32, I32, P, I32, P)

@@ -160,9 +160,9 @@ def kkt(n=8192):
     rng = np.random.default_rng(20261206)
     _, A = W.ocp_kkt_batch(rng, n)
     At, Bt = _dev(A), _dev(rng.standard_normal((sp.size1(), n)))
-    for solver, engine in (("ldl", "jit"), ("ldl", "interp"), ("qr", None)):
+    for solver, engine in (("ldl", "jit"), ("ldl", "interp"), ("qr", "jit"), ("qr", "interp")):
         t0 = time.time()
-        Ls = Linsol("kkt", solver, sp, {"engine": engine} if engine else None)
+        Ls = Linsol("kkt", solver, sp, {"engine": engine})
         Ls.solve_batch(At, Bt)
         ct = time.time() - t0
         ms = _timed(lambda: Ls.solve_batch(At, Bt), 5)

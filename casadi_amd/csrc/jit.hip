@@ -772,12 +772,18 @@ struct JitJac {
   int group = 1, n_group = 1, n_seg = 1;
   std::vector<int> t0;                                         // segment boundaries
   std::vector<std::vector<int>> live;                          // live slots per boundary
+  std::vector<std::vector<int>> def;                           // [b][i]: instruction that wrote live[b][i] (-1: none)
   std::vector<std::vector<std::vector<std::pair<int, int>>>> tnz;  // [g][b]: nonzero (slot, dir)
-  int64_t max_rows = 0;                                        // boundary state rows
+  std::vector<std::unordered_map<int64_t, int>> home;          // [g]: home row of a value crossing a boundary
+  int64_t home_rows = 0;                                       // home rows per group
   int64_t spill_rows = 0;                                      // spill rows (max over kernels)
   std::vector<hipFunction_t> fn;                               // [s], all groups
   DeviceBuffer st, spill, zero;
 };
+
+// Key of a value in the home map: its defining instruction and the direction
+// within the group (0: the primal).
+static int64_t home_key(int def, int dir_in_group) { return static_cast<int64_t>(def) * 128 + dir_in_group; }
 
 // Directions per group.  Each group is a body of its own over all instances
 // (n/64 waves) and the kernels hold about one wave per SIMD (register-bound),
@@ -840,8 +846,70 @@ static JitJac& jac_plan(cxh_jac_s* j, int64_t n) {
         ++b;
       }
     }
-    for (int bb = 1; bb < J->n_seg; ++bb)
-      J->max_rows = std::max<int64_t>(J->max_rows, static_cast<int64_t>(J->live[bb].size() + J->tnz[g][bb].size()));
+  }
+  // the instruction that wrote each live slot at each boundary
+  J->def.assign(J->n_seg + 1, {});
+  {
+    std::vector<int> defk(t->ws, -1);
+    int b = 1;
+    for (int64_t k = 0; k < ni; ++k) {
+      const Instr& e = t->alg[k];
+      if (e.op != OP_OUTPUT) defk[e.i0] = static_cast<int>(k);
+      while (b < J->n_seg && J->t0[b] == k + 1) {
+        for (int slot : J->live[b]) J->def[b].push_back(defk[slot]);
+        ++b;
+      }
+    }
+  }
+  // Home rows.  A value that crosses a boundary keeps one row from the
+  // segment that defines it to the last segment that reads it: it is stored
+  // once (when evicted or at the end of its segment) and later segments load
+  // it where they use it, instead of every boundary copying every live value.
+  // Rows are reused between values whose segment spans do not overlap.
+  auto seg_of = [&](int k) {
+    return static_cast<int>(std::upper_bound(J->t0.begin(), J->t0.end(), k) - J->t0.begin()) - 1;
+  };
+  J->home.assign(J->n_group, {});
+  for (int g = 0; g < J->n_group; ++g) {
+    const int d0 = g * J->group;
+    std::map<int64_t, std::pair<int, int>> span;  // key -> (defining segment, last reading segment)
+    for (int b = 1; b < J->n_seg; ++b) {
+      std::unordered_map<int, int> slot_def;
+      for (size_t i = 0; i < J->live[b].size(); ++i) {
+        const int def = J->def[b][i];
+        slot_def[J->live[b][i]] = def;
+        if (def < 0) continue;
+        auto it = span.emplace(home_key(def, 0), std::make_pair(seg_of(def), b)).first;
+        it->second.second = std::max(it->second.second, b);
+      }
+      for (const auto& sd : J->tnz[g][b]) {
+        const int def = slot_def.at(sd.first);
+        auto it = span.emplace(home_key(def, sd.second - d0 + 1), std::make_pair(seg_of(def), b)).first;
+        it->second.second = std::max(it->second.second, b);
+      }
+    }
+    std::vector<std::vector<int64_t>> starts(J->n_seg + 1), ends(J->n_seg + 1);
+    for (const auto& kv : span) {
+      starts[kv.second.first].push_back(kv.first);
+      ends[kv.second.second].push_back(kv.first);
+    }
+    std::set<int> free_rows;
+    int rows = 0;
+    for (int sg = 0; sg < J->n_seg; ++sg) {
+      if (sg > 0)
+        for (int64_t key : ends[sg - 1]) free_rows.insert(J->home[g].at(key));
+      for (int64_t key : starts[sg]) {
+        int r;
+        if (free_rows.empty()) {
+          r = rows++;
+        } else {
+          r = *free_rows.begin();
+          free_rows.erase(free_rows.begin());
+        }
+        J->home[g][key] = r;
+      }
+    }
+    J->home_rows = std::max<int64_t>(J->home_rows, rows);
   }
   J->fn.assign(J->n_seg, nullptr);
   j->jit = J;
@@ -932,21 +1000,24 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
   std::vector<int64_t> cur(t->ws, 0);
   std::vector<uint64_t> mask(t->ws, 0);
   auto vof = [&](int64_t c, int d) { return vid.at(key(c, d)); };
+  // live-ins: read from their home rows where used (never-written slots: 0)
   const std::vector<int>& lin = J.live[s];
+  const auto& home = J.home[g];
+  std::unordered_map<int, int> in_def;
   for (size_t r = 0; r < lin.size(); ++r) {
     const int64_t c = -(lin[r] + 1);
     cur[lin[r]] = c;
-    const int v = add(c, -1, s > 0 ? 1 : 3, s > 0 ? Vn(c) : std::string("0.0"));
-    val[v].in_row = static_cast<int>(r);
+    const int def = s > 0 ? J.def[s][r] : -1;
+    in_def[lin[r]] = def;
+    const int v = add(c, -1, def >= 0 ? 1 : 3, def >= 0 ? Vn(c) : std::string("0.0"));
+    if (def >= 0) val[v].in_row = home.at(home_key(def, 0));
   }
-  if (s > 0) {
-    int r = static_cast<int>(lin.size());
+  if (s > 0)
     for (const auto& sd : J.tnz[g][s]) {
       mask[sd.first] |= uint64_t(1) << (sd.second - d0);
       const int v = add(-(sd.first + 1), sd.second, 1, Tn(-(sd.first + 1), sd.second));
-      val[v].in_row = r++;
+      val[v].in_row = home.at(home_key(in_def.at(sd.first), sd.second - d0 + 1));
     }
-  }
   // sin and cos of one operand in this segment: one sincos gives both
   std::unordered_set<int64_t> pair_ops;
   {
@@ -1054,15 +1125,22 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
     }
   }
   // the end: live-out values (state rows of the next segment)
+  // values this segment defines that later segments read: to their home rows
   std::vector<std::pair<int, int>> outs;  // (row, value id or -1 for a structural zero)
   if (!last_seg) {
     Op op;
     op.k = t1;
-    int r = 0;
-    for (int slot : J.live[s + 1]) outs.push_back({r++, vof(cur[slot], -1)});
+    std::unordered_map<int, int> out_def;
+    for (size_t i = 0; i < J.live[s + 1].size(); ++i) {
+      const int slot = J.live[s + 1][i], def = J.def[s + 1][i];
+      out_def[slot] = def;
+      if (def >= t0 && def < t1) outs.push_back({home.at(home_key(def, 0)), vof(cur[slot], -1)});
+    }
     for (const auto& sd : J.tnz[g][s + 1]) {
+      const int def = out_def.at(sd.first);
+      if (def < t0 || def >= t1) continue;  // a live-in: already home
       const bool nzv = mask[sd.first] >> (sd.second - d0) & 1;
-      outs.push_back({r++, nzv ? vof(cur[sd.first], sd.second) : -1});
+      outs.push_back({home.at(home_key(def, sd.second - d0 + 1)), nzv ? vof(cur[sd.first], sd.second) : -1});
     }
     for (auto& ro : outs)
       if (ro.second >= 0 && val[ro.second].kind != 3) {
@@ -1318,7 +1396,7 @@ static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s
   std::string o = kSxOpsSource;
   o += kPrologue;
   o.reserve(o.size() + bodies.size() + 1024);
-  o += "#define CXJ_TROWS " + std::to_string(J.max_rows) + "\n#define CXJ_SROWS " + std::to_string(sp) + "\n";
+  o += "#define CXJ_TROWS " + std::to_string(J.home_rows) + "\n#define CXJ_SROWS " + std::to_string(sp) + "\n";
   o += bodies;
   o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k, const int grp) {\n"
        "  switch (grp) {\n";
@@ -1349,8 +1427,8 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
     }
   }
   const int64_t lds = round_up(n, 64);  // instances, padded to whole tiles of 64
-  const int64_t half = static_cast<int64_t>(J.n_group) * J.max_rows * lds;
-  if (J.n_seg > 1) J.st.reserve(2 * static_cast<size_t>(std::max<int64_t>(half, 1)) * sizeof(double));
+  const int64_t homes = static_cast<int64_t>(J.n_group) * J.home_rows * lds;
+  if (homes > 0) J.st.reserve(static_cast<size_t>(homes) * sizeof(double));
   if (J.spill_rows > 0) J.spill.reserve(static_cast<size_t>(J.n_group) * J.spill_rows * lds * sizeof(double));
   double* st = J.st.as<double>();
   CxjArgs A{};
@@ -1374,9 +1452,9 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
   A.spill = J.spill_rows > 0 ? J.spill.as<double>() : nullptr;
   const int block = j->tape->block;
   const unsigned grid = static_cast<unsigned>((n + block - 1) / block);
+  A.st_in = homes > 0 ? st : nullptr;  // home rows: read and written in place
+  A.st_out = homes > 0 ? st : nullptr;
   for (int sg = 0; sg < J.n_seg; ++sg) {
-    A.st_in = st ? st + static_cast<int64_t>(sg % 2) * half : nullptr;
-    A.st_out = st ? st + static_cast<int64_t>((sg + 1) % 2) * half : nullptr;
     void* params[] = {&A};
     hip_check(hipModuleLaunchKernel(J.fn[sg], grid, static_cast<unsigned>(J.n_group), 1, block, 1, 1, 0, s, params,
                                     nullptr),
@@ -1410,7 +1488,7 @@ std::string jit_tangent_source(cxh_jac_s* j, int s, int* n_group, int* n_seg) {
 // of n).  Outputs: block 0 = the solutions (same layout as block 1; the same
 // memory may hold both), block 1 = the number of zero or non-finite pivots.
 
-struct LdlJit {
+struct LinsolJit {
   cxh_tape_s tape;
   cxh_jac_s jac;
   int nrhs = 0;
@@ -1439,6 +1517,17 @@ struct TapeEmit {
     alg.push_back(e);
     return ws++;
   }
+  int un(int code, int x) { return op(code, x, x); }
+  // c ? a : b for c in {0, 1}, exactly (a selected -0 stays -0): the sum of
+  // the two if_else_zero terms is right up to the sign of a zero, and the
+  // sign comes from the same selection applied to copysign(1, .).
+  int select(int c, int a, int b, int one) {
+    const int nc = un(OP_NOT, c);
+    const int mag = op(OP_ADD, op(OP_IF_ELSE_ZERO, c, a), op(OP_IF_ELSE_ZERO, nc, b));
+    const int sgn = op(OP_ADD, op(OP_IF_ELSE_ZERO, c, op(OP_COPYSIGN, one, a)),
+                       op(OP_IF_ELSE_ZERO, nc, op(OP_COPYSIGN, one, b)));
+    return op(OP_COPYSIGN, mag, sgn);
+  }
   int input(int blk, int nz) {
     Instr e{};
     e.op = OP_INPUT;
@@ -1465,7 +1554,7 @@ __global__ void count_to_status(const double* __restrict__ cnt, int64_t n, int32
 
 }  // namespace
 
-std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
+std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
                                        const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
                                        const std::vector<int>& p, int nrhs) {
   TapeEmit e;
@@ -1500,7 +1589,7 @@ std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, 
   for (int c = 0; c < n; ++c) {
     const int nz = e.op(OP_NE, d[c], Z);
     const int fin = e.op(OP_EQ, e.op(OP_SUB, d[c], d[c]), Z);
-    const int bad = e.op(OP_NOT, e.op(OP_AND, nz, fin), 0);
+    const int bad = e.un(OP_NOT, e.op(OP_AND, nz, fin));
     cnt = e.op(OP_ADD, cnt, bad);
   }
   e.output(1, 0, cnt);
@@ -1520,9 +1609,7 @@ std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, 
     }
     for (int i = 0; i < n; ++i) e.output(0, q * n + p[i], w[i]);
   }
-  for (Instr& x : e.alg)  // unary operations: second operand = first, as in the tapes
-    if (x.op == OP_NOT) x.i2 = x.i1;
-  auto L = std::make_shared<LdlJit>();
+  auto L = std::make_shared<LinsolJit>();
   L->nrhs = nrhs;
   L->tape.alg = std::move(e.alg);
   L->tape.ws = e.ws;
@@ -1542,7 +1629,7 @@ std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, 
   return L;
 }
 
-int jit_ldl_run(LdlJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x, int64_t x_s,
+int jit_linsol_run(LinsolJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x, int64_t x_s,
                 int64_t x_t, int32_t* status, hipStream_t s) {
   const int64_t ld = round_up(std::max<int64_t>(n_inst, 1), 64);
   L.cnt.reserve(static_cast<size_t>(ld) * sizeof(double));
@@ -1561,7 +1648,116 @@ int jit_ldl_run(LdlJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t
   return 0;
 }
 
-std::string jit_ldl_source(LdlJit& L) { return jit_tangent_source(&L.jac, 0, nullptr, nullptr); }
+std::string jit_linsol_source(LinsolJit& L, int segment) {
+  return jit_tangent_source(&L.jac, segment, nullptr, nullptr);
+}
+
+int jit_linsol_segments(LinsolJit& L) {
+  int ns = 0;
+  jit_tangent_source(&L.jac, 0, nullptr, &ns);
+  return ns;
+}
+
+// casadi_qr + casadi_qr_singular + casadi_qr_solve
+// (casadi/core/runtime/casadi_qr.hpp:24-230) for one sparsity pattern, the
+// same way: every operation of the batched QR kernel (linsol_kernels.hip) in
+// its order, the selections of casadi_house exact (LinsolJit::select).  The
+// program is about four times the LDL's, so it runs in segments whose values
+// that cross a boundary keep home rows.
+std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<int>& a_colind,
+                                         const std::vector<int>& a_row, const std::vector<int>& v_colind,
+                                         const std::vector<int>& v_row, const std::vector<int>& r_colind,
+                                         const std::vector<int>& r_row, const std::vector<int>& prinv,
+                                         const std::vector<int>& pc, double eps, int nrhs, bool tr) {
+  TapeEmit e;
+  const int Z = e.konst(0.0), ONE = e.konst(1.0), TWO = e.konst(2.0), M1 = e.konst(-1.0), EPS = e.konst(eps);
+  const int nnz_a = a_colind[ncol];
+  std::vector<int> x(nrow, Z), v(v_colind[ncol], Z), r(r_colind[ncol], Z), beta(ncol, Z);
+  auto add_dot = [&](const std::vector<int>& w, int c) {  // 0.0 + sum v[k] * w[v_row[k]], in order
+    int alpha = Z;
+    for (int k = v_colind[c]; k < v_colind[c + 1]; ++k) alpha = e.op(OP_ADD, alpha, e.op(OP_MUL, v[k], w[v_row[k]]));
+    return alpha;
+  };
+  auto sub_axpy = [&](int alpha, std::vector<int>& w, int c) {
+    for (int k = v_colind[c]; k < v_colind[c + 1]; ++k)
+      w[v_row[k]] = e.op(OP_SUB, w[v_row[k]], e.op(OP_MUL, alpha, v[k]));
+  };
+  int nz_r = 0;
+  for (int c = 0; c < ncol; ++c) {
+    const int pcc = pc[c];
+    for (int k = a_colind[pcc]; k < a_colind[pcc + 1]; ++k) x[prinv[a_row[k]]] = e.input(0, k);
+    for (int k = r_colind[c]; k < r_colind[c + 1] && r_row[k] < c; ++k) {
+      const int rr = r_row[k];
+      int alpha = add_dot(x, rr);
+      alpha = e.op(OP_MUL, alpha, beta[rr]);
+      sub_axpy(alpha, x, rr);
+      r[nz_r++] = x[rr];
+      x[rr] = Z;
+    }
+    for (int k = v_colind[c]; k < v_colind[c + 1]; ++k) {
+      v[k] = x[v_row[k]];
+      x[v_row[k]] = Z;
+    }
+    // casadi_house on v[v_colind[c] ..]
+    const int off = v_colind[c], nv = v_colind[c + 1] - v_colind[c];
+    const int v0 = v[off];
+    int sigma = Z;
+    for (int i = 1; i < nv; ++i) sigma = e.op(OP_ADD, sigma, e.op(OP_MUL, v[off + i], v[off + i]));
+    const int sq = e.un(OP_SQRT, e.op(OP_ADD, e.op(OP_MUL, v0, v0), sigma));
+    const int sz = e.op(OP_EQ, sigma, Z), np = e.op(OP_LE, v0, Z);
+    const int inner = e.select(np, e.op(OP_SUB, v0, sq), e.op(OP_DIV, e.un(OP_NEG, sigma), e.op(OP_ADD, v0, sq)), ONE);
+    const int v0n = e.select(sz, ONE, inner, ONE);
+    v[off] = v0n;
+    beta[c] = e.select(sz, e.op(OP_MUL, TWO, np), e.op(OP_DIV, M1, e.op(OP_MUL, sq, v0n)), ONE);
+    r[nz_r++] = sq;
+  }
+  // casadi_qr_singular: number of !(|R_cc| >= eps)
+  int cnt = Z;
+  for (int c = 0; c < ncol; ++c)
+    cnt = e.op(OP_ADD, cnt, e.un(OP_NOT, e.op(OP_LE, EPS, e.un(OP_FABS, r[r_colind[c + 1] - 1]))));
+  e.output(1, 0, cnt);
+  std::vector<int>& w = x;  // the solve reuses the factor's work vector, as the kernel does
+  for (int q = 0; q < nrhs; ++q) {
+    if (tr) {
+      for (int c = 0; c < ncol; ++c) w[c] = e.input(1, q * ncol + pc[c]);
+      for (int c = 0; c < ncol; ++c)
+        for (int k = r_colind[c]; k < r_colind[c + 1]; ++k) {
+          const int rr = r_row[k];
+          w[c] = rr == c ? e.op(OP_DIV, w[c], r[k]) : e.op(OP_SUB, w[c], e.op(OP_MUL, r[k], w[rr]));
+        }
+      for (int c1 = 0; c1 < ncol; ++c1) {
+        const int c = ncol - 1 - c1;
+        sub_axpy(e.op(OP_MUL, add_dot(w, c), beta[c]), w, c);
+      }
+      for (int c = 0; c < ncol; ++c) e.output(0, q * ncol + c, w[prinv[c]]);
+    } else {
+      for (int c = 0; c < nrow; ++c) w[c] = Z;
+      for (int c = 0; c < ncol; ++c) w[prinv[c]] = e.input(1, q * ncol + c);
+      for (int c = 0; c < ncol; ++c) sub_axpy(e.op(OP_MUL, add_dot(w, c), beta[c]), w, c);
+      for (int c = ncol - 1; c >= 0; --c)
+        for (int k = r_colind[c + 1] - 1; k >= r_colind[c]; --k) {
+          const int rr = r_row[k];
+          w[rr] = rr == c ? e.op(OP_DIV, w[rr], r[k]) : e.op(OP_SUB, w[rr], e.op(OP_MUL, r[k], w[c]));
+        }
+      for (int c = 0; c < ncol; ++c) e.output(0, q * ncol + pc[c], w[c]);
+    }
+  }
+  auto L = std::make_shared<LinsolJit>();
+  L->nrhs = nrhs;
+  L->tape.alg = std::move(e.alg);
+  L->tape.ws = e.ws;
+  L->tape.nnz_in = {nnz_a, static_cast<int64_t>(ncol) * nrhs};
+  L->tape.nnz_out = {static_cast<int64_t>(ncol) * nrhs, 1};
+  L->tape.engine = CXH_ENGINE_JIT;
+  L->jac.tape = &L->tape;
+  L->jac.ndir = 0;
+  L->jac.in_off = {0, nnz_a};
+  L->jac.out_off = {0, ncol * nrhs};
+  L->jac.seg_len = 16384;
+  L->jac.reg_budget = 160;
+  L->jac.prefetch = 64;
+  return L;
+}
 
 // ------------------------------------------------------------ mapaccum ---
 //

@@ -321,7 +321,7 @@ struct cxh_ldl_s {
   DeviceBuffer lt_vals, d_vals, w;
   bool uploaded = false;                               // pattern arrays on the device
   int engine = CXH_ENGINE_JIT;                          // specialised program or the generic kernel
-  std::map<int, std::shared_ptr<sxh::LdlJit>> jit;     // specialised factor + solve per nrhs
+  std::map<int, std::shared_ptr<sxh::LinsolJit>> jit;  // specialised factor + solve per nrhs
 };
 
 struct cxh_qr_s {
@@ -330,7 +330,18 @@ struct cxh_qr_s {
   double eps = 1e-12;
   DeviceBuffer d_a_colind, d_a_row, d_v_colind, d_v_row, d_r_colind, d_r_row, d_prinv, d_pc;
   DeviceBuffer v_vals, r_vals, beta, w;
+  bool uploaded = false;                               // pattern arrays on the device
+  int engine = CXH_ENGINE_JIT;                         // specialised program or the generic kernel
+  std::map<int, std::shared_ptr<sxh::LinsolJit>> jit;  // specialised factor + solve per (nrhs, tr)
 };
+
+static sxh::LinsolJit& qr_program(cxh_qr_s* q, int nrhs, bool tr) {
+  auto& J = q->jit[2 * nrhs + (tr ? 1 : 0)];
+  if (!J)
+    J = sxh::jit_qr_create(q->a.ncol, q->v.nrow, q->a.colind, q->a.row, q->v.colind, q->v.row, q->r.colind,
+                           q->r.row, q->prinv, q->pc, q->eps, nrhs, tr);
+  return *J;
+}
 
 static void upload_ints(DeviceBuffer& b, const std::vector<int>& v) {
   b.upload(v.data(), v.size() * sizeof(int));
@@ -375,7 +386,7 @@ int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, 
     auto& J = l->jit[nrhs];
     if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs);
     const bool soa = layout == CXH_SOA;
-    return jit_ldl_run(*J, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x, soa ? n_inst : 1,
+    return jit_linsol_run(*J, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x, soa ? n_inst : 1,
                        soa ? 1 : static_cast<int64_t>(n) * nrhs, status, static_cast<hipStream_t>(stream));
   }
   if (!l->uploaded) {  // the generic kernel reads the pattern from device memory
@@ -440,7 +451,34 @@ int cxh_ldl_jit_source(cxh_ldl l, int32_t nrhs, char* buf, int64_t cap, int64_t*
   if (nrhs < 0) return cxh_fail("negative nrhs");
   auto& J = l->jit[nrhs];
   if (!J) J = jit_ldl_create(l->lt.ncol, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs);
-  const std::string src = jit_ldl_source(*J);
+  const std::string src = jit_linsol_source(*J, 0);
+  if (len) *len = static_cast<int64_t>(src.size());
+  if (buf && cap > 0) {
+    const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, src.data(), m);
+    buf[m] = '\0';
+  }
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_qr_engine(cxh_qr q, int32_t engine) {
+  CXH_TRY
+  if (!q) return cxh_fail("null qr handle");
+  if (engine != CXH_ENGINE_INTERP && engine != CXH_ENGINE_JIT) return cxh_fail("cxh_qr_engine: unknown engine");
+  q->engine = engine;
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_qr_jit_source(cxh_qr q, int32_t nrhs, int32_t tr, int32_t segment, char* buf, int64_t cap, int64_t* len,
+                      int32_t* n_segments) {
+  CXH_TRY
+  if (!q) return cxh_fail("null qr handle");
+  if (nrhs < 0) return cxh_fail("negative nrhs");
+  sxh::LinsolJit& J = qr_program(q, nrhs, tr != 0);
+  if (n_segments) *n_segments = jit_linsol_segments(J);
+  const std::string src = jit_linsol_source(J, segment);
   if (len) *len = static_cast<int64_t>(src.size());
   if (buf && cap > 0) {
     const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));
@@ -472,14 +510,6 @@ int cxh_qr_create(const int64_t* sp_a, const int64_t* sp_v, const int64_t* sp_r,
   q->prinv = to_perm(prinv, nrow_ext, nrow_ext);
   q->pc = to_perm(pc, ncol, ncol);
   q->eps = eps;
-  upload_ints(q->d_a_colind, q->a.colind);
-  upload_ints(q->d_a_row, q->a.row);
-  upload_ints(q->d_v_colind, q->v.colind);
-  upload_ints(q->d_v_row, q->v.row);
-  upload_ints(q->d_r_colind, q->r.colind);
-  upload_ints(q->d_r_row, q->r.row);
-  upload_ints(q->d_prinv, q->prinv);
-  upload_ints(q->d_pc, q->pc);
   *out = q.release();
   return 0;
   CXH_CATCH
@@ -500,6 +530,23 @@ int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, in
   if (!a || (!x && nrhs > 0)) return cxh_fail("null data pointer");
   const int ncol = q->a.ncol, nrow_ext = q->v.nrow;
   const int64_t nnz_a = q->a.colind.back();
+  if (q->engine == CXH_ENGINE_JIT) {
+    const bool soa = layout == CXH_SOA;
+    return jit_linsol_run(qr_program(q, nrhs, tr != 0), n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x,
+                          soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(ncol) * nrhs, status,
+                          static_cast<hipStream_t>(stream));
+  }
+  if (!q->uploaded) {  // the generic kernel reads the pattern from device memory
+    upload_ints(q->d_a_colind, q->a.colind);
+    upload_ints(q->d_a_row, q->a.row);
+    upload_ints(q->d_v_colind, q->v.colind);
+    upload_ints(q->d_v_row, q->v.row);
+    upload_ints(q->d_r_colind, q->r.colind);
+    upload_ints(q->d_r_row, q->r.row);
+    upload_ints(q->d_prinv, q->prinv);
+    upload_ints(q->d_pc, q->pc);
+    q->uploaded = true;
+  }
   const int bs = 64;
   const int64_t grid = (n_inst + bs - 1) / bs;
   QrArgs A{};

@@ -117,12 +117,18 @@ namespace sxh {
 int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
                 int64_t jac_s_inst, hipStream_t s);
 
-// Batched LDL factor + solve specialised to one sparsity pattern (jit.hip).
-struct LdlJit;
-std::shared_ptr<LdlJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
-                                       const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
-                                       const std::vector<int>& p, int nrhs);
-int jit_ldl_run(LdlJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x, int64_t x_s,
-                int64_t x_t, int32_t* status, hipStream_t s);
-std::string jit_ldl_source(LdlJit& L);
+// Batched LDL / QR factor + solve specialised to one sparsity pattern (jit.hip).
+struct LinsolJit;
+std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
+                                          const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
+                                          const std::vector<int>& p, int nrhs);
+std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<int>& a_colind,
+                                         const std::vector<int>& a_row, const std::vector<int>& v_colind,
+                                         const std::vector<int>& v_row, const std::vector<int>& r_colind,
+                                         const std::vector<int>& r_row, const std::vector<int>& prinv,
+                                         const std::vector<int>& pc, double eps, int nrhs, bool tr);
+int jit_linsol_run(LinsolJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x,
+                   int64_t x_s, int64_t x_t, int32_t* status, hipStream_t s);
+std::string jit_linsol_source(LinsolJit& L, int segment);
+int jit_linsol_segments(LinsolJit& L);
 }

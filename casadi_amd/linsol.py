@@ -55,6 +55,10 @@ class Linsol:
             h = L.P()
             L.check(L.cxh_qr_create(*[a.ctypes.data_as(L.PI64) for a in arrs], self.eps, C.byref(h)),
                     "cxh_qr_create")
+            engine = opts.get("engine", "jit")
+            if engine not in ("jit", "interp"):
+                raise ValueError(f"unknown engine '{engine}' (jit, interp)")
+            L.check(L.cxh_qr_engine(h, 1 if engine == "jit" else 0), "cxh_qr_engine")
         self._h = h
 
     def __del__(self):
@@ -66,13 +70,22 @@ class Linsol:
             fn(h)
         self._h = None
 
-    def jit_source(self, nrhs=1):
-        """Generated source of the specialised LDL factor + solve kernel."""
+    def jit_source(self, nrhs=1, tr=False, segment=0):
+        """Generated source of the specialised factor + solve kernel (LDL: one
+        kernel; QR: segment `segment`, with the number of segments)."""
         n = L.I64()
-        L.check(L.cxh_ldl_jit_source(self._h, int(nrhs), None, 0, C.byref(n)), "cxh_ldl_jit_source")
+        if self.solver == "ldl":
+            L.check(L.cxh_ldl_jit_source(self._h, int(nrhs), None, 0, C.byref(n)), "cxh_ldl_jit_source")
+            buf = C.create_string_buffer(n.value + 1)
+            L.check(L.cxh_ldl_jit_source(self._h, int(nrhs), buf, n.value + 1, C.byref(n)), "cxh_ldl_jit_source")
+            return buf.value.decode()
+        ns = L.I32()
+        L.check(L.cxh_qr_jit_source(self._h, int(nrhs), int(tr), int(segment), None, 0, C.byref(n), C.byref(ns)),
+                "cxh_qr_jit_source")
         buf = C.create_string_buffer(n.value + 1)
-        L.check(L.cxh_ldl_jit_source(self._h, int(nrhs), buf, n.value + 1, C.byref(n)), "cxh_ldl_jit_source")
-        return buf.value.decode()
+        L.check(L.cxh_qr_jit_source(self._h, int(nrhs), int(tr), int(segment), buf, n.value + 1, C.byref(n),
+                                    C.byref(ns)), "cxh_qr_jit_source")
+        return buf.value.decode(), ns.value
 
     def name(self):
         return self._name

@@ -279,6 +279,16 @@ int cxh_qr_create(const int64_t* sp_a, const int64_t* sp_v, const int64_t* sp_r,
 int cxh_qr_destroy(cxh_qr q);
 int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, int32_t nrhs,
                         int32_t tr, int32_t* status, int32_t layout, void* stream);
+/* Engine of cxh_qr_factor_solve, as cxh_ldl_engine: CXH_ENGINE_JIT (default)
+ * compiles the factor + solve of this pattern per (nrhs, tr) into straight-
+ * line kernels (every operation of casadi_qr / casadi_qr_solve in its order:
+ * bit-identical results), CXH_ENGINE_INTERP runs the generic kernel.
+ * Replaces nothing in the reference (a launch choice of the batched plugin). */
+int cxh_qr_engine(cxh_qr q, int32_t engine);
+/* Introspection: generated source of segment `segment` of the specialised
+ * kernels for (nrhs, tr), with the number of segments.                      */
+int cxh_qr_jit_source(cxh_qr q, int32_t nrhs, int32_t tr, int32_t segment, char* buf, int64_t cap,
+                      int64_t* len, int32_t* n_segments);
 
 /* Launch duration in ms of the last kernel issued through a handle-less
  * timing probe: event-timed average of `reps` launches of cxh_map_eval.    */

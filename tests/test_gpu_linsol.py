@@ -55,6 +55,23 @@ def test_ldl_specialised_and_generic_engines_agree(cuda, batch):
     assert np.array_equal(Xj.cpu().numpy(), Xi.cpu().numpy(), equal_nan=True)
 
 
+@pytest.mark.parametrize("tr", [False, True])
+def test_qr_specialised_and_generic_engines_agree(cuda, batch, tr):
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    n_inst = 1031
+    v = vals[:, :n_inst].copy()
+    v[:, 5] = 0.0
+    A = torch.tensor(v, dtype=torch.float64, device="cuda")
+    B = torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda")
+    print(f"[case] qr engines tr={tr} instances={n_inst}", flush=True)
+    Xj, sj = Linsol("j", "qr", sp, {"engine": "jit"}).solve_batch(A, B, tr=tr)
+    Xi, si = Linsol("i", "qr", sp, {"engine": "interp"}).solve_batch(A, B, tr=tr)
+    assert torch.equal(sj, si) and int(sj[5]) == sp.nrow
+    assert np.array_equal(Xj.cpu().numpy(), Xi.cpu().numpy(), equal_nan=True)
+
+
 def test_qr_batch_bit_exact(cuda, batch):
     import torch
     from casadi_amd.linsol import Linsol

@@ -190,9 +190,8 @@ def test_generated_tangent_sweeps_run_on_host(monkeypatch):
         A.res = _table([gv], n)
         A.asens = _table([hnz], n)
         A.spill = sp.ctypes.data
-        for s, (_, fn) in enumerate(libs):
-            A.st_in = st[s % 2].ctypes.data
-            A.st_out = st[(s + 1) % 2].ctypes.data
+        A.st_in = A.st_out = st[0].ctypes.data    # home rows: read and written in place
+        for _, fn in libs:
             fn(C.byref(A), n)
     Hf = ca.Function("hess_lag", [w, lam], [H])
     (href,) = O.map_eval(Hf, n, [wv.T, lv.T])
@@ -288,4 +287,45 @@ def test_specialised_ldl_runs_on_host():
         xr = O.ldl_solve(b[:, k], Lt.compressed(), lt, dd, p)
         assert np.array_equal(x[:, k], xr, equal_nan=True), k
         assert cnt[0, k] == np.sum(~((dd != 0) & np.isfinite(dd)))
+    assert cnt[0, n - 1] == nn
+
+
+@pytest.mark.parametrize("tr", [False, True])
+def test_specialised_qr_runs_on_host(tr):
+    """The straight-line QR factor + solve for a KKT pattern (5 stages, two
+    right-hand sides, in segments whose crossing values keep home rows),
+    executed on the host, against the oracle's casadi_qr / casadi_qr_solve:
+    bit for bit, a zero matrix included (every diagonal counted)."""
+    from casadi_amd import workloads as W
+    from casadi_amd.linsol import Linsol
+    sp, _, _ = W.ocp_kkt_sparsity(N=5)
+    ls = Linsol("q", "qr", sp)
+    src0, ns = ls.jit_source(2, tr, 0)
+    assert ns > 1
+    srcs = [src0] + [ls.jit_source(2, tr, s)[0] for s in range(1, ns)]
+    n, nn = 5, sp.size1()
+    a = np.zeros((sp.nnz(), n))
+    for k in range(n - 1):
+        K = W.ocp_kkt_values(np.random.default_rng(k), N=5)
+        a[:, k] = K[sp.row, sp.get_col()]
+    b = np.random.default_rng(9).standard_normal((2 * nn, n))
+    x = np.zeros((2 * nn, n))
+    cnt = np.zeros((1, n))
+    st, spill, _, lds = _tiled(srcs, n, 1)
+    with tempfile.TemporaryDirectory() as d:
+        libs = _compile(srcs, d, 1)
+        A = CxjArgs()
+        A.n, A.lds = n, lds
+        A.arg = _table([a, b], n)
+        A.res = _table([x, cnt], n)
+        A.spill = spill.ctypes.data
+        A.st_in = A.st_out = st[0].ctypes.data
+        for _, fn in libs:
+            fn(C.byref(A), n)
+    svc, src_ = ls.sp_v.compressed(), ls.sp_r.compressed()
+    for k in range(n):
+        v, r, beta = O.qr(sp.compressed(), a[:, k], svc, src_, ls.prinv, ls.pc)
+        xr = O.qr_solve(b[:, k], int(tr), svc, v, src_, r, beta, ls.prinv, ls.pc)
+        assert np.array_equal(x[:, k], xr, equal_nan=True), k
+        assert cnt[0, k] == O.qr_singular(r, src_, ls.pc, ls.eps)[0]
     assert cnt[0, n - 1] == nn
