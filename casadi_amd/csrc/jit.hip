@@ -927,7 +927,7 @@ static int jac_reg_budget(const cxh_jac_s* j) {
 // a longer lookahead than those that fill the device.
 static int jac_prefetch(const cxh_jac_s* j) {
   if (const char* e = std::getenv("CXH_JIT_PREFETCH")) return std::max(0, std::atoi(e));
-  return j->prefetch >= 0 ? j->prefetch : 8;
+  return j->prefetch >= 0 ? j->prefetch : 16;
 }
 
 // Kernel of direction group g, segment s.  Block 0 of the asens table carries
