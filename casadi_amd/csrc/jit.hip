@@ -1403,7 +1403,7 @@ static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s
   for (int g = 0; g < J.n_group; ++g)
     o += "    case " + std::to_string(g) + ": cxj_body_g" + std::to_string(g) + "(A, k); break;\n";
   o += "  }\n}\n"
-       "extern \"C\" __global__ void __launch_bounds__(256) cxj_tan(const CxjArgs A) {\n"
+       "extern \"C\" __global__ void __launch_bounds__(256) " + std::string(j->kernel) + "(const CxjArgs A) {\n"
        "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
        "  if (k < A.n) cxj_body(A, k, (int)blockIdx.y);\n"
        "}\n";
@@ -1418,9 +1418,9 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
     // the budget counts values, not the compiler's temporaries: where the
     // compiler still needs scratch, regenerate with fewer values resident
     for (int R = jac_reg_budget(j); !f; R -= 24) {
-      if (R < 24) throw std::runtime_error("cxj_tan: no register budget avoids scratch spills");
+      if (R < 24) throw std::runtime_error(std::string(j->kernel) + ": no register budget avoids scratch spills");
       int ns = 0;
-      hipFunction_t g = jit_function(gen_tangent_kernel(j, J, sg, R, &ns), "cxj_tan");
+      hipFunction_t g = jit_function(gen_tangent_kernel(j, J, sg, R, &ns), j->kernel);
       if (scratch_bytes(g) > 0) continue;
       f = g;
       J.spill_rows = std::max<int64_t>(J.spill_rows, ns);
@@ -1626,6 +1626,7 @@ std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colin
   // at the tangent sweeps' 200 / 8 (whose kernel also needed scratch)
   L->jac.reg_budget = 160;
   L->jac.prefetch = 64;
+  L->jac.kernel = "cxj_ldl";
   return L;
 }
 
@@ -1756,6 +1757,7 @@ std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<i
   L->jac.seg_len = 16384;
   L->jac.reg_budget = 160;
   L->jac.prefetch = 64;
+  L->jac.kernel = "cxj_qr";
   return L;
 }
 

@@ -111,6 +111,7 @@ struct cxh_jac_s {
   int64_t seg_len = 0;               // instructions per kernel of the tangent sweep (0: default)
   int reg_budget = 0;                // starting register budget of its kernels (0: default)
   int prefetch = -1;                 // reload lookahead in instructions (-1: default)
+  const char* kernel = "cxj_tan";    // name of its kernels (profiles tell the programs apart)
 };
 
 namespace sxh {
