@@ -114,8 +114,17 @@ struct JitPlan {
   int64_t ck_rows = 0;
   int max_live = 0;
   int64_t snap_rows = 0;  // sub-block boundary states of one reverse kernel (max over segments)
+  int ipt = 1;            // instances per thread of the forward kernels
   std::vector<hipFunction_t> fwd, rev;
 };
+
+// Instances per thread of the forward segment kernels (CXH_JIT_IPT, 1-8):
+// with several, the loads of the later instances are issued while the first
+// one computes.
+static int fwd_ipt() {
+  if (const char* e = std::getenv("CXH_JIT_IPT")) return std::min(8, std::max(1, std::atoi(e)));
+  return 1;
+}
 
 struct JitTape {
   std::map<int64_t, JitPlan> plans;  // keyed by segment length
@@ -220,6 +229,7 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
   }
   P.fwd.assign(P.n_seg, nullptr);
   P.rev.assign(P.n_seg, nullptr);
+  P.ipt = fwd_ipt();
   return t->jit->plans.emplace(every, std::move(P)).first->second;
 }
 
@@ -501,11 +511,22 @@ static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, boo
   std::string o = kSxOpsSource;
   o += kPrologue;
   o += gen_body(t, P, s, rev, "cxj_body");
-  o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") + (rev ? "cxj_rev" : "cxj_fwd") +
-       "(const CxjArgs A) {\n"
-       "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-       "  if (k < A.n) cxj_body(A, k);\n"
-       "}\n";
+  if (rev || P.ipt == 1) {
+    o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") + (rev ? "cxj_rev" : "cxj_fwd") +
+         "(const CxjArgs A) {\n"
+         "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+         "  if (k < A.n) cxj_body(A, k);\n"
+         "}\n";
+  } else {  // P.ipt consecutive blocks' worth of instances per block, coalesced per step
+    o += "extern \"C\" __global__ void __launch_bounds__(256) cxj_fwd(const CxjArgs A) {\n"
+         "  const long long k0 = (long long)blockIdx.x * blockDim.x * " + std::to_string(P.ipt) + " + threadIdx.x;\n"
+         "#pragma unroll\n"
+         "  for (int j = 0; j < " + std::to_string(P.ipt) + "; ++j) {\n"
+         "    const long long k = k0 + (long long)j * blockDim.x;\n"
+         "    if (k < A.n) cxj_body(A, k);\n"
+         "  }\n"
+         "}\n";
+  }
   return o;
 }
 
@@ -660,7 +681,7 @@ int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& re
     A.res = res;
     A.st_in = st ? st + static_cast<int64_t>(g % 2) * P.max_live * lds : nullptr;
     A.st_out = st ? st + static_cast<int64_t>((g + 1) % 2) * P.max_live * lds : nullptr;
-    launch(f, n, t->block, A, s);
+    launch(f, (n + P.ipt - 1) / P.ipt, t->block, A, s);
   }
   return 0;
 }
@@ -689,7 +710,7 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
       A.res = res;
       A.st_in = state(g);
       A.st_out = state(g + 1);
-      launch(f, n, t->block, A, s);
+      launch(f, (n + P.ipt - 1) / P.ipt, t->block, A, s);
     }
   }
   if (mode == 1) return 0;
