@@ -75,3 +75,60 @@ def _make_autograd_function():
             return tuple(res)
 
     return _TapeAdjoint
+
+
+class TorchMapSum:
+    """Differentiable reduced map: inputs flagged in ``reduce_in`` are one block
+    shared by every instance (a vector of its nonzeros), outputs flagged in
+    ``reduce_out`` are summed over the instances (a vector); the other blocks
+    are ``(nnz, n)`` as for TorchFunction.  Forward is cxh_map_eval_sum;
+    backward is the fused forward + adjoint of the reduced map
+    (cxh_map_fwd_adj_sum, the reverse of MapSum, casadi/core/mapsum.cpp:354-415):
+    the cotangent of a summed output seeds every instance, the gradient of a
+    shared input is summed over the instances.  Backward re-runs the forward
+    sweep inside the same kernels instead of keeping checkpoints, so nothing
+    but the inputs is held between forward and backward."""
+
+    def __init__(self, mapsum, checkpoint_every=0):
+        if mapsum.parallelization != "hip":
+            raise TypeError("to_torch needs parallelization 'hip'")
+        self.mapsum = mapsum
+        self.tape = mapsum.tape
+        self.checkpoint_every = int(checkpoint_every)
+        self._fn = _make_mapsum_function()
+
+    def __call__(self, *args):
+        if len(args) != len(self.tape.nnz_in):
+            raise ValueError(f"{self.mapsum.name()} takes {len(self.tape.nnz_in)} inputs, got {len(args)}")
+        outs = self._fn.apply(self, *args)
+        return outs[0] if len(outs) == 1 else outs
+
+
+def _make_mapsum_function():
+    torch = _torch()
+
+    class _MapSumAdjoint(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, owner, *args):
+            ms = owner.mapsum
+            args = [a.contiguous() if a is not None else None for a in args]
+            outs = owner.tape.eval_sum(args, ms.n, ms.reduce_in, ms.reduce_out)
+            ctx.owner = owner
+            ctx.save_for_backward(*[a if a is not None else torch.empty(0) for a in args])
+            ctx.present = [a is not None for a in args]
+            like = next(a for a in args if a is not None)
+            return tuple(o if o is not None else
+                         like.new_zeros((0,) if ms.reduce_out[j] else (0, ms.n)) for j, o in enumerate(outs))
+
+        @staticmethod
+        def backward(ctx, *grads):
+            owner = ctx.owner
+            ms = owner.mapsum
+            args = [a if p else None for a, p in zip(ctx.saved_tensors, ctx.present)]
+            seeds = [None if g is None or k == 0 else g.to(torch.float64).contiguous()
+                     for g, k in zip(grads, owner.tape.nnz_out)]
+            _, adj = owner.tape.fwd_adj_sum(args, seeds, ms.n, ms.reduce_in, ms.reduce_out,
+                                            checkpoint_every=owner.checkpoint_every, res=False)
+            return (None,) + tuple(a if need else None for a, need in zip(adj, ctx.needs_input_grad[1:]))
+
+    return _MapSumAdjoint

@@ -211,6 +211,13 @@ class MapSumFunction:
             out.append(acc)
         return out
 
+    def to_torch(self, checkpoint_every=0):
+        """The reduced map as a differentiable torch op on device tensors
+        (casadi_amd.autograd.TorchMapSum): gradients of shared inputs come
+        back summed over the instances."""
+        from .autograd import TorchMapSum
+        return TorchMapSum(self, checkpoint_every)
+
     def fwd_adj(self, args, seeds, checkpoint_every=0, res=True):
         """Device tensors: outputs and the reverse sweep of the reduced map in
         one pass (the MapSum's reverse, casadi/core/mapsum.cpp:354-415): the

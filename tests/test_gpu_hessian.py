@@ -62,16 +62,17 @@ def test_quadrotor_lagrangian_hessian(cuda):
     assert np.array_equal(hnz[:, :64], hnz[mirror, :64])
 
 
-def test_quadrotor_hessian_specialised_kernels(cuda):
-    """The specialised tangent kernels (direction groups, segments) give the
-    interpreter's Hessian bit for bit at full size."""
+@pytest.mark.parametrize("n", [4096, 1001])
+def test_quadrotor_hessian_specialised_kernels(cuda, n):
+    """The specialised tangent kernels (direction groups, segments, home and
+    spill rows tiled by 64 instances) give the interpreter's Hessian bit for
+    bit, on whole tiles and on a ragged last tile."""
     import torch
     import casadi_amd as ca
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipJacobian
     w, lam, Lag = W.quadrotor_ocp_lagrangian()
     G = ca.Function("grad_lag", [w, lam], [ca.gradient(Lag, w)])
-    n = 4096
     wv, lv = _inputs(n, seed_of(1) + 1)
     args = [torch.tensor(a, dtype=torch.float64, device="cuda") for a in (wv, lv)]
     print(f"[case] quadrotor hessian specialised n={n}", flush=True)

@@ -99,3 +99,5 @@ def test_reduced_map_arguments(ca):
         f.map("F", "gpu", 10, [], [])                     # unknown parallelization
     F = f.map("F", "hip", 10, [1], [0])
     assert F.reduce_in == [False, True] and F.reduce_out == [True]
+    with pytest.raises(TypeError):
+        f.map("H", "serial", 10, [1], [0]).to_torch()     # the differentiable op runs on the GPU
