@@ -100,7 +100,29 @@ struct CxjArgs {
 #define CXJ_OPAQUE(p) asm volatile("" : "+v"(p))
 #define CXJ_OPAQUE_AFTER(p, x) asm volatile("" : "+v"(p) : "v"(x))
 #endif
+#ifdef CXJ_TRIG_CALLS
+#define CXJ_SINCOS(x_, s_, c_) do { const sxh::SinCos r_ = sxh::sxh_sincos_call(x_); s_ = r_.s; c_ = r_.c; } while (0)
+#else
+#define CXJ_SINCOS(x_, s_, c_) sxh::sxh_sincos(x_, &s_, &c_)
+#endif
 )CXJ";
+
+// Sites of sin/cos pairs above which a program calls one out-of-line copy
+// (CXH_JIT_TRIG_CALLS overrides; 0 always calls, a negative value never).
+static int trig_call_sites() {
+  if (const char* e = std::getenv("CXH_JIT_TRIG_CALLS")) return std::atoi(e);
+  return 32;
+}
+
+// The finished program: switches its sin/cos pairs to calls when it has more
+// than trig_call_sites() of them.
+static std::string finish_source(std::string o) {
+  const int lim = trig_call_sites();
+  if (lim < 0) return o;
+  int64_t sites = 0;  // "  CXJ_SINCOS(" starts an emitted line; the definitions have one space
+  for (size_t p = o.find("  CXJ_SINCOS("); p != std::string::npos; p = o.find("  CXJ_SINCOS(", p + 1)) ++sites;
+  return sites > lim ? "#define CXJ_TRIG_CALLS\n" + o : o;
+}
 
 // ---------------------------------------------------------------- plans ---
 
@@ -333,8 +355,7 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
         if (it == trig.end()) {
           const std::string id = std::to_string(code);
           it = trig.emplace(x, std::make_pair("ts" + id, "tc" + id)).first;
-          o += "  double ts" + id + ", tc" + id + ";\n  sxh::sxh_sincos(" + V(x) + ", &ts" + id + ", &tc" + id +
-               ");\n";
+          o += "  double ts" + id + ", tc" + id + ";\n  CXJ_SINCOS(" + V(x) + ", ts" + id + ", tc" + id + ");\n";
         }
         o += "  const double " + nm + " = " + (e.op == OP_SIN ? it->second.first : it->second.second) + ";\n";
         ox[k - t0] = oy[k - t0] = x;
@@ -527,7 +548,7 @@ static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, boo
          "  }\n"
          "}\n";
   }
-  return o;
+  return finish_source(std::move(o));
 }
 
 // ------------------------------------------------------------ compile ---
@@ -767,7 +788,7 @@ static bool emit_trig(std::string& o, const Instr& e, int64_t k, int64_t x, cons
   if (it == trig.end()) {
     const std::string id = std::to_string(k);
     it = trig.emplace(x, std::make_pair("ts" + id, "tc" + id)).first;
-    o += "  double ts" + id + ", tc" + id + ";\n  sxh::sxh_sincos(" + xname + ", &ts" + id + ", &tc" + id + ");\n";
+    o += "  double ts" + id + ", tc" + id + ";\n  CXJ_SINCOS(" + xname + ", ts" + id + ", tc" + id + ");\n";
   }
   o += "  const double v" + std::to_string(k) + " = " + (e.op == OP_SIN ? it->second.first : it->second.second) +
        ";\n";
@@ -1321,7 +1342,7 @@ static std::string gen_tangent(const cxh_jac_s* j, const JitJac& J, int g, int s
         const std::string& X = val[op.x].name;
         const std::string& Y = val[op.y].name;
         if (op.trig == 1) {
-          b += "  double ts" + ks + ", tc" + ks + ";\n  sxh::sxh_sincos(" + X + ", &ts" + ks + ", &tc" + ks + ");\n";
+          b += "  double ts" + ks + ", tc" + ks + ";\n  CXJ_SINCOS(" + X + ", ts" + ks + ", tc" + ks + ");\n";
         } else if (op.trig == 0) {
           b += "  const double v" + ks + " = sxh::op_fun_full(" + std::to_string(e.op) + ", " + X + ", " + Y + ");\n";
         }
@@ -1428,7 +1449,7 @@ static std::string gen_tangent_kernel(const cxh_jac_s* j, const JitJac& J, int s
        "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
        "  if (k < A.n) cxj_body(A, k, (int)blockIdx.y);\n"
        "}\n";
-  return o;
+  return finish_source(std::move(o));
 }
 
 int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res, double* jac, int64_t jac_s_nz,
@@ -2053,7 +2074,7 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
        "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
        "  if (k < A.n) cxj_body(A, k);\n"
        "}\n";
-  return o;
+  return finish_source(std::move(o));
 }
 
 // ----------------------------------------------------- heterogeneous batch ---
@@ -2092,7 +2113,7 @@ static std::string gen_batch(const std::vector<cxh_tape_s*>& tapes) {
     o += "      case " + fs + ": { const CxjArgs& A = args[" + fs + "]; if (k < A.n) cxj_f" + fs + "(A, k); } break;\n";
   }
   o += "    }\n  }\n}\n";
-  return o;
+  return finish_source(std::move(o));
 }
 
 }  // namespace sxh

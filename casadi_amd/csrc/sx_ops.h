@@ -150,6 +150,19 @@ SXH_HD void sxh_sincos(double x, double* s, double* c) {
 #endif
 }
 
+// The same pair returned by value from one out-of-line copy.  A long
+// straight-line kernel calls it instead of inlining the reduction and
+// polynomials at every site (about 1.3 KB of gfx950 code each), which keeps
+// the kernel's code small enough to stay in the instruction cache.
+struct SinCos {
+  double s, c;
+};
+SXH_HD_NOINLINE SinCos sxh_sincos_call(double x) {
+  SinCos r;
+  sxh_sincos(x, &r.s, &r.c);
+  return r;
+}
+
 SXH_HD double op_fun_full(int op, double x, double y);
 SXH_HD void op_der_full(int op, double x, double y, double f, double* d);
 

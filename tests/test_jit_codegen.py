@@ -114,3 +114,22 @@ def test_batch_kernel_source_compiles(ca):
     for i in range(len(fs)):
         assert f"cxj_f{i}(A, k)" in src and f"void cxj_f{i}(const CxjArgs& A" in src
     jit_compile_check(src)
+
+
+def test_long_programs_call_one_sincos(ca, monkeypatch):
+    """Programs with more than 32 sin/cos sites call one out-of-line copy
+    (CXH_JIT_TRIG_CALLS sets the limit; 0 always calls, negative never);
+    shorter ones inline it.  Both forms compile."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape, jit_compile_check
+    step = HipTape(W.cartpole_step(), engine="jit")
+    src, _ = step.jit_source(0, 0, True)
+    assert src.count("  CXJ_SINCOS(") == 4 and not src.startswith("#define CXJ_TRIG_CALLS")
+    roll = W.cartpole_rollout(10)
+    long_src, _ = HipTape(roll, engine="jit").jit_source(0, 0, True)
+    assert long_src.count("  CXJ_SINCOS(") > 32 and long_src.startswith("#define CXJ_TRIG_CALLS\n")
+    jit_compile_check(long_src)
+    monkeypatch.setenv("CXH_JIT_TRIG_CALLS", "0")
+    assert HipTape(W.cartpole_step(), engine="jit").jit_source(0, 0, True)[0].startswith("#define CXJ_TRIG_CALLS\n")
+    monkeypatch.setenv("CXH_JIT_TRIG_CALLS", "-1")
+    assert not HipTape(roll, engine="jit").jit_source(0, 0, True)[0].startswith("#define CXJ_TRIG_CALLS")
