@@ -65,6 +65,8 @@ cxh_map_eval = _fn("cxh_map_eval", I32, P, I64, PP, PP, I32, I64, P)
 cxh_map_eval_host = _fn("cxh_map_eval_host", I32, P, I64, PP, PP)
 cxh_map_fwd_adj = _fn("cxh_map_fwd_adj", I32, P, I64, PP, PP, PP, PP, I32, I64, I64, P)
 cxh_map_eval_sum = _fn("cxh_map_eval_sum", I32, P, I64, PP, PP, C.POINTER(I32), C.POINTER(I32), I64, P)
+cxh_sp_forward = _fn("cxh_sp_forward", I32, P, I64, PP, PP, I64, P)
+cxh_sp_reverse = _fn("cxh_sp_reverse", I32, P, I64, PP, PP, I64, P)
 cxh_map_fwd_adj_sum = _fn("cxh_map_fwd_adj_sum", I32, P, I64, PP, PP, PP, PP, C.POINTER(I32), C.POINTER(I32),
                           I64, I64, P)
 cxh_ckpt_size = _fn("cxh_ckpt_size", I32, P, I64, I64, I32, PI64)

@@ -10,6 +10,8 @@ c4     heterogeneous batch: 40000 cart-pole + 12000 quadrotor + 25000 arm,
        forward + adjoint, one hipGraph launch (autotuned) vs plain launches
 c5     batched KKT solve, n = 274, 8192 instances, ldl and qr
 c3macc the 100-step rollout cost + gradient as a mapaccum of one step, 32768
+mapsum reduced map: cart-pole step over 1M states, shared control, summed
+sparsity Jacobian patterns from dependency sweeps, host against GPU
 
 Inputs are synthetic, from numpy.random.default_rng(20261202 + workload
 index).  Times are HIP-event times on torch's current stream, device-resident
@@ -200,6 +202,40 @@ def mapsum(n=1 << 20):
     _out(k="mapsum_fwd_adj_unfused", n=n, ms=ms)
 
 
+def sparsity():
+    """Jacobian sparsity from dependency sweeps: the host's 64-column sweeps
+    (SXFunction::get_jac_sparsity) against the GPU's, all words side by side
+    (HipTape.jac_sparsity: seeds up, cxh_sp_forward / cxh_sp_reverse, words
+    down, pattern assembled).  Patterns compared for equality."""
+    import torch
+    from . import SX, Function, sin
+    from . import workloads as W
+    from .hip import HipTape
+    from .sx import gradient
+    w, lam, Lag = W.quadrotor_ocp_lagrangian()
+    x = SX.sym("x", 20000)
+    cases = [("hessian_pattern_quadrotor_ocp", lambda: Function("grad_lag", [w, lam], [gradient(Lag, w)])),
+             ("banded_20000", lambda: Function("band", [x], [sin(x[:-1]) * x[1:]]))]
+    for name, make in cases:
+        f = make()
+        t0 = time.perf_counter()
+        hs = f.jac_sparsity(0, 0)
+        host_ms = (time.perf_counter() - t0) * 1e3
+        t = HipTape(f)
+        t.jac_sparsity(0, 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gs = t.jac_sparsity(0, 0)
+        torch.cuda.synchronize()
+        gpu_ms = (time.perf_counter() - t0) * 1e3
+        nw = -(-f.nnz_in(0) // 64)
+        seeds = [torch.zeros((f.nnz_in(i), nw), dtype=torch.int64, device="cuda") for i in range(f.n_in())]
+        kern_ms = _timed(lambda: t.sp_forward(seeds, nw), 5)
+        _out(k="jac_sparsity", case=name, instr=f.n_instructions(), worksize=f.sz_w(), nnz_in=f.nnz_in(0),
+             nnz_out=f.nnz_out(0), words=nw, nnz=hs.nnz(), equal=bool(gs == hs), host_ms=host_ms,
+             gpu_ms=gpu_ms, sp_forward_ms=kern_ms)
+
+
 def main(argv):
     which = argv or ["c1", "c2", "c4", "c5"]
     if "c1" in which:
@@ -216,6 +252,8 @@ def main(argv):
         rollout_mapaccum()
     if "mapsum" in which:
         mapsum()
+    if "sparsity" in which:
+        sparsity()
 
 
 if __name__ == "__main__":

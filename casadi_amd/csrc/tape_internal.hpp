@@ -67,6 +67,7 @@ struct cxh_tape_s {
   bool uploaded = false;
   sxh::DeviceBuffer scratch, trail, ckpt;
   sxh::DeviceBuffer red, partial;  // summed blocks of a reduced map (mapsum.hip)
+  sxh::DeviceBuffer spw;           // work words of a sparsity sweep too large for LDS (sparsity_kernels.hip)
   int block = 64;
   int w_lds = 1;
   int ipw = 64;  // instances per wavefront (8, 16, 32, 64)

@@ -225,6 +225,96 @@ class HipTape:
                                       _stream_ptr(stream)), "cxh_map_fwd_adj_sum")
         return outs, adj
 
+    # -- dependency (sparsity) propagation -------------------------------------
+    @staticmethod
+    def _check_words(tensors, nnz, n_words, what):
+        """int64 CUDA tensors (nnz, >= n_words) holding 64-bit words; returns ld."""
+        torch = _torch()
+        ld = None
+        for i, t in enumerate(tensors):
+            if t is None:
+                continue
+            if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.int64:
+                raise TypeError(f"{what}[{i}] must be an int64 CUDA tensor of 64-bit words")
+            if t.dim() != 2 or t.shape[0] != nnz[i] or t.shape[1] < n_words:
+                raise ValueError(f"{what}[{i}] has shape {tuple(t.shape)}, expected ({nnz[i]}, {n_words})")
+            if nnz[i] > 1 and t.shape[1] > 0 and t.stride(1) != 1:
+                raise ValueError(f"{what}[{i}] must have unit stride along words")
+            s = t.stride(0) if nnz[i] > 1 else n_words
+            if ld is None:
+                ld = s
+            elif nnz[i] > 1 and s != ld:
+                raise ValueError(f"{what}[{i}]: every block must share the leading dimension")
+        return ld if ld is not None else n_words
+
+    def sp_forward(self, args, n_words, stream=None):
+        """SXFunction::sp_forward over n_words 64-bit seed words per input
+        nonzero (int64 tensors (nnz_in(i), n_words), None = no dependency):
+        the dependency words of every output nonzero, (nnz_out(j), n_words)."""
+        torch = _torch()
+        ld = self._check_words(args, self.nnz_in, n_words, "arg")
+        res = [torch.zeros((k, ld), dtype=torch.int64, device="cuda")[:, :n_words] if k else None
+               for k in self.nnz_out]
+        L.check(L.cxh_sp_forward(self._h, n_words, _ptrs(args), _ptrs(res), ld, _stream_ptr(stream)),
+                "cxh_sp_forward")
+        return res
+
+    def sp_reverse(self, seeds, n_words, stream=None):
+        """SXFunction::sp_reverse: seed words per output nonzero (left as they
+        are; the sweep runs on a copy) to the dependency words of every input
+        nonzero, (nnz_in(i), n_words)."""
+        torch = _torch()
+        self._check_words(seeds, self.nnz_out, n_words, "seed")
+        work = [None if t is None else
+                torch.empty((t.shape[0], n_words), dtype=torch.int64, device=t.device).copy_(t[:, :n_words])
+                for t in seeds]
+        ld = self._check_words(work, self.nnz_out, n_words, "seed")
+        arg = [torch.zeros((k, ld), dtype=torch.int64, device="cuda")[:, :n_words] if k else None
+               for k in self.nnz_in]
+        L.check(L.cxh_sp_reverse(self._h, n_words, _ptrs(arg), _ptrs(work), ld, _stream_ptr(stream)),
+                "cxh_sp_reverse")
+        return arg
+
+    def jac_sparsity(self, oind=0, iind=0):
+        """Compact Jacobian pattern of output oind w.r.t. input iind
+        (nnz_out(oind) x nnz_in(iind)) from GPU sweeps, one word per 64
+        columns (forward) or rows (reverse), whichever needs fewer words --
+        FunctionInternal::get_jac_sparsity's choice
+        (casadi/core/function_internal.cpp:1928-1976)."""
+        import numpy as np
+        from .sx import Sparsity
+        torch = _torch()
+        n_in, n_out = self.nnz_in[iind], self.nnz_out[oind]
+        w_fwd, w_adj = -(-n_in // 64), -(-n_out // 64)
+        if n_in == 0 or n_out == 0:
+            return Sparsity(n_out, n_in)
+
+        def unit_words(n, nw):
+            u = np.zeros((n, nw), np.uint64)
+            r = np.arange(n)
+            u[r, r // 64] = np.left_shift(np.uint64(1), (r % 64).astype(np.uint64))
+            return torch.from_numpy(u.view(np.int64)).to("cuda")
+
+        def set_bits(words):
+            """(rows, nw) words -> (row, 64 s + b) of every set bit b of word s;
+            only the nonzero words are expanded."""
+            rr, ss = np.nonzero(words)
+            b = np.unpackbits(np.ascontiguousarray(words[rr, ss]).view(np.uint8).reshape(-1, 8), axis=1,
+                              bitorder="little")
+            i, bit = np.nonzero(b)
+            return rr[i], ss[i] * 64 + bit
+        if w_fwd <= w_adj:
+            seeds = [None] * len(self.nnz_in)
+            seeds[iind] = unit_words(n_in, w_fwd)
+            out = self.sp_forward(seeds, w_fwd)[oind]
+            r, c = set_bits(out.cpu().numpy())      # output nonzero, input nonzero
+        else:
+            seeds = [None] * len(self.nnz_out)
+            seeds[oind] = unit_words(n_out, w_adj)
+            sens = self.sp_reverse(seeds, w_adj)[iind]
+            c, r = set_bits(sens.cpu().numpy())     # input nonzero, output nonzero
+        return Sparsity.triplet(n_out, n_in, r, c)
+
     def ckpt_size(self, n, checkpoint_every):
         v = L.I64()
         L.check(L.cxh_ckpt_size(self._h, n, int(checkpoint_every), -1, C.byref(v)))

@@ -148,6 +148,23 @@ int cxh_map_fwd_adj_sum(cxh_tape t, int64_t n, const double* const* arg,
                         const int32_t* reduce_in, const int32_t* reduce_out, int64_t ld,
                         int64_t checkpoint_every, void* stream);
 
+/* Dependency propagation, forward: n_words independent 64-bit seed words
+ * (64 directions each) through the tape.  arg[i] holds nnz_in(i) rows of
+ * words, res[j] receives nnz_out(j) rows; SoA, word k of nonzero r at
+ * [r * ld + k].  A null arg[i] reads as zero, a null res[j] is skipped.
+ * Replaces: SXFunction::sp_forward (casadi/core/sx_function.cpp:1657-1681),
+ *   called once per 64 columns by FunctionInternal::get_jac_sparsity
+ *   (casadi/core/function_internal.cpp:1928-2066).                          */
+int cxh_sp_forward(cxh_tape t, int64_t n_words, const uint64_t* const* arg, uint64_t* const* res,
+                   int64_t ld, void* stream);
+
+/* Dependency propagation, reverse: the words of res[j] are seeds (cleared on
+ * return, as the reference does), and each arg[i] word is ORed with the
+ * dependencies found (the caller zeroes it first for a fresh sweep).
+ * Replaces: SXFunction::sp_reverse (casadi/core/sx_function.cpp:1733-1782).  */
+int cxh_sp_reverse(cxh_tape t, int64_t n_words, uint64_t* const* arg, uint64_t* const* res, int64_t ld,
+                   void* stream);
+
 /* Split adjoint for autograd: a forward pass that stores the work vector at
  * the start of every segment into a caller-owned device buffer `ckpt`
  * (cxh_ckpt_size doubles), then the reverse sweep restarting from those

@@ -10,6 +10,10 @@
  *   or_eval             SXFunction::eval (casadi/core/sx_function.cpp:72-127)
  *   or_map_eval         Map::eval_gen serial loop (casadi/core/map.cpp:141-157)
  *                       and OmpMap::eval (casadi/core/map.cpp:340-386)
+ *   or_sp_forward / or_sp_reverse
+ *                       SXFunction::sp_forward / sp_reverse, one 64-bit word
+ *                       per nonzero (casadi/core/sx_function.cpp:1657-1681,
+ *                       1733-1782)
  *   or_ldl / or_ldl_solve  casadi_ldl / casadi_ldl_trs / casadi_ldl_solve
  *                       (casadi/core/runtime/casadi_ldl.hpp:24-109)
  *   or_qr / or_qr_solve / or_qr_singular
@@ -230,6 +234,57 @@ int or_eval(const void* instr, int64_t n_instr, const double** arg, double** res
         break;
       case O_PARAMETER: case O_CALL: return 1;
       default: w[e->i0] = or_fun(e->op, w[e->u.s.i1], w[e->u.s.i2]);
+    }
+  }
+  return 0;
+}
+
+/* SXFunction::sp_forward (casadi/core/sx_function.cpp:1657-1681): the
+ * dependency word of every output nonzero from the seed words of the input
+ * nonzeros; every input and output differentiable. */
+int or_sp_forward(const void* instr, int64_t n_instr, const uint64_t** arg, uint64_t** res, uint64_t* w) {
+  const or_instr* a = (const or_instr*)instr;
+  for (int64_t k = 0; k < n_instr; ++k) {
+    const or_instr* e = a + k;
+    switch (e->op) {
+      case O_CONST: case O_PARAMETER: w[e->i0] = 0; break;
+      case O_INPUT: w[e->i0] = arg[e->u.s.i1] ? arg[e->u.s.i1][e->u.s.i2] : 0; break;
+      case O_OUTPUT:
+        if (res[e->i0]) res[e->i0][e->u.s.i2] = w[e->u.s.i1];
+        break;
+      case O_CALL: return 1;
+      default: w[e->i0] = w[e->u.s.i1] | w[e->u.s.i2];
+    }
+  }
+  return 0;
+}
+
+/* SXFunction::sp_reverse (casadi/core/sx_function.cpp:1733-1782): seeds in
+ * res (cleared), dependencies ORed into arg. */
+int or_sp_reverse(const void* instr, int64_t n_instr, int64_t ws, uint64_t** arg, uint64_t** res, uint64_t* w) {
+  const or_instr* a = (const or_instr*)instr;
+  memset(w, 0, (size_t)ws * sizeof(uint64_t));
+  for (int64_t k = n_instr - 1; k >= 0; --k) {
+    const or_instr* e = a + k;
+    uint64_t seed;
+    switch (e->op) {
+      case O_CONST: case O_PARAMETER: w[e->i0] = 0; break;
+      case O_INPUT:
+        if (arg[e->u.s.i1]) arg[e->u.s.i1][e->u.s.i2] |= w[e->i0];
+        w[e->i0] = 0;
+        break;
+      case O_OUTPUT:
+        if (res[e->i0]) {
+          w[e->u.s.i1] |= res[e->i0][e->u.s.i2];
+          res[e->i0][e->u.s.i2] = 0;
+        }
+        break;
+      case O_CALL: return 1;
+      default:
+        seed = w[e->i0];
+        w[e->i0] = 0;
+        w[e->u.s.i1] |= seed;
+        w[e->u.s.i2] |= seed;
     }
   }
   return 0;

@@ -46,6 +46,10 @@ def _load():
     lib.or_set_perturb.argtypes = [C.c_int]
     lib.or_adj_map.restype = C.c_int
     lib.or_adj_map.argtypes = [P, I64, I64, C.c_int, PI64, C.c_int, PI64, I64, P, P, P, P, C.c_int]
+    lib.or_sp_forward.restype = C.c_int
+    lib.or_sp_forward.argtypes = [P, I64, P, P, P]
+    lib.or_sp_reverse.restype = C.c_int
+    lib.or_sp_reverse.argtypes = [P, I64, I64, P, P, P]
     lib.or_ldl_batch.restype = I64
     lib.or_ldl_batch.argtypes = [PI64, PI64, PI64, I64, PD, PD, PD, C.c_int]
     return lib
@@ -91,6 +95,39 @@ def map_eval(f, n, args_aos, n_threads=1):
                            len(nnz_out), _pi(nnz_out), n, _ptrs(arg), _ptrs(res), n_threads)
     assert flag == 0, "oracle VM rejected the tape"
     return res
+
+
+def sp_forward(f, seeds):
+    """SXFunction::sp_forward on the oracle, word by word.  seeds[i]:
+    (nnz_in(i), n_words) uint64 or None; returns (nnz_out(j), n_words) uint64."""
+    instr = f.instructions()
+    nw = next(s.shape[1] for s in seeds if s is not None)
+    out = [np.zeros((f.nnz_out(j), nw), np.uint64) for j in range(f.n_out())]
+    w = np.zeros(max(f.sz_w(), 1), np.uint64)
+    for k in range(nw):
+        a = [None if s is None else np.ascontiguousarray(s[:, k]) for s in seeds]
+        r = [np.zeros(f.nnz_out(j), np.uint64) for j in range(f.n_out())]
+        assert lib.or_sp_forward(instr.ctypes.data, instr.size, _ptrs(a), _ptrs(r), w.ctypes.data) == 0
+        for j in range(f.n_out()):
+            out[j][:, k] = r[j]
+    return out
+
+
+def sp_reverse(f, seeds):
+    """SXFunction::sp_reverse on the oracle, word by word.  seeds[j]:
+    (nnz_out(j), n_words) uint64 or None; returns (nnz_in(i), n_words) uint64."""
+    instr = f.instructions()
+    nw = next(s.shape[1] for s in seeds if s is not None)
+    out = [np.zeros((f.nnz_in(i), nw), np.uint64) for i in range(f.n_in())]
+    w = np.zeros(max(f.sz_w(), 1), np.uint64)
+    for k in range(nw):
+        r = [None if s is None else np.ascontiguousarray(s[:, k]) for s in seeds]
+        a = [np.zeros(f.nnz_in(i), np.uint64) for i in range(f.n_in())]
+        assert lib.or_sp_reverse(instr.ctypes.data, instr.size, f.sz_w(), _ptrs(a), _ptrs(r), w.ctypes.data) == 0
+        assert all(x is None or not x.any() for x in r)   # seeds are consumed
+        for i in range(f.n_in()):
+            out[i][:, k] = a[i]
+    return out
 
 
 def adj_map(f, n, args_aos, seeds_aos, n_threads=1):
