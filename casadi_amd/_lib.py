@@ -66,6 +66,9 @@ cxh_map_eval_host = _fn("cxh_map_eval_host", I32, P, I64, PP, PP)
 cxh_map_fwd_adj = _fn("cxh_map_fwd_adj", I32, P, I64, PP, PP, PP, PP, I32, I64, I64, P)
 cxh_map_eval_sum = _fn("cxh_map_eval_sum", I32, P, I64, PP, PP, C.POINTER(I32), C.POINTER(I32), I64, P)
 cxh_sp_forward = _fn("cxh_sp_forward", I32, P, I64, PP, PP, I64, P)
+cxh_tape_create_nested = _fn("cxh_tape_create_nested", I32, P, I64, I64, I32, PI64, I32, PI64, I32, PP, PP, PP,
+                             PP)
+cxh_tape_instructions = _fn("cxh_tape_instructions", I32, P, P, I64, PI64, PI64)
 cxh_sp_reverse = _fn("cxh_sp_reverse", I32, P, I64, PP, PP, I64, P)
 cxh_map_fwd_adj_sum = _fn("cxh_map_fwd_adj_sum", I32, P, I64, PP, PP, PP, PP, C.POINTER(I32), C.POINTER(I32),
                           I64, I64, P)

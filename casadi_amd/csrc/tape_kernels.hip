@@ -499,6 +499,29 @@ void validate_tape(const std::vector<Instr>& a, int64_t ws, const std::vector<in
   }
 }
 
+// A validated tape: its input nonzeros read by no INPUT instruction are noted
+// (their adjoint is zero).
+std::unique_ptr<cxh_tape_s> make_tape(std::vector<Instr> alg, int64_t worksize, int32_t n_in, const int64_t* nnz_in,
+                                      int32_t n_out, const int64_t* nnz_out) {
+  if (n_in > CXH_MAX_IO || n_out > CXH_MAX_IO)
+    throw std::runtime_error("cxh_tape_create: at most " + std::to_string(CXH_MAX_IO) +
+                             " input and output blocks");
+  auto t = std::make_unique<cxh_tape_s>();
+  t->alg = std::move(alg);
+  t->ws = std::max<int64_t>(worksize, 1);
+  t->nnz_in.assign(nnz_in, nnz_in + n_in);
+  t->nnz_out.assign(nnz_out, nnz_out + n_out);
+  validate_tape(t->alg, t->ws, t->nnz_in, t->nnz_out);
+  std::vector<std::vector<char>> used(n_in);
+  for (int i = 0; i < n_in; ++i) used[i].assign(t->nnz_in[i], 0);
+  for (const Instr& e : t->alg)
+    if (e.op == OP_INPUT) used[e.i1][e.i2] = 1;
+  for (int i = 0; i < n_in; ++i)
+    for (int64_t j = 0; j < t->nnz_in[i]; ++j)
+      if (!used[i][j]) t->unused_in.push_back({i, static_cast<int>(j)});
+  return t;
+}
+
 // Kernel-argument table of the I/O blocks (passed by value, no staging copy).
 IOTable make_io(const double* const* p, const std::vector<int64_t>& nnz, int32_t layout, int64_t ld) {
   IOTable v{};
@@ -552,24 +575,116 @@ int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize, int32_
   if (!out) return cxh_fail("cxh_tape_create: null output handle");
   if (n_instr < 0 || worksize < 0 || n_in < 0 || n_out < 0) return cxh_fail("cxh_tape_create: negative size");
   if (n_instr > (1 << 30) || worksize > (1 << 24)) return cxh_fail("cxh_tape_create: tape too large");
-  if (n_in > CXH_MAX_IO || n_out > CXH_MAX_IO)
-    return cxh_fail("cxh_tape_create: at most " + std::to_string(CXH_MAX_IO) + " input and output blocks");
-  auto t = std::make_unique<cxh_tape_s>();
-  t->alg.resize(n_instr);
-  if (n_instr) std::memcpy(t->alg.data(), instr, n_instr * sizeof(Instr));
-  t->ws = std::max<int64_t>(worksize, 1);
-  t->nnz_in.assign(nnz_in, nnz_in + n_in);
-  t->nnz_out.assign(nnz_out, nnz_out + n_out);
-  validate_tape(t->alg, t->ws, t->nnz_in, t->nnz_out);
-  // input nonzeros read by no INPUT instruction (their adjoint is zero)
-  std::vector<std::vector<char>> used(n_in);
-  for (int i = 0; i < n_in; ++i) used[i].assign(t->nnz_in[i], 0);
-  for (const Instr& e : t->alg)
-    if (e.op == OP_INPUT) used[e.i1][e.i2] = 1;
-  for (int i = 0; i < n_in; ++i)
-    for (int64_t j = 0; j < t->nnz_in[i]; ++j)
-      if (!used[i][j]) t->unused_in.push_back({i, static_cast<int>(j)});
-  *out = t.release();
+  std::vector<Instr> alg(n_instr);
+  if (n_instr) std::memcpy(alg.data(), instr, n_instr * sizeof(Instr));
+  *out = make_tape(std::move(alg), worksize, n_in, nnz_in, n_out, nnz_out).release();
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_tape_create_nested(const void* instr, int64_t n_instr, int64_t worksize, int32_t n_in,
+                           const int64_t* nnz_in, int32_t n_out, const int64_t* nnz_out, int32_t n_calls,
+                           const cxh_tape* callees, const int32_t* const* call_dep,
+                           const int32_t* const* call_res, cxh_tape* out) {
+  CXH_TRY
+  if (!out) return cxh_fail("cxh_tape_create_nested: null output handle");
+  if (n_instr < 0 || worksize < 0 || n_in < 0 || n_out < 0 || n_calls < 0)
+    return cxh_fail("cxh_tape_create_nested: negative size");
+  if (n_instr > (1 << 30) || worksize > (1 << 24)) return cxh_fail("cxh_tape_create_nested: tape too large");
+  if (n_calls > 0 && (!callees || !call_dep || !call_res))
+    return cxh_fail("cxh_tape_create_nested: null call table");
+  const Instr* a = static_cast<const Instr*>(instr);
+  std::vector<Instr> flat;
+  flat.reserve(static_cast<size_t>(n_instr));
+  int64_t ws = std::max<int64_t>(worksize, 1);
+  auto slot_ok = [&](int64_t v) { return v >= 0 && v < worksize; };
+  auto assign = [&](int64_t dst, int64_t src) {
+    Instr e{};
+    e.op = OP_ASSIGN;
+    e.i0 = static_cast<int>(dst);
+    e.i1 = e.i2 = static_cast<int>(src);  // a unary operation's second operand is its first
+    flat.push_back(e);
+  };
+  for (int64_t t = 0; t < n_instr; ++t) {
+    const Instr& e = a[t];
+    if (e.op != OP_CALL) {
+      flat.push_back(e);
+      continue;
+    }
+    const std::string where = "cxh_tape_create_nested: instruction " + std::to_string(t) + ": ";
+    if (e.i1 < 0 || e.i1 >= n_calls) throw std::runtime_error(where + "call index out of range");
+    const cxh_tape_s* c = callees[e.i1];
+    if (!c) throw std::runtime_error(where + "null callee");
+    const int32_t* dep = call_dep[e.i1];
+    const int32_t* res = call_res[e.i1];
+    std::vector<int64_t> in_off(c->nnz_in.size() + 1, 0), out_off(c->nnz_out.size() + 1, 0);
+    for (size_t i = 0; i < c->nnz_in.size(); ++i) in_off[i + 1] = in_off[i] + c->nnz_in[i];
+    for (size_t i = 0; i < c->nnz_out.size(); ++i) out_off[i + 1] = out_off[i] + c->nnz_out[i];
+    const int64_t nd = in_off.back(), nr = out_off.back();
+    if ((nd > 0 && !dep) || (nr > 0 && !res)) throw std::runtime_error(where + "null dep or res slots");
+    // Slots after the caller's work vector, shared by every call (one call at a
+    // time, as call_fwd): the arguments, the callee's work vector, the results.
+    const int64_t in0 = worksize, cw = in0 + nd, out0 = cw + c->ws;
+    ws = std::max(ws, out0 + nr);
+    if (ws > (1 << 24)) throw std::runtime_error(where + "work vector too large");
+    // call_fwd gathers every argument before the call ...
+    for (int64_t i = 0; i < nd; ++i) {
+      if (!slot_ok(dep[i])) throw std::runtime_error(where + "argument slot out of range");
+      assign(in0 + i, dep[i]);
+    }
+    // ... the callee runs on its own work vector; an output nonzero its tape
+    // does not write reads as zero ...
+    std::vector<char> written(static_cast<size_t>(nr), 0);
+    for (const Instr& ce : c->alg)
+      if (ce.op == OP_OUTPUT) written[out_off[ce.i0] + ce.i2] = 1;
+    for (int64_t i = 0; i < nr; ++i)
+      if (!written[i]) {
+        Instr z{};
+        z.op = OP_CONST;
+        z.i0 = static_cast<int>(out0 + i);
+        z.d = 0.0;
+        flat.push_back(z);
+      }
+    for (const Instr& ce : c->alg) {
+      Instr f = ce;
+      switch (ce.op) {
+        case OP_CONST:
+          f.i0 = static_cast<int>(cw + ce.i0);
+          flat.push_back(f);
+          break;
+        case OP_INPUT:
+          assign(cw + ce.i0, in0 + in_off[ce.i1] + ce.i2);
+          break;
+        case OP_OUTPUT:
+          assign(out0 + out_off[ce.i0] + ce.i2, cw + ce.i1);
+          break;
+        default:
+          f.i0 = static_cast<int>(cw + ce.i0);
+          f.i1 = static_cast<int>(cw + ce.i1);
+          f.i2 = static_cast<int>(cw + ce.i2);
+          flat.push_back(f);
+      }
+    }
+    // ... and its results are stored after it returns, where they are needed.
+    for (int64_t i = 0; i < nr; ++i) {
+      if (res[i] < 0) continue;
+      if (!slot_ok(res[i])) throw std::runtime_error(where + "result slot out of range");
+      assign(res[i], out0 + i);
+    }
+  }
+  if (static_cast<int64_t>(flat.size()) > (1 << 30)) throw std::runtime_error("cxh_tape_create_nested: tape too large");
+  *out = make_tape(std::move(flat), ws, n_in, nnz_in, n_out, nnz_out).release();
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_tape_instructions(cxh_tape t, void* instr, int64_t cap, int64_t* n_instr, int64_t* worksize) {
+  CXH_TRY
+  if (!t) return cxh_fail("null tape");
+  const int64_t n = static_cast<int64_t>(t->alg.size());
+  if (n_instr) *n_instr = n;
+  if (worksize) *worksize = t->ws;
+  if (instr && cap > 0) std::memcpy(instr, t->alg.data(), static_cast<size_t>(std::min(cap, n)) * sizeof(Instr));
   return 0;
   CXH_CATCH
 }

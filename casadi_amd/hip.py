@@ -41,7 +41,7 @@ def _check_soa(tensors, nnz, n, what):
             raise TypeError(f"{what}[{i}] must be a float64 CUDA tensor")
         if t.dim() != 2 or t.shape[0] != nnz[i] or t.shape[1] < n:
             raise ValueError(f"{what}[{i}] has shape {tuple(t.shape)}, expected ({nnz[i]}, {n})")
-        if nnz[i] > 1 and t.shape[1] > 0 and t.stride(1) != 1:
+        if nnz[i] > 1 and t.shape[1] > 1 and t.stride(1) != 1:   # one instance: its stride is never used
             raise ValueError(f"{what}[{i}] must have unit stride along instances")
         s = t.stride(0) if nnz[i] > 1 else n
         if ld is None:
@@ -68,6 +68,45 @@ class HipTape:
         self._h = h
         self.config(block, w_in_lds)
         self.set_engine(engine)
+
+    @classmethod
+    def nested(cls, instr, worksize, nnz_in, nnz_out, calls, block=64, w_in_lds=True, engine="interp"):
+        """A tape holding nested Function calls (cxh_tape_create_nested): an
+        OP_CALL record's i1 is the index k of calls[k] = (callee HipTape,
+        argument slots, result slots), the slots as CasADi's call records hold
+        them (one per callee input / output nonzero; -1 = result not needed).
+        instr is a structured array of Function.instructions()'s layout."""
+        from .function import INSTR_DTYPE
+        self = cls.__new__(cls)
+        self.f = None
+        self.nnz_in, self.nnz_out = [int(k) for k in nnz_in], [int(k) for k in nnz_out]
+        instr = np.ascontiguousarray(instr, INSTR_DTYPE)
+        ni = np.array(self.nnz_in, np.int64)
+        no = np.array(self.nnz_out, np.int64)
+        dep = [np.ascontiguousarray(c[1], np.int32) for c in calls]
+        res = [np.ascontiguousarray(c[2], np.int32) for c in calls]
+        h = L.P()
+        L.check(L.cxh_tape_create_nested(instr.ctypes.data if instr.size else None, instr.size, int(worksize),
+                                         len(ni), ni.ctypes.data_as(L.PI64), len(no), no.ctypes.data_as(L.PI64),
+                                         len(calls), L.ptr_array([c[0]._h.value for c in calls]),
+                                         L.ptr_array([d.ctypes.data if d.size else None for d in dep]),
+                                         L.ptr_array([r.ctypes.data if r.size else None for r in res]),
+                                         C.byref(h)), "cxh_tape_create_nested")
+        self._h = h
+        self.config(block, w_in_lds)
+        self.set_engine(engine)
+        return self
+
+    def instructions(self):
+        """The tape the kernels run (calls inlined) and its work-vector size."""
+        from .function import INSTR_DTYPE
+        n, ws = L.I64(), L.I64()
+        L.check(L.cxh_tape_instructions(self._h, None, 0, C.byref(n), C.byref(ws)), "cxh_tape_instructions")
+        a = np.zeros(n.value, INSTR_DTYPE)
+        if n.value:
+            L.check(L.cxh_tape_instructions(self._h, a.ctypes.data, n.value, C.byref(n), C.byref(ws)),
+                    "cxh_tape_instructions")
+        return a, ws.value
 
     def set_engine(self, engine):
         """'interp': tape interpreter kernels; 'jit': straight-line kernels
@@ -238,7 +277,7 @@ class HipTape:
                 raise TypeError(f"{what}[{i}] must be an int64 CUDA tensor of 64-bit words")
             if t.dim() != 2 or t.shape[0] != nnz[i] or t.shape[1] < n_words:
                 raise ValueError(f"{what}[{i}] has shape {tuple(t.shape)}, expected ({nnz[i]}, {n_words})")
-            if nnz[i] > 1 and t.shape[1] > 0 and t.stride(1) != 1:
+            if nnz[i] > 1 and t.shape[1] > 1 and t.stride(1) != 1:
                 raise ValueError(f"{what}[{i}] must have unit stride along words")
             s = t.stride(0) if nnz[i] > 1 else n_words
             if ld is None:

@@ -62,6 +62,30 @@ int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize,
                     int32_t n_in, const int64_t* nnz_in,
                     int32_t n_out, const int64_t* nnz_out, cxh_tape* out);
 int cxh_tape_destroy(cxh_tape t);
+
+/* A tape holding nested Function calls.  An OP_CALL record's i1 is the index
+ * k of its call: callees[k] is the called Function's tape, call_dep[k] its
+ * argument slots (one per input nonzero of the callee, inputs in order) and
+ * call_res[k] its result slots (one per output nonzero; -1: not needed).
+ * Every call is inlined -- its arguments copied to slots of their own, the
+ * callee's tape run on slots appended to the work vector, its results copied
+ * out after it ends -- so every kernel sees one flat tape computing what the
+ * call computes.  A callee may itself have been created nested.
+ * Replaces: OP_CALL with its ExtendedAlgEl record
+ *   (casadi/core/sx_function.hpp:287-313), evaluated by SXFunction::call_fwd
+ *   (casadi/core/sx_function.cpp:1596-1620); the tape without calls is the
+ *   same as cxh_tape_create's.                                               */
+int cxh_tape_create_nested(const void* instr, int64_t n_instr, int64_t worksize,
+                           int32_t n_in, const int64_t* nnz_in, int32_t n_out,
+                           const int64_t* nnz_out, int32_t n_calls, const cxh_tape* callees,
+                           const int32_t* const* call_dep, const int32_t* const* call_res,
+                           cxh_tape* out);
+
+/* The tape the kernels run (after inlining): *n_instr records of 16 bytes
+ * (at most cap copied to instr) and its work-vector size.
+ * Replaces: SXFunction::instructions_sx / instruction_* accessors
+ *   (casadi/core/sx_function.hpp:187-231) for a tape held by the library.   */
+int cxh_tape_instructions(cxh_tape t, void* instr, int64_t cap, int64_t* n_instr, int64_t* worksize);
 /* Kernel configuration: threads per block (64/128/256), whether the
  * interpreter's work vector lives in LDS (1) or in HBM scratch (0), and
  * instances per wavefront of the interpreter (8/16/32/64; the 64/ipw lanes of

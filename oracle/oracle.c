@@ -10,6 +10,9 @@
  *   or_eval             SXFunction::eval (casadi/core/sx_function.cpp:72-127)
  *   or_map_eval         Map::eval_gen serial loop (casadi/core/map.cpp:141-157)
  *                       and OmpMap::eval (casadi/core/map.cpp:340-386)
+ *   or_eval_calls       SXFunction::eval of a tape with OP_CALL records, each
+ *                       call as SXFunction::call_fwd does it
+ *                       (casadi/core/sx_function.cpp:72-127, 1596-1620)
  *   or_sp_forward / or_sp_reverse
  *                       SXFunction::sp_forward / sp_reverse, one 64-bit word
  *                       per nonzero (casadi/core/sx_function.cpp:1657-1681,
@@ -233,6 +236,63 @@ int or_eval(const void* instr, int64_t n_instr, const double** arg, double** res
         if (res[e->i0]) res[e->i0][e->u.s.i2] = w[e->u.s.i1];
         break;
       case O_PARAMETER: case O_CALL: return 1;
+      default: w[e->i0] = or_fun(e->op, w[e->u.s.i1], w[e->u.s.i2]);
+    }
+  }
+  return 0;
+}
+
+/* SXFunction::eval with OP_CALL records (casadi/core/sx_function.cpp:72-127).
+ * A call (i1 = k) as SXFunction::call_fwd (1596-1620): every argument
+ * w[dep[k][i]] is gathered first, callee k's tape runs on a work vector of
+ * its own (or_eval), then each result goes to w[res[k][i]] when that slot is
+ * >= 0.  Callee k: c_n[k] instructions at c_instr[k], work vector c_ws[k],
+ * c_nin[k] inputs of c_nnz_in[k][j] nonzeros, c_nout[k] outputs of
+ * c_nnz_out[k][j] nonzeros (at most 16 each); callees hold no calls. */
+int or_eval_calls(const void* instr, int64_t n_instr, const double** arg, double** res, double* w,
+                  const void* const* c_instr, const int64_t* c_n, const int64_t* c_ws, const int* c_nin,
+                  const int64_t* const* c_nnz_in, const int* c_nout, const int64_t* const* c_nnz_out,
+                  const int32_t* const* dep, const int32_t* const* res_slot) {
+  const or_instr* a = (const or_instr*)instr;
+  for (int64_t t = 0; t < n_instr; ++t) {
+    const or_instr* e = a + t;
+    switch (e->op) {
+      case O_CONST: w[e->i0] = e->u.d; break;
+      case O_INPUT: w[e->i0] = arg[e->u.s.i1] ? arg[e->u.s.i1][e->u.s.i2] : 0; break;
+      case O_OUTPUT:
+        if (res[e->i0]) res[e->i0][e->u.s.i2] = w[e->u.s.i1];
+        break;
+      case O_PARAMETER: return 1;
+      case O_CALL: {
+        const int k = e->u.s.i1;
+        int64_t nd = 0, nr = 0;
+        for (int j = 0; j < c_nin[k]; ++j) nd += c_nnz_in[k][j];
+        for (int j = 0; j < c_nout[k]; ++j) nr += c_nnz_out[k][j];
+        double* nz_in = (double*)malloc((size_t)(nd + 1) * sizeof(double));
+        double* nz_out = (double*)calloc((size_t)(nr + 1), sizeof(double));
+        double* cw = (double*)malloc((size_t)(c_ws[k] + 1) * sizeof(double));
+        const double* ca[16];
+        double* cr[16];
+        int64_t off = 0;
+        for (int64_t i = 0; i < nd; ++i) nz_in[i] = w[dep[k][i]];
+        for (int j = 0; j < c_nin[k]; ++j) {
+          ca[j] = nz_in + off;
+          off += c_nnz_in[k][j];
+        }
+        off = 0;
+        for (int j = 0; j < c_nout[k]; ++j) {
+          cr[j] = nz_out + off;
+          off += c_nnz_out[k][j];
+        }
+        const int flag = or_eval(c_instr[k], c_n[k], ca, cr, cw);
+        for (int64_t i = 0; i < nr; ++i)
+          if (res_slot[k][i] >= 0) w[res_slot[k][i]] = nz_out[i];
+        free(nz_in);
+        free(nz_out);
+        free(cw);
+        if (flag) return flag;
+        break;
+      }
       default: w[e->i0] = or_fun(e->op, w[e->u.s.i1], w[e->u.s.i2]);
     }
   }

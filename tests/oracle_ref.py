@@ -46,6 +46,8 @@ def _load():
     lib.or_set_perturb.argtypes = [C.c_int]
     lib.or_adj_map.restype = C.c_int
     lib.or_adj_map.argtypes = [P, I64, I64, C.c_int, PI64, C.c_int, PI64, I64, P, P, P, P, C.c_int]
+    lib.or_eval_calls.restype = C.c_int
+    lib.or_eval_calls.argtypes = [P, I64, P, P, PD, P, PI64, PI64, P, P, P, P, P, P]
     lib.or_sp_forward.restype = C.c_int
     lib.or_sp_forward.argtypes = [P, I64, P, P, P]
     lib.or_sp_reverse.restype = C.c_int
@@ -95,6 +97,36 @@ def map_eval(f, n, args_aos, n_threads=1):
                            len(nnz_out), _pi(nnz_out), n, _ptrs(arg), _ptrs(res), n_threads)
     assert flag == 0, "oracle VM rejected the tape"
     return res
+
+
+def eval_calls(instr, ws, nnz_in, nnz_out, calls, args):
+    """A tape with OP_CALL records on the oracle, instance by instance.
+    calls[k] = (callee Function, argument slots, result slots); args[i]:
+    (n, nnz_in[i]) arrays.  Returns list of (n, nnz_out[j])."""
+    n = args[0].shape[0]
+    cf = [c[0] for c in calls]
+    ci = [f.instructions() for f in cf]
+    c_n = np.array([a.size for a in ci], np.int64)
+    c_ws = np.array([f.sz_w() for f in cf], np.int64)
+    c_nin = (C.c_int * max(len(cf), 1))(*[f.n_in() for f in cf])
+    c_nout = (C.c_int * max(len(cf), 1))(*[f.n_out() for f in cf])
+    nin = [np.array([f.nnz_in(j) for j in range(f.n_in())], np.int64) for f in cf]
+    nout = [np.array([f.nnz_out(j) for j in range(f.n_out())], np.int64) for f in cf]
+    dep = [np.ascontiguousarray(c[1], np.int32) for c in calls]
+    res_slot = [np.ascontiguousarray(c[2], np.int32) for c in calls]
+    instr = np.ascontiguousarray(instr)
+    out = [np.zeros((n, k)) for k in nnz_out]
+    w = np.zeros(max(int(ws), 1))
+    for k in range(n):
+        a = [np.ascontiguousarray(x[k], dtype=float) for x in args]
+        r = [np.zeros(m) for m in nnz_out]
+        flag = lib.or_eval_calls(instr.ctypes.data, instr.size, _ptrs(a), _ptrs(r), w.ctypes.data_as(PD),
+                                 _ptrs(ci), c_n.ctypes.data_as(PI64), c_ws.ctypes.data_as(PI64), c_nin,
+                                 _ptrs(nin), c_nout, _ptrs(nout), _ptrs(dep), _ptrs(res_slot))
+        assert flag == 0, "oracle rejected the tape"
+        for j in range(len(nnz_out)):
+            out[j][k] = r[j]
+    return out
 
 
 def sp_forward(f, seeds):
