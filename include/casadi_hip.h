@@ -57,7 +57,8 @@ typedef struct cxh_qr_s* cxh_qr;
  *   {int op; int i0; union {double d; struct {int i1, i2;};}}, with
  *   casadi::Operation codes (casadi/core/calculus.hpp:60-218).
  * The tape is validated (slot and I/O indices in range, supported ops);
- * OP_CALL and free parameters are rejected.                                 */
+ * Free parameters are rejected; a tape with OP_CALL records goes to
+ * cxh_tape_create_nested.                                                   */
 int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize,
                     int32_t n_in, const int64_t* nnz_in,
                     int32_t n_out, const int64_t* nnz_out, cxh_tape* out);
