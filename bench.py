@@ -128,7 +128,11 @@ def main():
     ms_macc = timed(fused_macc, args.steps, args.warmup)
 
     # dominant kernel: the reverse segment kernels (cxj_rev), timed live with
-    # events on the stream they are launched on (torch's current stream)
+    # events on the stream they are launched on (torch's current stream).  The
+    # sweep is one cxj_rev per segment -- each but the last holding the
+    # boundary-state pass of the segment before it beside its own sub-block
+    # recompute + adjoint sweep -- plus one lone cxj_snap for the last
+    # segment; its time divided by the segment count is the per-launch time.
     outs, ck = tape_f.fwd_ckpt([X0, U], n, every)
     ms_rev = timed(lambda: tape_f.rev_ckpt([X0, U], [SEED], n, every, ck), args.steps, 1)
     rev_launch_ms = ms_rev / st_f["n_seg"]
@@ -219,7 +223,8 @@ def main():
                      "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "hbm_gbs_from_traffic": traffic / (rev_launch_ms * 1e-3) / 1e9 if traffic else None,
                      "hw_fp64_tflops_from_counters": hw_tflops,
-                     "kernel": "cxj_rev (reverse segment: recompute + adjoint sweep)",
+                     "kernel": "cxj_rev (reverse segment: sub-block recompute + adjoint sweep, beside the "
+                               "boundary-state pass of the segment before it)",
                      "kernel_ms": rev_launch_ms, "flops_per_launch": flops_per_launch,
                      "hbm_algorithmic_gbs": bytes_per_inst * n / (ms_fused * 1e-3) / 1e9,
                      "note": "FP64 vector peak; algorithmic tape ops, transcendentals counted once"},

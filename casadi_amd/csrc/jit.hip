@@ -138,6 +138,9 @@ struct JitPlan {
   int64_t snap_rows = 0;  // sub-block boundary states of one reverse kernel (max over segments)
   int ipt = 1;            // instances per thread of the forward kernels
   std::vector<hipFunction_t> fwd, rev;
+  std::vector<int> n_sub;                      // sub-blocks of each segment's reverse kernel
+  std::vector<hipFunction_t> snap, back, pair;  // a reverse kernel split: first pass, the rest, and
+                                                // pair[g] = the rest of g beside the first pass of g-1
 };
 
 // Instances per thread of the forward segment kernels (CXH_JIT_IPT, 1-8):
@@ -153,6 +156,13 @@ struct JitTape {
   int64_t eval_every = 0;            // segment length of forward-only evaluation (0: not chosen yet)
   DeviceBuffer st, adj, ck, snap;
 };
+
+// Whether reverse kernels run split in two, the halves of consecutive
+// segments in one launch (CXH_JIT_SPLIT=0 keeps one kernel per segment).
+static bool split_reverse() {
+  if (const char* e = std::getenv("CXH_JIT_SPLIT")) return std::atoi(e) != 0;
+  return true;
+}
 
 // Instructions per sub-block of the reverse kernels (0: whole segment).
 static int sub_block_len() {
@@ -242,15 +252,20 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
   }
   P.ck_rows = rows;
   for (auto& L : P.live) P.max_live = std::max<int>(P.max_live, static_cast<int>(L.size()));
+  P.n_sub.assign(P.n_seg, 1);
   for (int s = 0; s < P.n_seg; ++s) {
     const std::vector<int> c = sub_bounds(P.t0[s], P.t0[s + 1]);
     const auto lc = live_sets(t, P.t0[s], P.t0[s + 1], P.live[s + 1], c);
     int64_t rows = 0;
     for (size_t j = 1; j + 1 < c.size(); ++j) rows += static_cast<int64_t>(lc[j].size());
     P.snap_rows = std::max(P.snap_rows, rows);
+    P.n_sub[s] = static_cast<int>(c.size()) - 1;
   }
   P.fwd.assign(P.n_seg, nullptr);
   P.rev.assign(P.n_seg, nullptr);
+  P.snap.assign(P.n_seg, nullptr);
+  P.back.assign(P.n_seg, nullptr);
+  P.pair.assign(P.n_seg, nullptr);
   P.ipt = fwd_ipt();
   return t->jit->plans.emplace(every, std::move(P)).first->second;
 }
@@ -303,8 +318,13 @@ static std::string adj_name(int64_t c, int64_t ws) {
 // boundary values enter each recomputation through an empty asm statement so
 // that the compiler cannot merge the recomputation with the first pass (which
 // would keep every first-pass value live again).
+//
+// phase splits a reverse kernel in two: 1 = its first pass only (the sub-block
+// boundary states, and the outputs of every sub-block but the last), 2 = the
+// rest, with the last sub-block reloaded from its stored state as well.  The
+// two halves of consecutive segments then run side by side (jit_adj).
 static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, const std::string& name,
-                            bool sub_blocks = true) {
+                            bool sub_blocks = true, int phase = 0) {
   const int t0 = P.t0[s], t1 = P.t0[s + 1];
   const int len = t1 - t0;
   const int64_t ws = t->ws;
@@ -405,10 +425,13 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   const int m = static_cast<int>(c.size()) - 1;
   const std::vector<std::vector<int>> lc = live_sets(t, t0, t1, lout, c);
   // pass 1: forward over sub-blocks 0..m-2, storing the boundary states to the
-  // snapshot buffer (A.st_out, unused otherwise by reverse kernels)
+  // snapshot buffer (A.st_out, unused otherwise by reverse kernels).  Phase 2
+  // keeps its bookkeeping (names, rows) and drops its code.
   std::vector<std::vector<int64_t>> snap(m + 1), in_code(m + 1), row(m + 1);
   snap[0] = cur;
   int64_t nrow = 0;
+  std::string dropped;
+  if (phase == 2) std::swap(o, dropped);
   for (int j = 0; j + 1 < m; ++j) {
     emit_fwd(c[j], c[j + 1], 0, true);
     snap[j + 1] = cur;
@@ -417,6 +440,11 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
       row[j + 1][slot] = nrow;
       o += "  CXJ_SNAP(A.st_out, " + std::to_string(nrow++) + ") = " + V(cur[slot]) + ";\n";
     }
+  }
+  if (phase == 2) std::swap(o, dropped);
+  if (phase == 1) {
+    o += "}\n";
+    return o;
   }
   std::unordered_set<int64_t> decl;
   auto acc = [&](int64_t code, const std::string& term) {
@@ -429,8 +457,9 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   };
   for (int j = m - 1; j >= 0; --j) {
     const bool last = j == m - 1;
+    const bool cont = last && phase != 2;  // continues from pass 1's values
     cur = snap[j];
-    if (!last && !lc[j].empty()) {
+    if (!cont && !lc[j].empty()) {
       // reload the boundary state through an opaque pointer: neither the
       // values nor the loads can be taken over from the first pass.  The
       // pointer also takes the adjoints sub-block j+1 hands over as inputs, so
@@ -439,10 +468,11 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
       const std::string sp = "cxj_sp" + std::to_string(j);
       o += std::string("  const double* ") + sp + " = " + (j == 0 ? "A.st_in" : "A.st_out") + ";\n  CXJ_KEEP_PTR(" +
            sp + ");\n";
-      for (int slot : lc[j + 1]) {
-        const int64_t from = in_code[j + 1][slot];
-        if (decl.count(from)) o += "  CXJ_AFTER(" + sp + ", " + Adj(from) + ");\n";
-      }
+      if (!last)
+        for (int slot : lc[j + 1]) {
+          const int64_t from = in_code[j + 1][slot];
+          if (decl.count(from)) o += "  CXJ_AFTER(" + sp + ", " + Adj(from) + ");\n";
+        }
       for (int slot : lc[j]) {
         const int64_t h = 4 * (static_cast<int64_t>(j) * ws + slot) + 2;
         if (j == 0 && s == 0) {
@@ -455,7 +485,7 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
       }
     }
     in_code[j] = cur;
-    emit_fwd(c[j], c[j + 1], last ? 0 : 1, last, true);
+    emit_fwd(c[j], c[j + 1], cont ? 0 : 1, last, true);
     if (last) {
       if (s + 1 < P.n_seg)
         for (size_t r = 0; r < lout.size(); ++r) {
@@ -527,13 +557,15 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   return o;
 }
 
-// One segment kernel: one lane per instance, the arguments by value.
-static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev) {
+// One segment kernel: one lane per instance, the arguments by value.  phase
+// (reverse kernels): 0 whole, 1 first pass (cxj_snap), 2 the rest.
+static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, int phase = 0) {
   std::string o = kSxOpsSource;
   o += kPrologue;
-  o += gen_body(t, P, s, rev, "cxj_body");
+  o += gen_body(t, P, s, rev, "cxj_body", true, rev ? phase : 0);
   if (rev || P.ipt == 1) {
-    o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") + (rev ? "cxj_rev" : "cxj_fwd") +
+    o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") +
+         (rev ? (phase == 1 ? "cxj_snap" : "cxj_rev") : "cxj_fwd") +
          "(const CxjArgs A) {\n"
          "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
          "  if (k < A.n) cxj_body(A, k);\n"
@@ -646,19 +678,69 @@ static hipFunction_t segment_kernel(cxh_tape_s* t, JitPlan& P, int s, bool rev) 
   return f;
 }
 
+// The two halves of a split reverse kernel (phase 1 or 2).
+static hipFunction_t split_kernel(cxh_tape_s* t, JitPlan& P, int s, int phase) {
+  hipFunction_t& f = phase == 1 ? P.snap[s] : P.back[s];
+  if (!f)
+    f = jit_function_checked(gen_segment(t, P, s, true, phase), phase == 1 ? "cxj_snap" : "cxj_rev",
+                             "use sub-blocks of at most 256 instructions (CXH_JIT_SUBBLOCK), a shorter checkpoint "
+                             "interval or the interpreter engine");
+  return f;
+}
+
+// Segment g's reverse kernel after its first pass (blockIdx.y = 0, arguments
+// A) beside the first pass of segment g-1 (blockIdx.y = 1, arguments B): the
+// two halves are independent, and one launch runs them on twice the SIMDs.
+static hipFunction_t pair_kernel(cxh_tape_s* t, JitPlan& P, int g) {
+  hipFunction_t& f = P.pair[g];
+  if (!f) {
+    std::string o = kSxOpsSource;
+    o += kPrologue;
+    o += gen_body(t, P, g, true, "cxj_back_body", true, 2);
+    o += gen_body(t, P, g - 1, true, "cxj_snap_body", true, 1);
+    o += "extern \"C\" __global__ void __launch_bounds__(256) cxj_rev(const CxjArgs A, const CxjArgs B) {\n"
+         "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+         "  if (k >= A.n) return;\n"
+         "  if (blockIdx.y == 0) cxj_back_body(A, k);\n"
+         "  else cxj_snap_body(B, k);\n"
+         "}\n";
+    f = jit_function_checked(finish_source(std::move(o)), "cxj_rev",
+                             "use sub-blocks of at most 256 instructions (CXH_JIT_SUBBLOCK), a shorter checkpoint "
+                             "interval or the interpreter engine");
+  }
+  return f;
+}
+
 static void launch(hipFunction_t f, int64_t n, int block, CxjArgs& A, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>((n + block - 1) / block);
   void* params[] = {&A};
   hip_check(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, s, params, nullptr), "jit kernel launch");
 }
 
+static void launch_pair(hipFunction_t f, int64_t n, int block, CxjArgs& A, CxjArgs& B, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((n + block - 1) / block);
+  void* params[] = {&A, &B};
+  hip_check(hipModuleLaunchKernel(f, grid, 2, 1, block, 1, 1, 0, s, params, nullptr), "jit kernel launch");
+}
+
 // ------------------------------------------------------------ drivers ---
+
+// Whether the reverse sweep of plan P runs split (jit_adj).
+static bool split_plan(const JitPlan& P) { return split_reverse() && P.n_seg > 1 && P.snap_rows > 0; }
 
 void jit_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
   JitPlan& P = plan_for(t, every);
+  const bool split = split_plan(P);
   for (int s = 0; s < P.n_seg; ++s) {
     if (fwd) segment_kernel(t, P, s, false);
-    if (rev) segment_kernel(t, P, s, true);
+    if (!rev) continue;
+    if (!split) {
+      segment_kernel(t, P, s, true);
+    } else {
+      if (s == P.n_seg - 1) split_kernel(t, P, s, 1);
+      if (s == 0) split_kernel(t, P, s, 2);
+      else pair_kernel(t, P, s);
+    }
   }
 }
 
@@ -737,26 +819,60 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
   if (mode == 1) return 0;
   if (P.n_seg > 1) t->jit->adj.reserve(2 * static_cast<size_t>(P.max_live) * lds * sizeof(double));
   double* adj = t->jit->adj.as<double>();
-  if (P.snap_rows) t->jit->snap.reserve(static_cast<size_t>(P.snap_rows) * lds * sizeof(double));
+  const bool split = split_plan(P);
+  const size_t snap_elems = static_cast<size_t>(P.snap_rows) * lds;
+  if (P.snap_rows) t->jit->snap.reserve((split ? 2 : 1) * snap_elems * sizeof(double));
   double* snap = P.snap_rows ? t->jit->snap.as<double>() : nullptr;
   A.aseed = aseed;
   A.asens = asens;
-  for (int g = P.n_seg - 1; g >= 0; --g) {
-    hipFunction_t f = segment_kernel(t, P, g, true);
-    A.res = (mode == 0 && g == P.n_seg - 1) ? res : none;
+  auto set_adj = [&](CxjArgs& B, int g) {
+    B.adj_in = adj ? adj + static_cast<int64_t>((g + 1) % 2) * P.max_live * lds : nullptr;
+    B.adj_out = adj ? adj + static_cast<int64_t>(g % 2) * P.max_live * lds : nullptr;
+  };
+  if (!split) {
+    for (int g = P.n_seg - 1; g >= 0; --g) {
+      hipFunction_t f = segment_kernel(t, P, g, true);
+      A.res = (mode == 0 && g == P.n_seg - 1) ? res : none;
+      A.st_in = state(g);
+      A.st_out = snap;
+      set_adj(A, g);
+      launch(f, n, t->block, A, s);
+    }
+    return 0;
+  }
+  // Split: the first pass of the last segment alone, then for g = n_seg-1..1
+  // the rest of segment g beside the first pass of segment g-1 in one launch,
+  // then the rest of segment 0.  The boundary states of consecutive segments
+  // alternate between two buffers.
+  auto buf = [&](int g) { return snap + static_cast<size_t>(g % 2) * snap_elems; };
+  const int last = P.n_seg - 1;
+  CxjArgs B = A;
+  B.res = mode == 0 ? res : none;
+  B.st_in = state(last);
+  B.st_out = buf(last);
+  launch(split_kernel(t, P, last, 1), n, t->block, B, s);
+  for (int g = last; g >= 0; --g) {
+    A.res = (mode == 0 && g == last) ? res : none;
     A.st_in = state(g);
-    A.st_out = snap;
-    A.adj_in = adj ? adj + static_cast<int64_t>((g + 1) % 2) * P.max_live * lds : nullptr;
-    A.adj_out = adj ? adj + static_cast<int64_t>(g % 2) * P.max_live * lds : nullptr;
-    launch(f, n, t->block, A, s);
+    A.st_out = buf(g);
+    set_adj(A, g);
+    if (g == 0) {
+      launch(split_kernel(t, P, 0, 2), n, t->block, A, s);
+      break;
+    }
+    B.res = none;
+    B.st_in = state(g - 1);
+    B.st_out = buf(g - 1);
+    launch_pair(pair_kernel(t, P, g), n, t->block, A, B, s);
   }
   return 0;
 }
 
-std::string jit_source(cxh_tape_s* t, int64_t every, int seg, bool rev) {
+std::string jit_source(cxh_tape_s* t, int64_t every, int seg, int rev) {
   JitPlan& P = plan_for(t, every);
   if (seg < 0 || seg >= P.n_seg) throw std::runtime_error("segment index out of range");
-  return gen_segment(t, P, seg, rev);
+  if (rev < 0 || rev > 3) throw std::runtime_error("reverse must be 0, 1, 2 or 3");
+  return gen_segment(t, P, seg, rev != 0, rev >= 2 ? rev - 1 : 0);
 }
 
 // Operand codes (instruction indices) of the values whose sin and cos are
@@ -2524,7 +2640,7 @@ int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, i
     if (len) *len = 0;
     return 0;
   }
-  const std::string src = jit_source(t, checkpoint_every, segment, reverse != 0);
+  const std::string src = jit_source(t, checkpoint_every, segment, reverse);
   if (len) *len = static_cast<int64_t>(src.size());
   if (buf && cap > 0) {
     const size_t m = std::min<size_t>(src.size(), static_cast<size_t>(cap - 1));

@@ -109,9 +109,11 @@ int cxh_tape_engine(cxh_tape t, int32_t engine);
 /* Compile (ahead of the first launch) the specialised kernels for a
  * checkpoint interval: forward segments and/or reverse segments.            */
 int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, int32_t reverse);
-/* Introspection: generated HIP source of one segment kernel (reverse != 0:
- * the recompute + reverse-sweep kernel) into buf (NUL-terminated, truncated
- * to cap), its full length in *len and the number of segments.               */
+/* Introspection: generated HIP source of one segment kernel into buf
+ * (NUL-terminated, truncated to cap), its full length in *len and the number
+ * of segments.  reverse: 0 the forward kernel, 1 the whole recompute +
+ * reverse-sweep kernel, 2 and 3 its two halves when split (2: the first pass
+ * storing the sub-block boundary states, 3: the rest).                      */
 int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, int32_t reverse,
                         char* buf, int64_t cap, int64_t* len, int32_t* n_segments);
 /* Introspection: generated source of the specialised tangent kernel of one

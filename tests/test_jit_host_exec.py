@@ -84,8 +84,11 @@ def _compile(sources, workdir, groups=None):
     return libs
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("sub", ["0", "40"])
-def test_generated_segments_run_on_host(monkeypatch, sub):
+def test_generated_segments_run_on_host(monkeypatch, sub, split):
+    """split: each reverse kernel as its two halves -- the first pass storing
+    the sub-block boundary states, then the rest reloading every sub-block."""
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     monkeypatch.setenv("CXH_JIT_SUBBLOCK", sub)
@@ -95,6 +98,12 @@ def test_generated_segments_run_on_host(monkeypatch, sub):
     nseg = (f.n_instructions() + every - 1) // every
     fwd = [tape.jit_source(every, s, False)[0] for s in range(nseg)]
     rev = [tape.jit_source(every, s, True)[0] for s in range(nseg)]
+    first = [tape.jit_source(every, s, 2)[0] for s in range(nseg)]
+    rest = [tape.jit_source(every, s, 3)[0] for s in range(nseg)]
+    assert all("__global__ void __launch_bounds__(256) cxj_snap(" in x for x in first)
+    body = [x.split("void cxj_body(")[1] for x in first + rest]
+    assert not any("op_der_full(" in x or "CXJ_ADJ" in x for x in body[:nseg])   # no adjoint work in the first pass
+    assert not any(re.search(r"CXJ_SNAP\(A\.st_out, \d+\) =", x) for x in body[nseg:])   # no states stored
     if sub != "0":
         assert any("CXJ_SNAP(" in r for r in rev)       # sub-blocks were generated
     rows = [len(re.findall(r"CXJ_ST_IN\(", r)) for r in rev]  # live slots at each segment start
@@ -103,8 +112,10 @@ def test_generated_segments_run_on_host(monkeypatch, sub):
     x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (4, n))
     seed = rng.standard_normal((1, n))
     with tempfile.TemporaryDirectory() as d:
-        libs = _compile(fwd[:-1] + rev, d)
+        libs = _compile(fwd[:-1] + (first + rest if split else rev), d)
         fk, rk = [fn for _, fn in libs[:nseg - 1]], [fn for _, fn in libs[nseg - 1:]]
+        if split:
+            fp, rk = rk[:nseg], rk[nseg:]
         state = [np.zeros((max(r, 1), n)) for r in rows] + [np.zeros((1, n))]
         J = np.zeros((1, n))
         gx, gu = np.zeros((4, n)), np.zeros((4, n))
@@ -126,6 +137,8 @@ def test_generated_segments_run_on_host(monkeypatch, sub):
             A.st_in = state[s].ctypes.data
             A.adj_in = adj[(s + 1) % 2].ctypes.data
             A.adj_out = adj[s % 2].ctypes.data
+            if split:
+                fp[s](C.byref(A), n)
             rk[s](C.byref(A), n)
     (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
     assert np.array_equal(J, Jo.T)
