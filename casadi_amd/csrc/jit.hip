@@ -2133,13 +2133,16 @@ static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
 // adjoint; 2: forward storing the checkpoints; 3: adjoint from stored
 // checkpoints.  Checkpoints (S = ceil(N/K) rows of the state, [row][lds]) at
 // A.st_in; the states of one segment's steps (K rows) at A.st_out.
-static const char* const kMaccKernel[] = {"cxj_macc_fwd", "cxj_macc_adj", "cxj_macc_fwd_ckpt", "cxj_macc_rev_ckpt"};
+// 4: one segment's reverse steps beside the next-earlier segment's recompute
+// (cxj_macc_pair, launched once per segment after mode 2).
+static const char* const kMaccKernel[] = {"cxj_macc_fwd", "cxj_macc_adj", "cxj_macc_fwd_ckpt", "cxj_macc_rev_ckpt",
+                                          "cxj_macc_pair"};
 
 static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, int64_t K) {
   std::string o = kSxOpsSource;
   o += kPrologue;
   o += gen_macc_step_fwd(t, M);
-  if (mode == 1 || mode == 3) o += gen_macc_step_rev(t, M);
+  if (mode == 1 || mode == 3 || mode == 4) o += gen_macc_step_rev(t, M);
   const std::string ns = std::to_string(std::max<int64_t>(M.ns, 1)), nsi = std::to_string(M.ns);
   const std::string N = std::to_string(M.N), Ks = std::to_string(K);
   const int64_t S = (M.N + K - 1) / K;
@@ -2150,6 +2153,52 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
       load_x0 += "  s[" + sj + "] = CXJ_IN(" + std::to_string(i) + ", " + std::to_string(j) + ");\n";
       store_sens += "  CXJ_SENS(" + std::to_string(i) + ", " + std::to_string(j) + ", as[" + sj + "]);\n";
     }
+  if (mode == 4) {
+    // Segment seg's reverse steps from its stored step states (A.st_out), the
+    // state adjoint handed from launch to launch in A.adj_in / A.adj_out; and
+    // the recompute of a segment from its checkpoint (A.st_in) storing its
+    // step states.  Same arithmetic, in the same order, as mode 1.
+    o += "__device__ __forceinline__ void cxj_back(const CxjArgs& A, const long long k, const long long seg) {\n"
+         "  double s[" + ns + "], as[" + ns + "];\n"
+         "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
+         "  if (seg == " + std::to_string(S) + " - 1)\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) as[j] = 0.0;\n"
+         "  else\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) as[j] = CXJ_ADJ_IN(j);\n"
+         "#pragma unroll 1\n"
+         "  for (long long step = t1 - 1; step >= t0; --step) {\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j);\n"
+         "    cxj_step_rev(A, k, step, s, as);\n"
+         "  }\n"
+         "  if (seg == 0) {\n" + store_sens + "  } else {\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) CXJ_ADJ_OUT(j) = as[j];\n"
+         "  }\n"
+         "}\n"
+         "__device__ __forceinline__ void cxj_recompute(const CxjArgs& A, const long long k, const long long seg) {\n"
+         "  double s[" + ns + "], sn[" + ns + "];\n"
+         "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
+         "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n"
+         "#pragma unroll 1\n"
+         "  for (long long step = t0; step < t1; ++step) {\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j) = s[j];\n"
+         "    if (step + 1 < t1) {\n"
+         "      cxj_step_fwd(A, k, step, s, sn, false);\n"
+         "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+         "    }\n"
+         "  }\n"
+         "}\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) cxj_macc_pair(const CxjArgs A, const CxjArgs B, "
+         "const long long seg) {\n"
+         "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+         "  if (k >= A.n) return;\n"
+         "  if (blockIdx.y == 0) {\n"
+         "    if (seg < " + std::to_string(S) + ") cxj_back(A, k, seg);\n"
+         "  } else if (seg >= 1) {\n"
+         "    cxj_recompute(B, k, seg - 1);\n"
+         "  }\n"
+         "}\n";
+    return finish_source(std::move(o));
+  }
   o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
        "  double s[" + ns + "], sn[" + ns + "], as[" + ns + "];\n"
        "  double* ck = const_cast<double*>(A.st_in);\n";
@@ -2343,7 +2392,7 @@ int cxh_mapaccum_source(cxh_mapaccum m, int32_t adjoint, int64_t checkpoint_step
   CXH_TRY
   if (!m) return cxh_fail("null mapaccum");
   const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
-  if (adjoint < 0 || adjoint > 3) return cxh_fail("cxh_mapaccum_source: mode must be 0..3");
+  if (adjoint < 0 || adjoint > 4) return cxh_fail("cxh_mapaccum_source: mode must be 0..4");
   const std::string src = gen_macc(m->tape, m->plan, adjoint, K);
   if (len) *len = static_cast<int64_t>(src.size());
   if (buf && cap > 0) {
@@ -2364,10 +2413,6 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
   if (n == 0) return 0;
   const int64_t K = checkpoint_steps > 0 ? std::min(checkpoint_steps, m->plan.N) : m->plan.N;
-  hipFunction_t& f = m->fn[{mode, mode == 0 ? 0 : K}];
-  if (!f)
-    f = jit_function_checked(gen_macc(m->tape, m->plan, mode, K), kMaccKernel[mode],
-                             "evaluate the unrolled chain with the interpreter engine instead");
   const int64_t lds = round_up(n, 64);
   const int64_t S = (m->plan.N + K - 1) / K;
   const int64_t ns = std::max<int64_t>(m->plan.ns, 1);
@@ -2378,6 +2423,48 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
   A.asens = make_io(adj_in, m->nnz_in, layout, ld);
   A.n = n;
   A.lds = lds;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if ((mode == 1 || mode == 3) && split_reverse() && S > 1) {
+    // The reverse chain per segment, each launch running one segment's reverse
+    // steps beside the recompute of the segment before it: checkpoints (S
+    // states), the step states of two segments, the state adjoint twice.
+    const int64_t rows = (ckpt_ext ? 0 : S * ns) + 2 * K * ns + 2 * ns;
+    m->scratch.reserve(static_cast<size_t>(rows) * lds * sizeof(double));
+    double* base = m->scratch.as<double>();
+    double* ck = ckpt_ext ? ckpt_ext : base;
+    double* states = base + (ckpt_ext ? 0 : S * ns * lds);
+    double* adjb = states + 2 * K * ns * lds;
+    if (mode == 1) {  // forward storing the checkpoints (and the outputs)
+      hipFunction_t& ff = m->fn[{2, K}];
+      if (!ff)
+        ff = jit_function_checked(gen_macc(m->tape, m->plan, 2, K), kMaccKernel[2],
+                                  "evaluate the unrolled chain with the interpreter engine instead");
+      A.st_in = ck;
+      launch(ff, n, m->tape->block, A, st);
+    }
+    hipFunction_t& fp = m->fn[{4, K}];
+    if (!fp)
+      fp = jit_function_checked(gen_macc(m->tape, m->plan, 4, K), kMaccKernel[4],
+                                "evaluate the unrolled chain with the interpreter engine instead");
+    A.res = IOTable{};
+    A.st_in = ck;
+    CxjArgs B = A;
+    const unsigned grid = static_cast<unsigned>((n + m->tape->block - 1) / m->tape->block);
+    for (long long seg = S; seg >= 0; --seg) {
+      A.st_out = states + (seg % 2) * K * ns * lds;
+      A.adj_in = adjb + ((seg + 1) % 2) * ns * lds;
+      A.adj_out = adjb + (seg % 2) * ns * lds;
+      B.st_out = states + ((seg + 1) % 2) * K * ns * lds;  // the buffer of seg - 1
+      void* params[] = {&A, &B, &seg};
+      hip_check(hipModuleLaunchKernel(fp, grid, 2, 1, m->tape->block, 1, 1, 0, st, params, nullptr),
+                "jit kernel launch");
+    }
+    return 0;
+  }
+  hipFunction_t& f = m->fn[{mode, mode == 0 ? 0 : K}];
+  if (!f)
+    f = jit_function_checked(gen_macc(m->tape, m->plan, mode, K), kMaccKernel[mode],
+                             "evaluate the unrolled chain with the interpreter engine instead");
   if (mode != 0) {
     const int64_t rows = (ckpt_ext ? 0 : S * ns) + (mode == 2 ? 0 : K * ns);
     m->scratch.reserve(static_cast<size_t>(std::max<int64_t>(rows, 1)) * lds * sizeof(double));
@@ -2385,7 +2472,7 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
     A.st_in = ckpt_ext ? ckpt_ext : base;
     A.st_out = ckpt_ext ? base : base + S * ns * lds;
   }
-  launch(f, n, m->tape->block, A, static_cast<hipStream_t>(stream));
+  launch(f, n, m->tape->block, A, st);
   return 0;
 }
 

@@ -286,7 +286,9 @@ int cxh_mapaccum_rev_ckpt(cxh_mapaccum m, int64_t n, const double* const* arg,
                           const double* const* adj_out, double* const* adj_in, int32_t layout,
                           int64_t ld, int64_t checkpoint_steps, const double* ckpt, void* stream);
 /* Introspection: generated source of a kernel: mode 0 forward chain, 1 fused
- * forward + adjoint, 2 forward storing checkpoints, 3 adjoint from them.    */
+ * forward + adjoint, 2 forward storing checkpoints, 3 adjoint from them,
+ * 4 one segment's reverse steps beside the recompute of the segment before
+ * it (what modes 1 and 3 launch once per segment after the checkpoints).   */
 int cxh_mapaccum_source(cxh_mapaccum m, int32_t mode, int64_t checkpoint_steps, char* buf,
                         int64_t cap, int64_t* len);
 

@@ -98,3 +98,25 @@ def test_mapaccum_behind_autograd(cuda):
     (Yf,), (g0, gU) = HipMapAccum.from_function(F).fwd_adj([XC0.detach(), U.detach()], [S], n, checkpoint_steps=10)
     assert torch.equal(Y.detach(), Yf)
     assert torch.equal(XC0.grad, g0) and torch.equal(U.grad, gU)
+
+
+def test_mapaccum_split_reverse_is_bit_identical(cuda, monkeypatch):
+    """The reverse run per segment (each launch: one segment's reverse steps
+    beside the recompute of the segment before it) against the single
+    fused kernel (CXH_JIT_SPLIT=0): the same bits, fused and from stored
+    checkpoints."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipMapAccum
+    N, n, K = 30, 3000, 4
+    m = HipMapAccum(W.cartpole_cost_step(), N)
+    rng = np.random.default_rng(seed_of(2) + 7)
+    X0 = np.vstack([rng.uniform(-1, 1, (4, n)), np.zeros((1, n))])
+    U = rng.uniform(-3, 3, (N, n))
+    S = rng.standard_normal((5 * N, n))
+    got = {}
+    for split in ("1", "0"):
+        monkeypatch.setenv("CXH_JIT_SPLIT", split)
+        print(f"[case] mapaccum split={split} N={N} K={K} n={n}", flush=True)
+        (o,), (g0, gU) = m.fwd_adj(_dev(X0, U), _dev(S), n, checkpoint_steps=K)
+        got[split] = [x.cpu().numpy() for x in (o, g0, gU)]
+    assert all(np.array_equal(a, b) for a, b in zip(got["1"], got["0"]))

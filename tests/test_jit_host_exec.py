@@ -265,6 +265,51 @@ def test_mapaccum_loop_kernels_run_on_host(K):
     (_,), (r0, rU) = O.adj_map(F, n, [x0.T, U.T], [seed.T])
     assert O.rel_err(g0, r0.T) <= 1e-13 and O.rel_err(gU, rU.T) <= 1e-13
     assert np.array_equal(h0, g0) and np.array_equal(hU, gU)
+    # the reverse per segment (mode 4): checkpoints, then for seg = S .. 0 the
+    # reverse steps of seg beside the recompute of seg - 1 -- bit for bit the
+    # single-kernel adjoint
+    S = (N + K - 1) // K if K else 1
+    Kq = K if K else N
+    src = m.source(4, K)
+    cut = src.index('extern "C" __global__')
+    code = ("#define __device__\n#define __forceinline__ inline\n#define CXJ_KEEP(x) ((void)0)\n"
+            "#define CXJ_KEEP_PTR(x) ((void)0)\n#define CXJ_AFTER(p, x) ((void)0)\n#define CXJ_GLOBAL\n"
+            "#define CXJ_OPAQUE(p) ((void)0)\n#define CXJ_OPAQUE_AFTER(p, x) ((void)0)\n" + src[:cut] +
+            "extern \"C\" void run_pair(const CxjArgs* A, const CxjArgs* B, long long n, long long seg) {\n"
+            f"  if (seg < {S}) for (long long k = 0; k < n; ++k) cxj_back(*A, k, seg);\n"
+            "  if (seg >= 1) for (long long k = 0; k < n; ++k) cxj_recompute(*B, k, seg - 1);\n}\n")
+    with tempfile.TemporaryDirectory() as d:
+        path, so = os.path.join(d, "pair.cpp"), os.path.join(d, "pair.so")
+        with open(path, "w") as fh:
+            fh.write(code)
+        subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w", path, "-o", so],
+                       check=True)
+        run = C.CDLL(so).run_pair
+        run.argtypes = [C.POINTER(CxjArgs), C.POINTER(CxjArgs), C.c_longlong, C.c_longlong]
+        (_, fck2), = _compile([m.source(2, K)], d)
+        ck3 = np.zeros((S * 5, n))
+        states = [np.zeros((Kq * 5, n)) for _ in range(2)]
+        adjb = [np.zeros((5, n)) for _ in range(2)]
+        k0, kU, out3 = np.zeros((5, n)), np.zeros((N, n)), np.zeros((5 * N, n))
+        A3 = CxjArgs()
+        A3.n, A3.lds = n, n
+        A3.arg = _table([x0, U], n)
+        A3.res = _table([out3], n)
+        A3.st_in = ck3.ctypes.data
+        fck2(C.byref(A3), n)
+        A3.res = _table([], n)
+        A3.aseed = _table([seed], n)
+        A3.asens = _table([k0, kU], n)
+        B3 = CxjArgs()
+        C.memmove(C.byref(B3), C.byref(A3), C.sizeof(CxjArgs))
+        for seg in range(S, -1, -1):
+            A3.st_out = states[seg % 2].ctypes.data
+            A3.adj_in = adjb[(seg + 1) % 2].ctypes.data
+            A3.adj_out = adjb[seg % 2].ctypes.data
+            B3.st_out = states[(seg + 1) % 2].ctypes.data
+            run(C.byref(A3), C.byref(B3), n, seg)
+    assert np.array_equal(out3, ref.T)
+    assert np.array_equal(k0, g0) and np.array_equal(kU, gU)
 
 
 def test_specialised_ldl_runs_on_host():
