@@ -2209,6 +2209,10 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
     if (mode != 0)
       o += "    if (step % " + Ks + " == 0)\n"
            "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n";
+    if (mode == 2)  // with A.st_out, also the step states of the last segment (what its recompute would store)
+      o += "    if (A.st_out && step >= " + std::to_string((S - 1) * K) + ")\n"
+           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + std::to_string((S - 1) * K) +
+           ") * " + nsi + " + j) = s[j];\n";
     o += "    cxj_step_fwd(A, k, step, s, sn, true);\n"
          "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
          "  }\n";
@@ -2434,13 +2438,16 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
     double* ck = ckpt_ext ? ckpt_ext : base;
     double* states = base + (ckpt_ext ? 0 : S * ns * lds);
     double* adjb = states + 2 * K * ns * lds;
-    if (mode == 1) {  // forward storing the checkpoints (and the outputs)
+    long long first = S;  // the first pair launch: seg = S recomputes segment S - 1 only
+    if (mode == 1) {      // forward storing the checkpoints, the outputs and the last segment's step states
       hipFunction_t& ff = m->fn[{2, K}];
       if (!ff)
         ff = jit_function_checked(gen_macc(m->tape, m->plan, 2, K), kMaccKernel[2],
                                   "evaluate the unrolled chain with the interpreter engine instead");
       A.st_in = ck;
+      A.st_out = states + ((S - 1) % 2) * K * ns * lds;
       launch(ff, n, m->tape->block, A, st);
+      first = S - 1;
     }
     hipFunction_t& fp = m->fn[{4, K}];
     if (!fp)
@@ -2450,7 +2457,7 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
     A.st_in = ck;
     CxjArgs B = A;
     const unsigned grid = static_cast<unsigned>((n + m->tape->block - 1) / m->tape->block);
-    for (long long seg = S; seg >= 0; --seg) {
+    for (long long seg = first; seg >= 0; --seg) {
       A.st_out = states + (seg % 2) * K * ns * lds;
       A.adj_in = adjb + ((seg + 1) % 2) * ns * lds;
       A.adj_out = adjb + (seg % 2) * ns * lds;
@@ -2470,7 +2477,7 @@ static int macc_launch(cxh_mapaccum m, int mode, int64_t n, const double* const*
     m->scratch.reserve(static_cast<size_t>(std::max<int64_t>(rows, 1)) * lds * sizeof(double));
     double* base = m->scratch.as<double>();
     A.st_in = ckpt_ext ? ckpt_ext : base;
-    A.st_out = ckpt_ext ? base : base + S * ns * lds;
+    A.st_out = mode == 2 ? nullptr : ckpt_ext ? base : base + S * ns * lds;
   }
   launch(f, n, m->tape->block, A, st);
   return 0;

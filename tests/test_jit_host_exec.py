@@ -265,9 +265,9 @@ def test_mapaccum_loop_kernels_run_on_host(K):
     (_,), (r0, rU) = O.adj_map(F, n, [x0.T, U.T], [seed.T])
     assert O.rel_err(g0, r0.T) <= 1e-13 and O.rel_err(gU, rU.T) <= 1e-13
     assert np.array_equal(h0, g0) and np.array_equal(hU, gU)
-    # the reverse per segment (mode 4): checkpoints, then for seg = S .. 0 the
-    # reverse steps of seg beside the recompute of seg - 1 -- bit for bit the
-    # single-kernel adjoint
+    # the reverse per segment (mode 4): checkpoints and the last segment's step
+    # states, then for seg = S-1 .. 0 the reverse steps of seg beside the
+    # recompute of seg - 1 -- bit for bit the single-kernel adjoint
     S = (N + K - 1) // K if K else 1
     Kq = K if K else N
     src = m.source(4, K)
@@ -296,13 +296,14 @@ def test_mapaccum_loop_kernels_run_on_host(K):
         A3.arg = _table([x0, U], n)
         A3.res = _table([out3], n)
         A3.st_in = ck3.ctypes.data
+        A3.st_out = states[(S - 1) % 2].ctypes.data   # the forward also stores the last segment's step states
         fck2(C.byref(A3), n)
         A3.res = _table([], n)
         A3.aseed = _table([seed], n)
         A3.asens = _table([k0, kU], n)
         B3 = CxjArgs()
         C.memmove(C.byref(B3), C.byref(A3), C.sizeof(CxjArgs))
-        for seg in range(S, -1, -1):
+        for seg in range(S - 1, -1, -1):
             A3.st_out = states[seg % 2].ctypes.data
             A3.adj_in = adjb[(seg + 1) % 2].ctypes.data
             A3.adj_out = adjb[seg % 2].ctypes.data
