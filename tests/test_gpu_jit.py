@@ -176,3 +176,35 @@ def test_jit_jacobian_matches_interpreter(ca, cuda):
     assert np.array_equal(ji.cpu().numpy(), jj.cpu().numpy())
     ref = np.concatenate(O.map_eval(f.jacobian(), n, [x.T, u.T, None]), axis=1).T
     assert O.rel_err(jj.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("every", [900, 257])
+def test_jit_split_reverse_ragged_segments(ca, cuda, monkeypatch, every):
+    """The split reverse sweep (each launch: one segment's sub-block recompute
+    + adjoint beside the boundary-state pass of the segment before it) with a
+    short last segment of a single sub-block (every = 900: 900, 900, 900, 81
+    instructions) or segments of one or two sub-blocks (every = 257): the
+    bits of the unsplit kernels and of the interpreter, fused and from stored
+    checkpoints, on a ragged batch."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(20)
+    n = 1000
+    rng = np.random.default_rng(seed_of(2) + every)
+    x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (20, n))
+    seed = rng.standard_normal((1, n))
+    print(f"[case] jit split rollout20 n={n} every={every}", flush=True)
+    ti = HipTape(f)
+    (Ji,), gi = ti.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
+    got = {}
+    for split in ("1", "0"):
+        monkeypatch.setenv("CXH_JIT_SPLIT", split)
+        tj = HipTape(f, engine="jit")
+        (Jj,), gj = tj.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
+        outs, ck = tj.fwd_ckpt(_dev(x0, us), n, every)
+        g2 = tj.rev_ckpt(_dev(x0, us), _dev(seed), n, every, ck)
+        got[split] = [Jj.cpu().numpy()] + _np(gj) + _np(g2)
+    assert all(np.array_equal(a, b) for a, b in zip(got["1"], got["0"]))
+    assert np.array_equal(got["1"][0], Ji.cpu().numpy())
+    for x, y in zip(got["1"][1:3], _np(gi)):
+        assert np.array_equal(x, y)
