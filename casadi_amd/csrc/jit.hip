@@ -138,6 +138,7 @@ struct JitPlan {
   int64_t snap_rows = 0;  // sub-block boundary states of one reverse kernel (max over segments)
   int ipt = 1;            // instances per thread of the forward kernels
   std::vector<hipFunction_t> fwd, rev;
+  bool trig_cache = false;                     // pass 1 stores sin/cos pairs for the recomputation
   std::vector<int> n_sub;                      // sub-blocks of each segment's reverse kernel
   std::vector<hipFunction_t> snap, back, pair;  // a reverse kernel split: first pass, the rest, and
                                                 // pair[g] = the rest of g beside the first pass of g-1
@@ -161,6 +162,14 @@ struct JitTape {
 // segments in one launch (CXH_JIT_SPLIT=0 keeps one kernel per segment).
 static bool split_reverse() {
   if (const char* e = std::getenv("CXH_JIT_SPLIT")) return std::atoi(e) != 0;
+  return true;
+}
+
+// Whether the first pass of a reverse kernel stores the sin/cos pairs of its
+// sub-blocks for their recomputation to load (CXH_JIT_TRIG_CACHE=0: computed
+// again).  Fixed per plan when the plan is made.
+static bool trig_cache_enabled() {
+  if (const char* e = std::getenv("CXH_JIT_TRIG_CACHE")) return std::atoi(e) != 0;
   return true;
 }
 
@@ -253,11 +262,24 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
   P.ck_rows = rows;
   for (auto& L : P.live) P.max_live = std::max<int>(P.max_live, static_cast<int>(L.size()));
   P.n_sub.assign(P.n_seg, 1);
+  P.trig_cache = trig_cache_enabled();
   for (int s = 0; s < P.n_seg; ++s) {
     const std::vector<int> c = sub_bounds(P.t0[s], P.t0[s + 1]);
     const auto lc = live_sets(t, P.t0[s], P.t0[s + 1], P.live[s + 1], c);
     int64_t rows = 0;
     for (size_t j = 1; j + 1 < c.size(); ++j) rows += static_cast<int64_t>(lc[j].size());
+    if (P.trig_cache) {  // two rows per sincos site of sub-blocks 0..m-2 (one site per operand per sub-block)
+      std::vector<int64_t> code(t->ws);
+      for (int64_t q = 0; q < t->ws; ++q) code[q] = -(q + 1);
+      for (size_t j = 0; j + 2 < c.size(); ++j) {
+        std::unordered_set<int64_t> seen;
+        for (int k = c[j]; k < c[j + 1]; ++k) {
+          const Instr& e = t->alg[k];
+          if ((e.op == OP_SIN || e.op == OP_COS) && seen.insert(code[e.i1]).second) rows += 2;
+          if (e.op != OP_OUTPUT) code[e.i0] = k;
+        }
+      }
+    }
     P.snap_rows = std::max(P.snap_rows, rows);
     P.n_sub[s] = static_cast<int>(c.size()) - 1;
   }
@@ -346,6 +368,15 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   char buf[256];
   // sin and cos of one value: one sincos call, its names by operand code
   std::unordered_map<int64_t, std::pair<std::string, std::string>> trig;
+  // With trig_io set, the sincos sites met by emit_fwd are stored to (1) or
+  // loaded from (2) rows of the sub-block state buffer through pointer
+  // trig_ptr: site i of sub-block trig_sub at rows trig_rows[trig_sub][i]
+  // and + 1.  Pass 1 stores them, the recomputation loads them.
+  int trig_io = 0, trig_sub = 0;
+  size_t trig_next = 0;
+  std::string trig_ptr;
+  std::vector<std::vector<int64_t>> trig_rows;
+  int64_t* trig_counter = nullptr;  // the next free row (pass 1)
   auto trig_pairs = [&](int a, int b, int pass) {
     std::vector<int64_t> sim = cur;
     std::unordered_map<int64_t, int> seen;
@@ -375,7 +406,20 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
         if (it == trig.end()) {
           const std::string id = std::to_string(code);
           it = trig.emplace(x, std::make_pair("ts" + id, "tc" + id)).first;
-          o += "  double ts" + id + ", tc" + id + ";\n  CXJ_SINCOS(" + V(x) + ", ts" + id + ", tc" + id + ");\n";
+          if (trig_io == 2) {
+            const int64_t r = trig_rows[trig_sub][trig_next++];
+            o += "  const double ts" + id + " = CXJ_SNAP(" + trig_ptr + ", " + std::to_string(r) + "), tc" + id +
+                 " = CXJ_SNAP(" + trig_ptr + ", " + std::to_string(r + 1) + ");\n";
+          } else {
+            o += "  double ts" + id + ", tc" + id + ";\n  CXJ_SINCOS(" + V(x) + ", ts" + id + ", tc" + id + ");\n";
+          }
+          if (trig_io == 1) {
+            const int64_t r = (*trig_counter)++;
+            ++*trig_counter;
+            trig_rows[trig_sub].push_back(r);
+            o += "  CXJ_SNAP(" + trig_ptr + ", " + std::to_string(r) + ") = ts" + id + ";\n  CXJ_SNAP(" + trig_ptr +
+                 ", " + std::to_string(r + 1) + ") = tc" + id + ";\n";
+          }
         }
         o += "  const double " + nm + " = " + (e.op == OP_SIN ? it->second.first : it->second.second) + ";\n";
         ox[k - t0] = oy[k - t0] = x;
@@ -432,8 +476,18 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   int64_t nrow = 0;
   std::string dropped;
   if (phase == 2) std::swap(o, dropped);
+  const bool tcache = P.trig_cache && sub_blocks;
+  trig_rows.assign(m, {});
+  trig_counter = &nrow;
   for (int j = 0; j + 1 < m; ++j) {
-    emit_fwd(c[j], c[j + 1], 0, true);
+    if (tcache) {  // one site per operand per sub-block, as the recomputation meets them
+      trig.clear();
+      trig_io = 1;
+      trig_sub = j;
+      trig_ptr = "A.st_out";
+    }
+    emit_fwd(c[j], c[j + 1], 0, true, tcache);
+    trig_io = 0;
     snap[j + 1] = cur;
     row[j + 1].assign(ws, -1);
     for (int slot : lc[j + 1]) {
@@ -442,6 +496,8 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
     }
   }
   if (phase == 2) std::swap(o, dropped);
+  if (nrow > P.snap_rows)  // the buffer holds P.snap_rows rows per instance
+    throw std::runtime_error("reverse kernel: sub-block state rows exceed the plan's buffer");
   if (phase == 1) {
     o += "}\n";
     return o;
@@ -484,8 +540,22 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
         cur[slot] = h;
       }
     }
+    const bool tload = tcache && !cont && j + 1 < m && !trig_rows[j].empty();
+    if (tload) {  // the sin/cos pairs pass 1 stored, through a pointer ordered like the state reload
+      const std::string tp = "cxj_tp" + std::to_string(j);
+      o += "  const double* " + tp + " = A.st_out;\n  CXJ_KEEP_PTR(" + tp + ");\n";
+      for (int slot : lc[j + 1]) {
+        const int64_t from = in_code[j + 1][slot];
+        if (decl.count(from)) o += "  CXJ_AFTER(" + tp + ", " + Adj(from) + ");\n";
+      }
+      trig_io = 2;
+      trig_sub = j;
+      trig_next = 0;
+      trig_ptr = tp;
+    }
     in_code[j] = cur;
     emit_fwd(c[j], c[j + 1], cont ? 0 : 1, last, true);
+    trig_io = 0;
     if (last) {
       if (s + 1 < P.n_seg)
         for (size_t r = 0; r < lout.size(); ++r) {
