@@ -84,14 +84,18 @@ def _compile(sources, workdir, groups=None):
     return libs
 
 
+@pytest.mark.parametrize("trig_rows", ["1", "0"])
 @pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("sub", ["0", "40"])
-def test_generated_segments_run_on_host(monkeypatch, sub, split):
+def test_generated_segments_run_on_host(monkeypatch, sub, split, trig_rows):
     """split: each reverse kernel as its two halves -- the first pass storing
-    the sub-block boundary states, then the rest reloading every sub-block."""
+    the sub-block boundary states, then the rest reloading every sub-block.
+    trig_rows: the first pass also stores its sin/cos pairs for the
+    recomputation to load (CXH_JIT_TRIG_CACHE)."""
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     monkeypatch.setenv("CXH_JIT_SUBBLOCK", sub)
+    monkeypatch.setenv("CXH_JIT_TRIG_CACHE", trig_rows)
     f = W.cartpole_rollout(4)
     every = 170
     tape = HipTape(f, engine="jit")
