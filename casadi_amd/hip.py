@@ -329,29 +329,29 @@ class HipTape:
             return Sparsity(n_out, n_in)
 
         def unit_words(n, nw):
-            u = np.zeros((n, nw), np.uint64)
-            r = np.arange(n)
-            u[r, r // 64] = np.left_shift(np.uint64(1), (r % 64).astype(np.uint64))
-            return torch.from_numpy(u.view(np.int64)).to("cuda")
+            """(n, nw) words with bit r % 64 of word r // 64 set in row r, built on the device."""
+            u = torch.zeros((n, nw), dtype=torch.int64, device="cuda")
+            r = torch.arange(n, device="cuda")
+            u[r, r // 64] = torch.bitwise_left_shift(torch.ones_like(r), r % 64)   # bit 63: the sign bit
+            return u
 
         def set_bits(words):
-            """(rows, nw) words -> (row, 64 s + b) of every set bit b of word s;
-            only the nonzero words are expanded."""
-            rr, ss = np.nonzero(words)
-            b = np.unpackbits(np.ascontiguousarray(words[rr, ss]).view(np.uint8).reshape(-1, 8), axis=1,
-                              bitorder="little")
+            """(rows, nw) device words -> (row, 64 s + b) of every set bit b of
+            word s; only the nonzero words leave the device."""
+            idx = torch.nonzero(words)
+            vals = words[idx[:, 0], idx[:, 1]].cpu().numpy()
+            idx = idx.cpu().numpy()
+            b = np.unpackbits(np.ascontiguousarray(vals).view(np.uint8).reshape(-1, 8), axis=1, bitorder="little")
             i, bit = np.nonzero(b)
-            return rr[i], ss[i] * 64 + bit
+            return idx[i, 0], idx[i, 1] * 64 + bit
         if w_fwd <= w_adj:
             seeds = [None] * len(self.nnz_in)
             seeds[iind] = unit_words(n_in, w_fwd)
-            out = self.sp_forward(seeds, w_fwd)[oind]
-            r, c = set_bits(out.cpu().numpy())      # output nonzero, input nonzero
+            r, c = set_bits(self.sp_forward(seeds, w_fwd)[oind])      # output nonzero, input nonzero
         else:
             seeds = [None] * len(self.nnz_out)
             seeds[oind] = unit_words(n_out, w_adj)
-            sens = self.sp_reverse(seeds, w_adj)[iind]
-            c, r = set_bits(sens.cpu().numpy())     # input nonzero, output nonzero
+            c, r = set_bits(self.sp_reverse(seeds, w_adj)[iind])      # input nonzero, output nonzero
         return Sparsity.triplet(n_out, n_in, r, c)
 
     def ckpt_size(self, n, checkpoint_every):
