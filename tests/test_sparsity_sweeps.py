@@ -130,3 +130,19 @@ def test_random_words_forward_is_or_over_dependencies(ca):
                 for c in np.nonzero(P[r])[0]:
                     want[r] |= seeds[i][c]
         assert np.array_equal(out[o], want)
+
+
+def test_device_sweep_arguments_are_checked(ca):
+    """HipTape.sp_forward / sp_reverse take int64 CUDA tensors of 64-bit words,
+    (nnz, >= n_words) per block; anything else is refused before a launch."""
+    import torch
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    t = HipTape(W.cartpole_step())
+    with pytest.raises(TypeError):
+        t.sp_forward([torch.zeros((4, 2), dtype=torch.int64), None], 2)        # host tensor
+    with pytest.raises(TypeError):
+        t.sp_reverse([np.zeros((4, 2), np.int64)], 2)                           # not a tensor
+    if torch.cuda.is_available():
+        with pytest.raises(ValueError):
+            t.sp_forward([torch.zeros((3, 2), dtype=torch.int64, device="cuda"), None], 2)
