@@ -171,7 +171,7 @@ def main():
     # CPU baseline: CasADi's path on the host -- map serial of f, then of its
     # reverse(1) Function -- on the C oracle VM, one thread, over the batch's
     # first instances repeated until about 10 s of work
-    cpu = None
+    cpu, cpu_omp = None, None
     if not args.no_cpu:
         mc = min(args.cpu_sample, n)
         a_f = [x0[:, :mc].T.copy(), us[:, :mc].T.copy()]
@@ -187,6 +187,28 @@ def main():
                "sample": f"{reps} x {mc} instances: the rollout f ({st_f['n_instr']} instructions) then its "
                          f"reverse(1) Function ({st_r['n_instr']} instructions) on the C oracle VM, map serial, "
                          f"1 thread, {dt:.1f} s"}
+        # the same map over the host's cores (map openmp): OpenMP threads over
+        # instances, as many as the process may use (OMP_NUM_THREADS caps it)
+        nt = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            nt = min(nt, int(os.environ["OMP_NUM_THREADS"]))
+        reps_p, dt_p = 0, 0.0
+        while dt_p < 3.0 and reps_p < 200:
+            t0 = time.perf_counter()
+            O.map_eval(f, mc, a_f, n_threads=nt)
+            O.map_eval(R, mc, a_r, n_threads=nt)
+            dt_p += time.perf_counter() - t0
+            reps_p += 1
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as fh:
+                model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+        except OSError:
+            pass
+        cpu_omp = {"value": reps_p * mc / dt_p, "unit": "instances/s", "cores": nt, "kind": "port",
+                   "cpu_model": model, "nproc": os.cpu_count(),
+                   "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+                   "sample": f"{reps_p} x {mc} instances, same two maps, map openmp, {nt} threads, {dt_p:.1f} s"}
 
     line = {
         "metric": "instances/sec fwd+adjoint, fused batched vs plain GPU tape @1e-13 rel err",
@@ -213,11 +235,13 @@ def main():
         "fused_mapaccum": {"value": n / (ms_macc * 1e-3), "ms_per_step": ms_macc,
                            "speedup_vs_plain": ms_plain / ms_macc,
                            "err_grad_median": float(np.median(per_m)), "err_grad_max": float(per_m.max()),
+                           "err_grad_p99": float(np.percentile(per_m, 99)),
                            "note": "same cost and gradient as mapaccum(100) of the cost-carrying RK4 step: "
                                    "loop kernel around one specialised step, checkpoints every 10 steps"},
         "jit_compile_s": compile_s,
         "err_grad_median": float(np.median(per)),
         "err_grad_max": float(per.max()),
+        "err_grad_p99": float(np.percentile(per, 99)),
         "err_cost": err_J,
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
@@ -229,6 +253,7 @@ def main():
                      "hbm_algorithmic_gbs": bytes_per_inst * n / (ms_fused * 1e-3) / 1e9,
                      "note": "FP64 vector peak; algorithmic tape ops, transcendentals counted once"},
         "cpu_baseline": cpu,
+        "cpu_baseline_openmp": cpu_omp,
     }
     print(json.dumps(line))
 
