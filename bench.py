@@ -28,7 +28,6 @@ N_STEPS = 100
 CKPT_STEPS = 10
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-BINARY = {1, 2, 3, 4, 8, 9, 19, 20, 21, 22, 24, 25, 28, 31, 32, 34, 35, 43, 87, 88, 95, 97}
 
 
 def parse():
@@ -46,11 +45,11 @@ def tape_stats(f, every):
     """Instruction counts and the algorithmic FP64 operations of the reverse
     segment kernels: per arithmetic instruction 1 (recompute) + per operand 1
     (partial) + 2 (multiply and accumulate the adjoint)."""
-    from casadi_amd.ops import OP
+    from casadi_amd.ops import BINARY, OP
     a = f.instructions()
     op = a["op"]
     arith = (op != OP.CONST) & (op != OP.INPUT) & (op != OP.OUTPUT)
-    nops = np.where(np.isin(op, list(BINARY)), 2, 1)
+    nops = np.where(np.isin(op, [int(b) for b in BINARY]), 2, 1)
     rev_flops = int(np.sum(np.where(arith, 1 + 3 * nops, 0)))
     n_seg = (a.size + every - 1) // every
     return dict(n_instr=int(a.size), n_arith=int(arith.sum()), worksize=f.sz_w(), rev_flops=rev_flops,
