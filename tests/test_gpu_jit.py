@@ -208,3 +208,34 @@ def test_jit_split_reverse_ragged_segments(ca, cuda, monkeypatch, every):
     assert np.array_equal(got["1"][0], Ji.cpu().numpy())
     for x, y in zip(got["1"][1:3], _np(gi)):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("ipt", [2, 3, 8])
+def test_jit_instances_per_thread(ca, cuda, monkeypatch, ipt):
+    """CXH_JIT_IPT: each thread of the forward segment kernels takes `ipt`
+    instances, one block width apart.  On a ragged batch (1000 = 15 blocks of
+    64 + 40) the forward evaluation, the fused adjoint and the split
+    checkpointed sweep give the bits of one instance per thread."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(20)
+    n = 1000
+    every = (f.n_instructions() + 3) // 4
+    rng = np.random.default_rng(seed_of(2) + ipt)
+    x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (20, n))
+    seed = rng.standard_normal((1, n))
+    got = {}
+    for k in ("1", str(ipt)):
+        monkeypatch.setenv("CXH_JIT_IPT", k)
+        print(f"[case] jit rollout20 n={n} every={every} ipt={k}", flush=True)
+        tj = HipTape(f, engine="jit")
+        (Je,) = _np(tj.eval(_dev(x0, us)))
+        (Jj,), gj = tj.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
+        outs, ck = tj.fwd_ckpt(_dev(x0, us), n, every)
+        g2 = tj.rev_ckpt(_dev(x0, us), _dev(seed), n, every, ck)
+        step = HipTape(W.cartpole_step(), engine="jit")
+        (ys,) = _np(step.eval(_dev(x0, us[:1])))
+        got[k] = [Je, Jj.cpu().numpy(), outs[0].cpu().numpy(), ys] + _np(gj) + _np(g2)
+    assert all(np.array_equal(a, b) for a, b in zip(got["1"], got[str(ipt)]))
+    (Jo,) = O.map_eval(f, n, [x0.T, us.T])
+    assert O.rel_err(got[str(ipt)][0], Jo.T) <= TOL
