@@ -239,3 +239,37 @@ def test_jit_instances_per_thread(ca, cuda, monkeypatch, ipt):
     assert all(np.array_equal(a, b) for a, b in zip(got["1"], got[str(ipt)]))
     (Jo,) = O.map_eval(f, n, [x0.T, us.T])
     assert O.rel_err(got[str(ipt)][0], Jo.T) <= TOL
+
+
+@pytest.mark.parametrize("setting", [("CXH_JIT_SUBBLOCK", "128"), ("CXH_JIT_TRIG_CACHE", "0"),
+                                     ("CXH_JIT_TRIG_CALLS", "0"), ("CXH_JIT_TRIG_CALLS", "-1")])
+def test_jit_settings_keep_the_bits(ca, cuda, monkeypatch, setting):
+    """The code-shape settings of the reverse kernels change neither the
+    operations nor their order: sub-blocks of 128 instructions instead of 256,
+    sin/cos recomputed instead of loaded from the first pass, and sin/cos as
+    calls at every site or inline at every site (the programs compile with
+    -ffp-contract=off) give the default's bits, fused and split."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(20)
+    n = 1000
+    every = (f.n_instructions() + 1) // 2       # 2 segments of ~1380 instructions, ~40 sin/cos sites each
+    rng = np.random.default_rng(seed_of(3))
+    x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (20, n))
+    seed = rng.standard_normal((1, n))
+    got = []
+    for env in (None, setting):
+        if env:
+            monkeypatch.setenv(*env)
+        print(f"[case] jit rollout20 n={n} every={every} setting={env}", flush=True)
+        tj = HipTape(f, engine="jit")
+        (Jj,), gj = tj.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
+        outs, ck = tj.fwd_ckpt(_dev(x0, us), n, every)
+        g2 = tj.rev_ckpt(_dev(x0, us), _dev(seed), n, every, ck)
+        got.append([Jj.cpu().numpy()] + _np(gj) + _np(g2))
+    assert all(np.array_equal(a, b) for a, b in zip(*got))
+    (Jo,), (ox, ou) = O.adj_map(f, n, [x0.T, us.T], [seed.T])
+    g = np.concatenate(got[1][1:3]).T
+    go = np.concatenate([ox, ou], axis=1)
+    scale = np.maximum(np.max(np.abs(go), axis=1, keepdims=True), 1.0)
+    assert np.median(np.max(np.abs(g - go) / scale, axis=1)) <= TOL
