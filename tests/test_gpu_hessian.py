@@ -80,3 +80,34 @@ def test_quadrotor_hessian_specialised_kernels(cuda, n):
     (gj,), hj = HipJacobian(G, symmetric=True, first_block=True, engine="jit")(args, n)
     assert torch.equal(gi, gj)
     assert torch.equal(hi, hj)
+
+
+@pytest.mark.parametrize("settings", [
+    {"CXH_JIT_DIRGROUP": "4", "CXH_JIT_TANSEG": "400", "CXH_JIT_TANREGS": "24"},
+    {"CXH_JIT_DIRGROUP": "1", "CXH_JIT_PREFETCH": "0"}])
+def test_tangent_settings_keep_the_bits(cuda, monkeypatch, settings):
+    """Direction groups, segment length, register budget (24 values: spills
+    and reloads) and reload lookahead change where values live, not the
+    operations: the Hessian of a 2-step quadrotor OCP Lagrangian (several
+    segments and groups) equals the interpreter's bit for bit, on a ragged
+    batch."""
+    import torch
+    import casadi_amd as ca
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipJacobian
+    w, lam, Lag = W.quadrotor_ocp_lagrangian(N=2)
+    G = ca.Function("grad_lag", [w, lam], [ca.gradient(Lag, w)])
+    n = 1001
+    rng = np.random.default_rng(seed_of(1) + 7)
+    wv, lv = rng.uniform(-1, 1, (w.nnz(), n)), rng.standard_normal((lam.nnz(), n))
+    args = [torch.tensor(a, dtype=torch.float64, device="cuda") for a in (wv, lv)]
+    (gi,), hi = HipJacobian(G, symmetric=True, first_block=True)(args, n)
+    for k, v in settings.items():
+        monkeypatch.setenv(k, v)
+    print(f"[case] quadrotor N=2 hessian specialised n={n} settings={settings}", flush=True)
+    hh = HipJacobian(G, symmetric=True, first_block=True, engine="jit")
+    _, ng, ns = hh.jit_source(0)
+    assert ng >= 2 and (ns > 1 or "CXH_JIT_TANSEG" not in settings)
+    (gj,), hj = hh(args, n)
+    assert torch.equal(gi, gj)
+    assert torch.equal(hi, hj)
