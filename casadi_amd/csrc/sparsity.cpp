@@ -133,6 +133,39 @@ Sparsity Sparsity::mtimes(const Sparsity& B) const {
   return s;
 }
 
+Sparsity Sparsity::intersect(const Sparsity& B) const {
+  if (nrow != B.nrow || ncol != B.ncol) throw std::runtime_error("intersect: dimension mismatch");
+  Sparsity s(nrow, ncol);
+  for (int64_t c = 0; c < ncol; ++c) {
+    // both columns are sorted by row: merge and keep the common rows
+    int64_t ka = colind[c], kb = B.colind[c];
+    while (ka < colind[c + 1] && kb < B.colind[c + 1]) {
+      if (row[ka] < B.row[kb]) {
+        ++ka;
+      } else if (B.row[kb] < row[ka]) {
+        ++kb;
+      } else {
+        s.row.push_back(row[ka]);
+        ++ka;
+        ++kb;
+      }
+    }
+    s.colind[c + 1] = static_cast<int64_t>(s.row.size());
+  }
+  return s;
+}
+
+Sparsity Sparsity::enlarge(int64_t nr, int64_t nc, const ivec& rr, const ivec& cc) const {
+  if (static_cast<int64_t>(rr.size()) != nrow || static_cast<int64_t>(cc.size()) != ncol)
+    throw std::runtime_error("enlarge: index vectors do not match the pattern");
+  ivec r(nnz()), c(nnz()), col = get_col();
+  for (int64_t k = 0; k < nnz(); ++k) {
+    r[k] = rr[row[k]];
+    c[k] = cc[col[k]];
+  }
+  return triplet(nr, nc, r, c);
+}
+
 Sparsity Sparsity::sub(const ivec& rr, const ivec& cc, ivec* mapping) const {
   // rows of this -> list of positions in rr
   std::vector<ivec> rmap(nrow);

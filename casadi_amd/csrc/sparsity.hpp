@@ -49,6 +49,12 @@ struct Sparsity {
   bool operator!=(const Sparsity& o) const { return !(*this == o); }
   // Pattern of the product this * B
   Sparsity mtimes(const Sparsity& B) const;
+  // Entries present in both patterns (Sparsity::intersect, the pattern
+  // operator* of casadi/core/sparsity.cpp:462-465; same dimensions required)
+  Sparsity intersect(const Sparsity& B) const;
+  // Insert empty rows / columns: row k of this becomes row rr[k] of an
+  // nr-row pattern (enlargeRows), column k becomes column cc[k] (enlargeColumns)
+  Sparsity enlarge(int64_t nr, int64_t nc, const ivec& rr, const ivec& cc) const;
   // A(rr, cc) for index vectors rr, cc (CasADi Sparsity::sub); mapping gives
   // the nonzero of this for every nonzero of the result.
   Sparsity sub(const ivec& rr, const ivec& cc, ivec* mapping = nullptr) const;

@@ -505,16 +505,19 @@ Sparsity SXFunction::jac_sparsity(size_t oind, size_t iind, bool compact, bool s
     }
     jsp_cache_[1][key] = jsp;
   }
-  if (symmetric && !jsp.is_symmetric()) jsp = jsp.mtimes(jsp.T());
-  if (compact) return jsp;
   // from_compact: enlarge rows/columns to the full dimensions
+  // (FunctionInternal::from_compact, casadi/core/function_internal.cpp:1985-2001)
   ivec fo = find_nz(out_[oind].sp), fi = find_nz(in_[iind].sp);
-  ivec col = jsp.get_col(), rr(jsp.nnz()), cc(jsp.nnz());
-  for (int64_t k = 0; k < jsp.nnz(); ++k) {
-    rr[k] = fo[jsp.row[k]];
-    cc[k] = fi[col[k]];
+  Sparsity full = jsp.enlarge(out_[oind].sp.numel(), in_[iind].sp.numel(), fo, fi);
+  // A symmetric Jacobian keeps only the entries present in both J and J^T, in
+  // the full space (the pattern operator* is an intersection;
+  // casadi/core/function_internal.cpp:2051-2062).  A dependency that an
+  // operation such as floor or sign drops in one direction only is removed.
+  if (symmetric && !full.is_symmetric()) {
+    full = full.intersect(full.T());
+    if (compact) jsp = full.sub(fo, fi);  // to_compact (function_internal.cpp:1978-1983)
   }
-  return Sparsity::triplet(out_[oind].sp.numel(), in_[iind].sp.numel(), rr, cc);
+  return compact ? jsp : full;
 }
 
 void SXFunction::get_partition(size_t iind, size_t oind, Sparsity& D1, Sparsity& D2, bool compact,

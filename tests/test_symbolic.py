@@ -193,3 +193,41 @@ def test_symbolic_call_inlines():
     g = ca.Function("g", [x, u], [y])
     x0 = np.array([0.1, 0.2, 0.3, 0.4])
     assert np.array_equal(g(x0, 1.0), f(f(x0, 1.0), 1.0))
+
+
+def test_hessian_pattern_drops_one_sided_dependencies():
+    # The gradient of L = x0*x1 + x0*floor(x2) is [x1 + floor(x2), x0, 0]: its
+    # first entry depends on x2 structurally (floor passes the dependency on),
+    # but the derivative with respect to x2 carries no x0 term.  The Jacobian
+    # of the gradient therefore has (0,2) and not (2,0); the symmetric
+    # pattern is the intersection with the transpose
+    # (casadi/core/function_internal.cpp:2051-2062), here {(0,1), (1,0)}.
+    x = ca.SX.sym("x", 3)
+    L = x[0] * x[1] + x[0] * ca.floor(x[2])
+    _, g = ca.hessian(L, x)
+    J = ca.jacobian(g, x)
+    assert J.sparsity.dense_mask()[0, 2] and not J.sparsity.dense_mask()[2, 0]
+    H, _ = ca.hessian(L, x)
+    assert np.array_equal(H.sparsity.dense_mask(), np.array([[0, 1, 0], [1, 0, 0], [0, 0, 0]], bool))
+    fH = ca.Function("h", [x], [H])
+    for x0 in (np.array([0.3, 0.7, 1.5]), np.array([-2.0, 0.1, -0.4])):
+        assert np.array_equal(fH(x0), np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 0.0]]))
+    # A second case with sign and fmin: L = x0*x1*x2 + x1*sign(x0) + fmin(x2, 1)*x0
+    L2 = x[0] * x[1] * x[2] + x[1] * ca.sign(x[0]) + ca.fmin(x[2], 1.0) * x[0]
+    H2, _ = ca.hessian(L2, x)
+    assert H2.sparsity.is_symmetric()
+    a, b, c = 0.3, -0.6, 0.4          # fmin(c, 1) = c: its partial is 1
+    ref = np.array([[0.0, c, b + 1.0], [c, 0.0, a], [b + 1.0, a, 0.0]])
+    assert np.array_equal(ca.Function("h2", [x], [H2])(np.array([a, b, c])), ref)
+
+
+def test_symmetric_jac_sparsity_full_and_compact():
+    # the intersection is taken in the full space; the compact pattern is the
+    # full one stripped back to the nonzeros (the same here: dense blocks)
+    x = ca.SX.sym("x", 3)
+    L = x[0] * x[1] + x[0] * ca.floor(x[2]) + ca.sin(x[2]) * x[2]
+    f = ca.Function("g", [x], [ca.gradient(L, x)])
+    full = f.jac_sparsity(0, 0, compact=False, symmetric=True)
+    assert np.array_equal(full.dense_mask(), np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]], bool))
+    compact = f.jac_sparsity(0, 0, compact=True, symmetric=True)
+    assert compact == full
