@@ -83,6 +83,7 @@ cxh_tape_engine = _fn("cxh_tape_engine", I32, P, I32)
 cxh_tape_jit_prepare = _fn("cxh_tape_jit_prepare", I32, P, I64, I32, I32)
 cxh_tape_jit_source = _fn("cxh_tape_jit_source", I32, P, I64, I32, I32, C.c_char_p, I64, PI64,
                           C.POINTER(I32))
+cxh_tape_jit_plan = _fn("cxh_tape_jit_plan", I32, P, I64, PI64, PI64)
 cxh_jit_compile_check = _fn("cxh_jit_compile_check", I32, CSTR, CSTR)
 cxh_jac_jit_source = _fn("cxh_jac_jit_source", I32, P, I32, C.c_char_p, I64, PI64, C.POINTER(I32),
                          C.POINTER(I32))

@@ -761,20 +761,24 @@ static hipFunction_t split_kernel(cxh_tape_s* t, JitPlan& P, int s, int phase) {
 // Segment g's reverse kernel after its first pass (blockIdx.y = 0, arguments
 // A) beside the first pass of segment g-1 (blockIdx.y = 1, arguments B): the
 // two halves are independent, and one launch runs them on twice the SIMDs.
+static std::string pair_source(const cxh_tape_s* t, const JitPlan& P, int g) {
+  std::string o = kSxOpsSource;
+  o += kPrologue;
+  o += gen_body(t, P, g, true, "cxj_back_body", true, 2);
+  o += gen_body(t, P, g - 1, true, "cxj_snap_body", true, 1);
+  o += "extern \"C\" __global__ void __launch_bounds__(256) cxj_rev(const CxjArgs A, const CxjArgs B) {\n"
+       "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+       "  if (k >= A.n) return;\n"
+       "  if (blockIdx.y == 0) cxj_back_body(A, k);\n"
+       "  else cxj_snap_body(B, k);\n"
+       "}\n";
+  return finish_source(std::move(o));
+}
+
 static hipFunction_t pair_kernel(cxh_tape_s* t, JitPlan& P, int g) {
   hipFunction_t& f = P.pair[g];
   if (!f) {
-    std::string o = kSxOpsSource;
-    o += kPrologue;
-    o += gen_body(t, P, g, true, "cxj_back_body", true, 2);
-    o += gen_body(t, P, g - 1, true, "cxj_snap_body", true, 1);
-    o += "extern \"C\" __global__ void __launch_bounds__(256) cxj_rev(const CxjArgs A, const CxjArgs B) {\n"
-         "  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-         "  if (k >= A.n) return;\n"
-         "  if (blockIdx.y == 0) cxj_back_body(A, k);\n"
-         "  else cxj_snap_body(B, k);\n"
-         "}\n";
-    f = jit_function_checked(finish_source(std::move(o)), "cxj_rev",
+    f = jit_function_checked(pair_source(t, P, g), "cxj_rev",
                              "use sub-blocks of at most 256 instructions (CXH_JIT_SUBBLOCK), a shorter checkpoint "
                              "interval or the interpreter engine");
   }
@@ -941,8 +945,24 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
 std::string jit_source(cxh_tape_s* t, int64_t every, int seg, int rev) {
   JitPlan& P = plan_for(t, every);
   if (seg < 0 || seg >= P.n_seg) throw std::runtime_error("segment index out of range");
-  if (rev < 0 || rev > 3) throw std::runtime_error("reverse must be 0, 1, 2 or 3");
+  if (rev < 0 || rev > 4) throw std::runtime_error("reverse must be 0, 1, 2, 3 or 4");
+  if (rev == 4) {
+    if (seg < 1) throw std::runtime_error("a pair kernel pairs segment g >= 1 with g-1");
+    return pair_source(t, P, seg);
+  }
   return gen_segment(t, P, seg, rev != 0, rev >= 2 ? rev - 1 : 0);
+}
+
+void jit_plan_info(cxh_tape_s* t, int64_t every, int64_t* info, int64_t* ck_off) {
+  JitPlan& P = plan_for(t, every);
+  info[0] = P.n_seg;
+  info[1] = P.ck_rows;
+  info[2] = P.snap_rows;
+  info[3] = P.max_live;
+  info[4] = split_plan(P) ? 1 : 0;
+  info[5] = P.ipt;
+  if (ck_off)
+    for (int s = 0; s <= P.n_seg; ++s) ck_off[s] = P.ck_off[s];
 }
 
 // Operand codes (instruction indices) of the values whose sin and cos are
@@ -2811,6 +2831,15 @@ int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, i
     std::memcpy(buf, src.data(), m);
     buf[m] = '\0';
   }
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_tape_jit_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info, int64_t* ck_off) {
+  CXH_TRY
+  if (!t || !info) return cxh_fail("null argument");
+  if (t->alg.empty()) return cxh_fail("empty tape");
+  jit_plan_info(t, checkpoint_every, info, ck_off);
   return 0;
   CXH_CATCH
 }

@@ -116,6 +116,15 @@ int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, 
  * storing the sub-block boundary states, 3: the rest).                      */
 int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, int32_t reverse,
                         char* buf, int64_t cap, int64_t* len, int32_t* n_segments);
+/* Introspection: the plan of a checkpoint interval as the drivers size their
+ * buffers from it.  info[6] = {segments, checkpoint rows, sub-block state rows
+ * of one reverse kernel, largest live set at a segment boundary, whether the
+ * reverse sweep runs split (1) or one kernel per segment (0), instances per
+ * thread of the forward kernels}; ck_off (may be NULL; segments + 1 entries):
+ * first checkpoint row of each boundary.  reverse = 4 in cxh_tape_jit_source
+ * gives the source of the launch pairing segment `segment`'s reverse half
+ * with the first pass of the segment before it.                            */
+int cxh_tape_jit_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info, int64_t* ck_off);
 /* Introspection: generated source of the specialised tangent kernel of one
  * segment of a Jacobian plan (the bodies of all direction groups, selected by
  * blockIdx.y), with the group / segment counts.  cxh_map_jac runs these

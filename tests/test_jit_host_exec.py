@@ -22,6 +22,12 @@ import pytest
 import oracle_ref as O
 
 MAX_IO = 16
+# g++ turns sin(x) and cos(x) of one argument into one glibc sincos call,
+# whose results differ from separate sin and cos in the last bit for some
+# arguments (the 12-step rollout's instance 9 of seed 4: cos differs by one
+# ulp).  The oracle calls sin and cos separately, so the host builds of the
+# generated code must too.
+NO_SINCOS_FUSION = ("-fno-builtin-sin", "-fno-builtin-cos")
 
 
 class IOArg(C.Structure):
@@ -74,7 +80,7 @@ def _compile(sources, workdir, groups=None):
         with open(path, "w") as fh:
             fh.write(code)
         so = os.path.join(workdir, f"seg{i}.so")
-        subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w", path, "-o", so],
+        subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", *NO_SINCOS_FUSION, "-shared", "-fPIC", "-w", path, "-o", so],
                        check=True)
         lib = C.CDLL(so)
         fn = getattr(lib, f"run_{i}")
@@ -286,7 +292,7 @@ def test_mapaccum_loop_kernels_run_on_host(K):
         path, so = os.path.join(d, "pair.cpp"), os.path.join(d, "pair.so")
         with open(path, "w") as fh:
             fh.write(code)
-        subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w", path, "-o", so],
+        subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", *NO_SINCOS_FUSION, "-shared", "-fPIC", "-w", path, "-o", so],
                        check=True)
         run = C.CDLL(so).run_pair
         run.argtypes = [C.POINTER(CxjArgs), C.POINTER(CxjArgs), C.c_longlong, C.c_longlong]
