@@ -106,6 +106,14 @@ cxh_qr_jit_source = _opt("cxh_qr_jit_source", I32, P, I32, I32, I32, C.c_char_p,
 cxh_qr_create = _opt("cxh_qr_create", I32, PI64, PI64, PI64, PI64, PI64, D, PP)
 cxh_qr_destroy = _opt("cxh_qr_destroy", I32, P)
 cxh_qr_factor_solve = _opt("cxh_qr_factor_solve", I32, P, I64, P, P, I32, I32, P, I32, P)
+cxh_ldl_nfact = _opt("cxh_ldl_nfact", I32, P, I64, P, P, I32, P)
+cxh_ldl_solve = _opt("cxh_ldl_solve", I32, P, I64, P, I32, I32, I32, P)
+cxh_ldl_neig = _opt("cxh_ldl_neig", I32, P, P, P)
+cxh_ldl_rank = _opt("cxh_ldl_rank", I32, P, P, P)
+cxh_qr_nfact = _opt("cxh_qr_nfact", I32, P, I64, P, P, I32, P)
+cxh_qr_solve = _opt("cxh_qr_solve", I32, P, I64, P, I32, I32, I32, P)
+cxh_qr_det = _opt("cxh_qr_det", I32, P, P, P)
+cxh_qr_cache = _opt("cxh_qr_cache", I32, P, I32)
 cxh_batch_create = _opt("cxh_batch_create", I32, I32, PP, PI64, PP)
 cxh_batch_bind = _opt("cxh_batch_bind", I32, P, I32, PP, PP, PP, PP, I32, I64)
 cxh_batch_config = _opt("cxh_batch_config", I32, P, I32, I32, I32)

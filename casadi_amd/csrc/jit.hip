@@ -1795,6 +1795,22 @@ struct TapeEmit {
   }
 };
 
+// Offsets of the input / output blocks in the flattened nonzero numbering.
+void set_io_offsets(LinsolJit& L) {
+  L.jac.in_off.clear();
+  L.jac.out_off.clear();
+  int acc = 0;
+  for (int64_t v : L.tape.nnz_in) {
+    L.jac.in_off.push_back(acc);
+    acc += static_cast<int>(v);
+  }
+  acc = 0;
+  for (int64_t v : L.tape.nnz_out) {
+    L.jac.out_off.push_back(acc);
+    acc += static_cast<int>(v);
+  }
+}
+
 __global__ void count_to_status(const double* __restrict__ cnt, int64_t n, int32_t* __restrict__ status) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k < n) status[k] = static_cast<int32_t>(cnt[k]);
@@ -1804,77 +1820,99 @@ __global__ void count_to_status(const double* __restrict__ cnt, int64_t n, int32
 
 std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
                                        const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
-                                       const std::vector<int>& p, int nrhs) {
+                                       const std::vector<int>& p, int nrhs, int part) {
   TapeEmit e;
   const int Z = e.konst(0.0);
   const int nnz_a = a_colind[n], nnz_lt = lt_colind[n];
   std::vector<int> w(n, Z), lt(nnz_lt, Z), d(n, Z);
-  // scatter the permuted A into L^T and D
-  for (int c = 0; c < n; ++c) {
-    const int c1 = p[c];
-    for (int k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = e.input(0, k);
-    for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) lt[k] = w[p[lt_row[k]]];
-    d[c] = w[p[c]];
-    for (int k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = Z;
-  }
-  // up-looking factorisation, column by column
-  for (int c = 0; c < n; ++c) {
-    int dc = d[c];
-    for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) {
-      const int r = lt_row[k];
-      int v = lt[k];
-      for (int k2 = lt_colind[r]; k2 < lt_colind[r + 1]; ++k2) v = e.op(OP_SUB, v, e.op(OP_MUL, lt[k2], w[lt_row[k2]]));
-      w[r] = v;
-      v = e.op(OP_DIV, v, d[r]);
-      lt[k] = v;
-      dc = e.op(OP_SUB, dc, e.op(OP_MUL, w[r], v));
-    }
-    d[c] = dc;
-    for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) w[lt_row[k]] = Z;
-  }
-  // zero or non-finite pivots: !(d != 0 && d - d == 0)
-  int cnt = Z;
-  for (int c = 0; c < n; ++c) {
-    const int nz = e.op(OP_NE, d[c], Z);
-    const int fin = e.op(OP_EQ, e.op(OP_SUB, d[c], d[c]), Z);
-    const int bad = e.un(OP_NOT, e.op(OP_AND, nz, fin));
-    cnt = e.op(OP_ADD, cnt, bad);
-  }
-  e.output(1, 0, cnt);
-  // x = P' L'^-1 D^-1 L^-1 P b for every right-hand side
-  for (int q = 0; q < nrhs; ++q) {
-    for (int i = 0; i < n; ++i) w[i] = e.input(1, q * n + p[i]);
+  if (part != LS_SOLVE) {
+    // scatter the permuted A into L^T and D
     for (int c = 0; c < n; ++c) {
-      int v = w[c];
-      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) v = e.op(OP_SUB, v, e.op(OP_MUL, lt[k], w[lt_row[k]]));
-      w[c] = v;
+      const int c1 = p[c];
+      for (int k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = e.input(0, k);
+      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) lt[k] = w[p[lt_row[k]]];
+      d[c] = w[p[c]];
+      for (int k = a_colind[c1]; k < a_colind[c1 + 1]; ++k) w[a_row[k]] = Z;
     }
-    for (int i = 0; i < n; ++i) w[i] = e.op(OP_DIV, w[i], d[i]);
-    for (int c = n - 1; c >= 0; --c) {
-      const int wc = w[c];
-      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k)
-        w[lt_row[k]] = e.op(OP_SUB, w[lt_row[k]], e.op(OP_MUL, wc, lt[k]));
+    // up-looking factorisation, column by column
+    for (int c = 0; c < n; ++c) {
+      int dc = d[c];
+      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) {
+        const int r = lt_row[k];
+        int v = lt[k];
+        for (int k2 = lt_colind[r]; k2 < lt_colind[r + 1]; ++k2)
+          v = e.op(OP_SUB, v, e.op(OP_MUL, lt[k2], w[lt_row[k2]]));
+        w[r] = v;
+        v = e.op(OP_DIV, v, d[r]);
+        lt[k] = v;
+        dc = e.op(OP_SUB, dc, e.op(OP_MUL, w[r], v));
+      }
+      d[c] = dc;
+      for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) w[lt_row[k]] = Z;
     }
-    for (int i = 0; i < n; ++i) e.output(0, q * n + p[i], w[i]);
+    // zero or non-finite pivots: !(d != 0 && d - d == 0)
+    int cnt = Z;
+    for (int c = 0; c < n; ++c) {
+      const int nz = e.op(OP_NE, d[c], Z);
+      const int fin = e.op(OP_EQ, e.op(OP_SUB, d[c], d[c]), Z);
+      const int bad = e.un(OP_NOT, e.op(OP_AND, nz, fin));
+      cnt = e.op(OP_ADD, cnt, bad);
+    }
+    if (part == LS_FACTOR) {  // L^T and D leave the program: the solve program reads them back
+      for (int k = 0; k < nnz_lt; ++k) e.output(0, k, lt[k]);
+      for (int c = 0; c < n; ++c) e.output(1, c, d[c]);
+      e.output(2, 0, cnt);
+    } else {
+      e.output(1, 0, cnt);
+    }
+  } else {
+    for (int k = 0; k < nnz_lt; ++k) lt[k] = e.input(0, k);
+    for (int c = 0; c < n; ++c) d[c] = e.input(1, c);
   }
+  // x = P' L'^-1 D^-1 L^-1 P b for every right-hand side
+  const int rhs_blk = part == LS_SOLVE ? 2 : 1;
+  if (part != LS_FACTOR)
+    for (int q = 0; q < nrhs; ++q) {
+      for (int i = 0; i < n; ++i) w[i] = e.input(rhs_blk, q * n + p[i]);
+      for (int c = 0; c < n; ++c) {
+        int v = w[c];
+        for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k) v = e.op(OP_SUB, v, e.op(OP_MUL, lt[k], w[lt_row[k]]));
+        w[c] = v;
+      }
+      for (int i = 0; i < n; ++i) w[i] = e.op(OP_DIV, w[i], d[i]);
+      for (int c = n - 1; c >= 0; --c) {
+        const int wc = w[c];
+        for (int k = lt_colind[c]; k < lt_colind[c + 1]; ++k)
+          w[lt_row[k]] = e.op(OP_SUB, w[lt_row[k]], e.op(OP_MUL, wc, lt[k]));
+      }
+      for (int i = 0; i < n; ++i) e.output(0, q * n + p[i], w[i]);
+    }
   auto L = std::make_shared<LinsolJit>();
   L->nrhs = nrhs;
   L->tape.alg = std::move(e.alg);
   L->tape.ws = e.ws;
-  L->tape.nnz_in = {nnz_a, static_cast<int64_t>(n) * nrhs};
-  L->tape.nnz_out = {static_cast<int64_t>(n) * nrhs, 1};
+  const int64_t nx = static_cast<int64_t>(n) * nrhs;
+  if (part == LS_FUSED) {
+    L->tape.nnz_in = {nnz_a, nx};
+    L->tape.nnz_out = {nx, 1};
+  } else if (part == LS_FACTOR) {
+    L->tape.nnz_in = {nnz_a};
+    L->tape.nnz_out = {nnz_lt, n, 1};
+  } else {
+    L->tape.nnz_in = {nnz_lt, n, nx};
+    L->tape.nnz_out = {nx};
+  }
   L->tape.engine = CXH_ENGINE_JIT;
   L->jac.tape = &L->tape;
   L->jac.ndir = 0;
-  L->jac.in_off = {0, nnz_a};
-  L->jac.out_off = {0, n * nrhs};
+  set_io_offsets(*L);
   L->jac.seg_len = static_cast<int64_t>(L->tape.alg.size());
   // measured on the 274 x 274 KKT batch (8192 instances = 128 wavefronts):
   // budget 160 with reloads 64 instructions ahead, 0.34 ms against 0.80 ms
   // at the tangent sweeps' 200 / 8 (whose kernel also needed scratch)
   L->jac.reg_budget = 160;
   L->jac.prefetch = 64;
-  L->jac.kernel = "cxj_ldl";
+  L->jac.kernel = part == LS_FUSED ? "cxj_ldl" : part == LS_FACTOR ? "cxj_ldl_fact" : "cxj_ldl_solve";
   return L;
 }
 
@@ -1897,6 +1935,34 @@ int jit_linsol_run(LinsolJit& L, int64_t n_inst, const double* a, int64_t a_s, i
   return 0;
 }
 
+int jit_linsol_factor(LinsolJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* const* fac,
+                      int n_fac, int64_t ld, int32_t* status, hipStream_t s) {
+  if (n_fac + 1 > CXH_MAX_IO) throw std::runtime_error("too many factor blocks");
+  L.cnt.reserve(static_cast<size_t>(ld) * sizeof(double));
+  IOTable arg{}, res{};
+  arg.a[0] = {a, a_s, a_t};
+  for (int b = 0; b < n_fac; ++b) res.a[b] = {fac[b], ld, 1};
+  res.a[n_fac] = {L.cnt.as<double>(), ld, 1};
+  const int rc = jit_map_jac(&L.jac, n_inst, arg, res, nullptr, 0, 0, s);
+  if (rc) return rc;
+  if (status) {
+    hipLaunchKernelGGL(count_to_status, dim3(static_cast<unsigned>((n_inst + 255) / 256)), dim3(256), 0, s,
+                       L.cnt.as<double>(), n_inst, status);
+    hip_check(hipGetLastError(), "count_to_status launch");
+  }
+  return 0;
+}
+
+int jit_linsol_solve(LinsolJit& L, int64_t n_inst, const double* const* fac, int n_fac, int64_t ld, double* x,
+                     int64_t x_s, int64_t x_t, hipStream_t s) {
+  if (n_fac + 1 > CXH_MAX_IO) throw std::runtime_error("too many factor blocks");
+  IOTable arg{}, res{};
+  for (int b = 0; b < n_fac; ++b) arg.a[b] = {fac[b], ld, 1};
+  arg.a[n_fac] = {x, x_s, x_t};
+  res.a[0] = {x, x_s, x_t};
+  return jit_map_jac(&L.jac, n_inst, arg, res, nullptr, 0, 0, s);
+}
+
 std::string jit_linsol_source(LinsolJit& L, int segment) {
   return jit_tangent_source(&L.jac, segment, nullptr, nullptr);
 }
@@ -1917,11 +1983,11 @@ std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<i
                                          const std::vector<int>& a_row, const std::vector<int>& v_colind,
                                          const std::vector<int>& v_row, const std::vector<int>& r_colind,
                                          const std::vector<int>& r_row, const std::vector<int>& prinv,
-                                         const std::vector<int>& pc, double eps, int nrhs, bool tr) {
+                                         const std::vector<int>& pc, double eps, int nrhs, bool tr, int part) {
   TapeEmit e;
   const int Z = e.konst(0.0), ONE = e.konst(1.0), TWO = e.konst(2.0), M1 = e.konst(-1.0), EPS = e.konst(eps);
-  const int nnz_a = a_colind[ncol];
-  std::vector<int> x(nrow, Z), v(v_colind[ncol], Z), r(r_colind[ncol], Z), beta(ncol, Z);
+  const int nnz_a = a_colind[ncol], nnz_v = v_colind[ncol], nnz_r = r_colind[ncol];
+  std::vector<int> x(nrow, Z), v(nnz_v, Z), r(nnz_r, Z), beta(ncol, Z);
   auto add_dot = [&](const std::vector<int>& w, int c) {  // 0.0 + sum v[k] * w[v_row[k]], in order
     int alpha = Z;
     for (int k = v_colind[c]; k < v_colind[c + 1]; ++k) alpha = e.op(OP_ADD, alpha, e.op(OP_MUL, v[k], w[v_row[k]]));
@@ -1931,81 +1997,107 @@ std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<i
     for (int k = v_colind[c]; k < v_colind[c + 1]; ++k)
       w[v_row[k]] = e.op(OP_SUB, w[v_row[k]], e.op(OP_MUL, alpha, v[k]));
   };
-  int nz_r = 0;
-  for (int c = 0; c < ncol; ++c) {
-    const int pcc = pc[c];
-    for (int k = a_colind[pcc]; k < a_colind[pcc + 1]; ++k) x[prinv[a_row[k]]] = e.input(0, k);
-    for (int k = r_colind[c]; k < r_colind[c + 1] && r_row[k] < c; ++k) {
-      const int rr = r_row[k];
-      int alpha = add_dot(x, rr);
-      alpha = e.op(OP_MUL, alpha, beta[rr]);
-      sub_axpy(alpha, x, rr);
-      r[nz_r++] = x[rr];
-      x[rr] = Z;
-    }
-    for (int k = v_colind[c]; k < v_colind[c + 1]; ++k) {
-      v[k] = x[v_row[k]];
-      x[v_row[k]] = Z;
-    }
-    // casadi_house on v[v_colind[c] ..]
-    const int off = v_colind[c], nv = v_colind[c + 1] - v_colind[c];
-    const int v0 = v[off];
-    int sigma = Z;
-    for (int i = 1; i < nv; ++i) sigma = e.op(OP_ADD, sigma, e.op(OP_MUL, v[off + i], v[off + i]));
-    const int sq = e.un(OP_SQRT, e.op(OP_ADD, e.op(OP_MUL, v0, v0), sigma));
-    const int sz = e.op(OP_EQ, sigma, Z), np = e.op(OP_LE, v0, Z);
-    const int inner = e.select(np, e.op(OP_SUB, v0, sq), e.op(OP_DIV, e.un(OP_NEG, sigma), e.op(OP_ADD, v0, sq)), ONE);
-    const int v0n = e.select(sz, ONE, inner, ONE);
-    v[off] = v0n;
-    beta[c] = e.select(sz, e.op(OP_MUL, TWO, np), e.op(OP_DIV, M1, e.op(OP_MUL, sq, v0n)), ONE);
-    r[nz_r++] = sq;
-  }
-  // casadi_qr_singular: number of !(|R_cc| >= eps)
-  int cnt = Z;
-  for (int c = 0; c < ncol; ++c)
-    cnt = e.op(OP_ADD, cnt, e.un(OP_NOT, e.op(OP_LE, EPS, e.un(OP_FABS, r[r_colind[c + 1] - 1]))));
-  e.output(1, 0, cnt);
-  std::vector<int>& w = x;  // the solve reuses the factor's work vector, as the kernel does
-  for (int q = 0; q < nrhs; ++q) {
-    if (tr) {
-      for (int c = 0; c < ncol; ++c) w[c] = e.input(1, q * ncol + pc[c]);
-      for (int c = 0; c < ncol; ++c)
-        for (int k = r_colind[c]; k < r_colind[c + 1]; ++k) {
-          const int rr = r_row[k];
-          w[c] = rr == c ? e.op(OP_DIV, w[c], r[k]) : e.op(OP_SUB, w[c], e.op(OP_MUL, r[k], w[rr]));
-        }
-      for (int c1 = 0; c1 < ncol; ++c1) {
-        const int c = ncol - 1 - c1;
-        sub_axpy(e.op(OP_MUL, add_dot(w, c), beta[c]), w, c);
+  if (part != LS_SOLVE) {
+    int nz_r = 0;
+    for (int c = 0; c < ncol; ++c) {
+      const int pcc = pc[c];
+      for (int k = a_colind[pcc]; k < a_colind[pcc + 1]; ++k) x[prinv[a_row[k]]] = e.input(0, k);
+      for (int k = r_colind[c]; k < r_colind[c + 1] && r_row[k] < c; ++k) {
+        const int rr = r_row[k];
+        int alpha = add_dot(x, rr);
+        alpha = e.op(OP_MUL, alpha, beta[rr]);
+        sub_axpy(alpha, x, rr);
+        r[nz_r++] = x[rr];
+        x[rr] = Z;
       }
-      for (int c = 0; c < ncol; ++c) e.output(0, q * ncol + c, w[prinv[c]]);
-    } else {
-      for (int c = 0; c < nrow; ++c) w[c] = Z;
-      for (int c = 0; c < ncol; ++c) w[prinv[c]] = e.input(1, q * ncol + c);
-      for (int c = 0; c < ncol; ++c) sub_axpy(e.op(OP_MUL, add_dot(w, c), beta[c]), w, c);
-      for (int c = ncol - 1; c >= 0; --c)
-        for (int k = r_colind[c + 1] - 1; k >= r_colind[c]; --k) {
-          const int rr = r_row[k];
-          w[rr] = rr == c ? e.op(OP_DIV, w[rr], r[k]) : e.op(OP_SUB, w[rr], e.op(OP_MUL, r[k], w[c]));
-        }
-      for (int c = 0; c < ncol; ++c) e.output(0, q * ncol + pc[c], w[c]);
+      for (int k = v_colind[c]; k < v_colind[c + 1]; ++k) {
+        v[k] = x[v_row[k]];
+        x[v_row[k]] = Z;
+      }
+      // casadi_house on v[v_colind[c] ..]
+      const int off = v_colind[c], nv = v_colind[c + 1] - v_colind[c];
+      const int v0 = v[off];
+      int sigma = Z;
+      for (int i = 1; i < nv; ++i) sigma = e.op(OP_ADD, sigma, e.op(OP_MUL, v[off + i], v[off + i]));
+      const int sq = e.un(OP_SQRT, e.op(OP_ADD, e.op(OP_MUL, v0, v0), sigma));
+      const int sz = e.op(OP_EQ, sigma, Z), np = e.op(OP_LE, v0, Z);
+      const int inner =
+          e.select(np, e.op(OP_SUB, v0, sq), e.op(OP_DIV, e.un(OP_NEG, sigma), e.op(OP_ADD, v0, sq)), ONE);
+      const int v0n = e.select(sz, ONE, inner, ONE);
+      v[off] = v0n;
+      beta[c] = e.select(sz, e.op(OP_MUL, TWO, np), e.op(OP_DIV, M1, e.op(OP_MUL, sq, v0n)), ONE);
+      r[nz_r++] = sq;
     }
+    // casadi_qr_singular: number of !(|R_cc| >= eps)
+    int cnt = Z;
+    for (int c = 0; c < ncol; ++c)
+      cnt = e.op(OP_ADD, cnt, e.un(OP_NOT, e.op(OP_LE, EPS, e.un(OP_FABS, r[r_colind[c + 1] - 1]))));
+    if (part == LS_FACTOR) {  // V, R and beta leave the program: the solve program reads them back
+      for (int k = 0; k < nnz_v; ++k) e.output(0, k, v[k]);
+      for (int k = 0; k < nnz_r; ++k) e.output(1, k, r[k]);
+      for (int c = 0; c < ncol; ++c) e.output(2, c, beta[c]);
+      e.output(3, 0, cnt);
+    } else {
+      e.output(1, 0, cnt);
+    }
+  } else {
+    // the factor leaves its work vector all zero (every entry it fills is
+    // cleared again), which is the state the solve starts from
+    for (int k = 0; k < nnz_v; ++k) v[k] = e.input(0, k);
+    for (int k = 0; k < nnz_r; ++k) r[k] = e.input(1, k);
+    for (int c = 0; c < ncol; ++c) beta[c] = e.input(2, c);
   }
+  std::vector<int>& w = x;  // the solve reuses the factor's work vector, as the kernel does
+  const int rhs_blk = part == LS_SOLVE ? 3 : 1;
+  if (part != LS_FACTOR)
+    for (int q = 0; q < nrhs; ++q) {
+      if (tr) {
+        for (int c = 0; c < ncol; ++c) w[c] = e.input(rhs_blk, q * ncol + pc[c]);
+        for (int c = 0; c < ncol; ++c)
+          for (int k = r_colind[c]; k < r_colind[c + 1]; ++k) {
+            const int rr = r_row[k];
+            w[c] = rr == c ? e.op(OP_DIV, w[c], r[k]) : e.op(OP_SUB, w[c], e.op(OP_MUL, r[k], w[rr]));
+          }
+        for (int c1 = 0; c1 < ncol; ++c1) {
+          const int c = ncol - 1 - c1;
+          sub_axpy(e.op(OP_MUL, add_dot(w, c), beta[c]), w, c);
+        }
+        for (int c = 0; c < ncol; ++c) e.output(0, q * ncol + c, w[prinv[c]]);
+      } else {
+        for (int c = 0; c < nrow; ++c) w[c] = Z;
+        for (int c = 0; c < ncol; ++c) w[prinv[c]] = e.input(rhs_blk, q * ncol + c);
+        for (int c = 0; c < ncol; ++c) sub_axpy(e.op(OP_MUL, add_dot(w, c), beta[c]), w, c);
+        for (int c = ncol - 1; c >= 0; --c)
+          for (int k = r_colind[c + 1] - 1; k >= r_colind[c]; --k) {
+            const int rr = r_row[k];
+            w[rr] = rr == c ? e.op(OP_DIV, w[rr], r[k]) : e.op(OP_SUB, w[rr], e.op(OP_MUL, r[k], w[c]));
+          }
+        for (int c = 0; c < ncol; ++c) e.output(0, q * ncol + pc[c], w[c]);
+      }
+    }
   auto L = std::make_shared<LinsolJit>();
   L->nrhs = nrhs;
   L->tape.alg = std::move(e.alg);
   L->tape.ws = e.ws;
-  L->tape.nnz_in = {nnz_a, static_cast<int64_t>(ncol) * nrhs};
-  L->tape.nnz_out = {static_cast<int64_t>(ncol) * nrhs, 1};
+  const int64_t nx = static_cast<int64_t>(ncol) * nrhs;
+  if (part == LS_FUSED) {
+    L->tape.nnz_in = {nnz_a, nx};
+    L->tape.nnz_out = {nx, 1};
+  } else if (part == LS_FACTOR) {
+    L->tape.nnz_in = {nnz_a};
+    L->tape.nnz_out = {nnz_v, nnz_r, ncol, 1};
+  } else {
+    L->tape.nnz_in = {nnz_v, nnz_r, ncol, nx};
+    L->tape.nnz_out = {nx};
+  }
   L->tape.engine = CXH_ENGINE_JIT;
   L->jac.tape = &L->tape;
   L->jac.ndir = 0;
-  L->jac.in_off = {0, nnz_a};
-  L->jac.out_off = {0, ncol * nrhs};
+  set_io_offsets(*L);
   L->jac.seg_len = 16384;
   L->jac.reg_budget = 160;
   L->jac.prefetch = 64;
-  L->jac.kernel = "cxj_qr";
+  L->jac.kernel = part == LS_FUSED ? "cxj_qr" : part == LS_FACTOR ? "cxj_qr_fact" : "cxj_qr_solve";
   return L;
 }
 

@@ -110,7 +110,9 @@ struct LdlArgs {
   int64_t n_inst;
 };
 
-template <bool WLDS>
+// MODE: LS_FUSED factor + solve, LS_FACTOR the factor alone (L^T and D stay
+// in A.lt / A.d), LS_SOLVE the solve alone from the L^T and D stored there.
+template <bool WLDS, int MODE>
 __global__ void __launch_bounds__(64) ldl_kernel(LdlArgs A) {
   extern __shared__ double lds[];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -119,6 +121,7 @@ __global__ void __launch_bounds__(64) ldl_kernel(LdlArgs A) {
   Col lt{A.lt + j, A.lanes}, d{A.d + j, A.lanes};
   Col w = WLDS ? Col{lds + threadIdx.x, static_cast<int64_t>(blockDim.x)} : Col{A.wg + j, A.lanes};
   const int n = A.n;
+  if (MODE != LS_SOLVE) {
   const double* a = A.a + jj * A.a_t;
   for (int r = 0; r < n; ++r) w[r] = 0;
   // scatter the permuted A into L^T and D
@@ -148,6 +151,8 @@ __global__ void __launch_bounds__(64) ldl_kernel(LdlArgs A) {
   // zero pivots, or a breakdown that already turned D non-finite
   for (int c = 0; c < n; ++c) zero_pivots += !(d[c] != 0 && isfinite(d[c]));
   if (active && A.status) A.status[j] = zero_pivots;
+  }
+  if (MODE == LS_FACTOR) return;
   // solve: x = P' L'^-1 D^-1 L^-1 P b for every right-hand side
   for (int q = 0; q < A.nrhs; ++q) {
     double* x = A.x + jj * A.x_t + static_cast<int64_t>(q) * n * A.x_s;
@@ -200,7 +205,8 @@ __device__ double qr_house(Col v, int64_t off, double* beta, int nv) {
   return s;
 }
 
-template <bool WLDS>
+// MODE as ldl_kernel: V, R and beta stay in A.v / A.r / A.beta.
+template <bool WLDS, int MODE>
 __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
   extern __shared__ double lds[];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -208,9 +214,12 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
   const int64_t jj = active ? j : A.n_inst - 1;
   Col v{A.v + j, A.lanes}, r{A.r + j, A.lanes}, beta{A.beta + j, A.lanes};
   Col x = WLDS ? Col{lds + threadIdx.x, static_cast<int64_t>(blockDim.x)} : Col{A.wg + j, A.lanes};
-  const double* a = A.a + jj * A.a_t;
   const int ncol = A.ncol, nrow = A.nrow_ext;
+  // the factor leaves x all zero (every entry it fills is cleared again):
+  // the state the solve starts from
   for (int i = 0; i < nrow; ++i) x[i] = 0;
+  if (MODE != LS_SOLVE) {
+  const double* a = A.a + jj * A.a_t;
   int nz_r = 0;
   for (int c = 0; c < ncol; ++c) {
     const int pcc = A.pc[c];
@@ -235,6 +244,8 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
   int nullity = 0;
   for (int c = 0; c < ncol; ++c) nullity += !(fabs(r[A.r_colind[c + 1] - 1]) >= A.eps);
   if (active && A.status) A.status[j] = nullity;
+  }
+  if (MODE == LS_FACTOR) return;
   // casadi_qr_solve
   Col w = x;
   for (int q = 0; q < A.nrhs; ++q) {
@@ -277,6 +288,55 @@ __global__ void __launch_bounds__(64) qr_kernel(QrArgs A) {
   }
 }
 
+// Per instance: the number of D entries that are negative (what = 0, the
+// inertia LinsolLdl::neig reports) or nonzero (what = 1, LinsolLdl::rank),
+// d laid out [row][ld].
+__global__ void ldl_count_kernel(const double* __restrict__ d, int n, int64_t ld, int64_t n_inst, int what,
+                                 int32_t* __restrict__ out) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n_inst) return;
+  int c = 0;
+  for (int i = 0; i < n; ++i) {
+    const double v = d[i * ld + k];
+    c += what == 0 ? (v < 0) : (v != 0);
+  }
+  out[k] = c;
+}
+
+// Per instance: casadi_det (casadi/core/runtime/casadi_det.hpp:26-45) of the
+// QR factors, times the sign of the row and column permutations
+// (LinsolQr::det, casadi/solvers/linsol_qr.cpp).  V, R, beta laid out
+// [nonzero][ld]; the products run in the reference's order.
+__global__ void qr_det_kernel(const double* __restrict__ v, const double* __restrict__ r,
+                              const double* __restrict__ beta, const int* __restrict__ v_colind,
+                              const int* __restrict__ r_colind, int ncol, int64_t ld, int64_t n_inst, double sign,
+                              double* __restrict__ det) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n_inst) return;
+  double dt = 1;
+  for (int c = 0; c < ncol; ++c) dt *= r[static_cast<int64_t>(r_colind[c + 1] - 1) * ld + k];
+  for (int c = 0; c < ncol; ++c) {
+    double vtv = 0;
+    for (int q = v_colind[c]; q < v_colind[c + 1]; ++q) {
+      const double x = v[static_cast<int64_t>(q) * ld + k];
+      vtv += x * x;
+    }
+    dt *= 1 - beta[static_cast<int64_t>(c) * ld + k] * vtv;
+  }
+  det[k] = sign * dt;
+}
+
+// flag |= 1 when any a[i] != b[i] (numeric comparison, as casadi_cache_check:
+// a NaN never matches)
+__global__ void differ_kernel(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                              int* __restrict__ flag) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  bool differ = false;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    differ |= a[i] != b[i];
+  if (__any(differ) && threadIdx.x % 64 == 0) atomicOr(flag, 1);
+}
+
 // CasADi compact sparsity -> (n, colind, row) as int32
 struct SpI {
   int nrow = 0, ncol = 0;
@@ -303,6 +363,18 @@ SpI parse_sp(const int64_t* sp) {
   return s;
 }
 
+// (-1)^(n - number of cycles): the determinant of a permutation matrix
+double permutation_sign(const std::vector<int>& p) {
+  std::vector<char> seen(p.size(), 0);
+  int64_t cycles = 0;
+  for (size_t i = 0; i < p.size(); ++i) {
+    if (seen[i]) continue;
+    ++cycles;
+    for (size_t j = i; !seen[j]; j = static_cast<size_t>(p[j])) seen[j] = 1;
+  }
+  return (static_cast<int64_t>(p.size()) - cycles) % 2 ? -1.0 : 1.0;
+}
+
 std::vector<int> to_perm(const int64_t* p, int n, int range) {
   std::vector<int> v(n);
   for (int i = 0; i < n; ++i) {
@@ -314,6 +386,9 @@ std::vector<int> to_perm(const int64_t* p, int n, int range) {
 
 }  // namespace
 
+// The factors of the last nfact stay in the handle's device memory, laid out
+// [nonzero][ld] with ld = n_inst rounded up to 64 (both engines write and read
+// this layout), until the next nfact or factor + solve.
 struct cxh_ldl_s {
   SpI a, lt;
   std::vector<int> p;
@@ -322,17 +397,34 @@ struct cxh_ldl_s {
   bool uploaded = false;                               // pattern arrays on the device
   int engine = CXH_ENGINE_JIT;                          // specialised program or the generic kernel
   std::map<int, std::shared_ptr<sxh::LinsolJit>> jit;  // specialised factor + solve per nrhs
+  std::shared_ptr<sxh::LinsolJit> jit_fact;            // specialised factor alone
+  std::map<int, std::shared_ptr<sxh::LinsolJit>> jit_solve;  // specialised solve alone per nrhs
+  int64_t fact_n = -1;                                 // instances factored by the last nfact (-1: none)
 };
 
 struct cxh_qr_s {
   SpI a, v, r;
   std::vector<int> prinv, pc;
   double eps = 1e-12;
+  double pivot_sign = 1;  // sign of the row and column permutations
   DeviceBuffer d_a_colind, d_a_row, d_v_colind, d_v_row, d_r_colind, d_r_row, d_prinv, d_pc;
   DeviceBuffer v_vals, r_vals, beta, w;
   bool uploaded = false;                               // pattern arrays on the device
   int engine = CXH_ENGINE_JIT;                         // specialised program or the generic kernel
   std::map<int, std::shared_ptr<sxh::LinsolJit>> jit;  // specialised factor + solve per (nrhs, tr)
+  std::shared_ptr<sxh::LinsolJit> jit_fact;            // specialised factor alone
+  std::map<int, std::shared_ptr<sxh::LinsolJit>> jit_solve;  // specialised solve alone per (nrhs, tr)
+  int64_t fact_n = -1;                                 // instances factored by the last nfact (-1: none)
+  // factorisations remembered (LinsolQr's "cache" option): each holds a copy
+  // of the batch of A values it was computed from, and its V, R, beta
+  struct CacheEntry {
+    DeviceBuffer a, v, r, beta;
+    int64_t n_inst = 0;
+    int32_t layout = 0;
+  };
+  std::vector<CacheEntry> cache;
+  std::vector<int> cache_loc;  // most recently used first; -1: never filled
+  DeviceBuffer cmp_flag;
 };
 
 static sxh::LinsolJit& qr_program(cxh_qr_s* q, int nrhs, bool tr) {
@@ -372,23 +464,9 @@ int cxh_ldl_destroy(cxh_ldl l) {
   return 0;
 }
 
-int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, int32_t nrhs, int32_t* status,
-                         int32_t layout, void* stream) {
-  CXH_TRY
-  if (!l) return cxh_fail("null ldl handle");
-  if (n_inst < 0 || nrhs < 0) return cxh_fail("negative size");
-  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
-  if (n_inst == 0) return 0;
-  if (!a || (!x && nrhs > 0)) return cxh_fail("null data pointer");
-  const int n = l->lt.ncol;
-  const int64_t nnz_a = l->a.colind.back(), nnz_lt = l->lt.colind.back();
-  if (l->engine == CXH_ENGINE_JIT) {
-    auto& J = l->jit[nrhs];
-    if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs);
-    const bool soa = layout == CXH_SOA;
-    return jit_linsol_run(*J, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x, soa ? n_inst : 1,
-                       soa ? 1 : static_cast<int64_t>(n) * nrhs, status, static_cast<hipStream_t>(stream));
-  }
+// The generic kernel in one of its modes (LS_FUSED, LS_FACTOR, LS_SOLVE).
+static void ldl_generic(cxh_ldl_s* l, int mode, int64_t n_inst, const double* a, double* x, int32_t nrhs,
+                        int32_t* status, int32_t layout, hipStream_t s) {
   if (!l->uploaded) {  // the generic kernel reads the pattern from device memory
     upload_ints(l->d_a_colind, l->a.colind);
     upload_ints(l->d_a_row, l->a.row);
@@ -397,6 +475,8 @@ int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, 
     upload_ints(l->d_p, l->p);
     l->uploaded = true;
   }
+  const int n = l->lt.ncol;
+  const int64_t nnz_a = l->a.colind.back(), nnz_lt = l->lt.colind.back();
   const int bs = 64;
   const int64_t grid = (n_inst + bs - 1) / bs;
   LdlArgs A{};
@@ -414,27 +494,137 @@ int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, 
   } else {
     A.a_s = 1; A.a_t = nnz_a; A.x_s = 1; A.x_t = static_cast<int64_t>(n) * nrhs;
   }
-  A.lanes = grid * bs;
-  l->lt_vals.reserve(std::max<size_t>(nnz_lt, 1) * A.lanes * sizeof(double));
-  l->d_vals.reserve(std::max<size_t>(n, 1) * A.lanes * sizeof(double));
+  A.lanes = grid * bs;  // = n_inst rounded up to 64
+  if (mode != LS_SOLVE) {
+    l->lt_vals.reserve(std::max<size_t>(nnz_lt, 1) * A.lanes * sizeof(double));
+    l->d_vals.reserve(std::max<size_t>(n, 1) * A.lanes * sizeof(double));
+  }
   A.lt = l->lt_vals.as<double>();
   A.d = l->d_vals.as<double>();
   A.status = status;
   A.n_inst = n_inst;
-  hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t lds = static_cast<size_t>(std::max(n, 1)) * bs * sizeof(double);
-  if (lds <= static_cast<size_t>(device_lds_bytes())) {
-    allow_lds(reinterpret_cast<const void*>(ldl_kernel<true>), lds);
-    hipLaunchKernelGGL(ldl_kernel<true>, dim3(grid), dim3(bs), lds, s, A);
-  } else {
+  const bool in_lds = lds <= static_cast<size_t>(device_lds_bytes());
+  if (!in_lds) {
     l->w.reserve(std::max<size_t>(n, 1) * A.lanes * sizeof(double));
     A.wg = l->w.as<double>();
-    hipLaunchKernelGGL(ldl_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
+  }
+  auto go = [&](auto kernel) {
+    if (in_lds) allow_lds(reinterpret_cast<const void*>(kernel), lds);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(bs), in_lds ? lds : 0, s, A);
+  };
+  if (in_lds) {
+    if (mode == LS_FUSED) go(ldl_kernel<true, LS_FUSED>);
+    else if (mode == LS_FACTOR) go(ldl_kernel<true, LS_FACTOR>);
+    else go(ldl_kernel<true, LS_SOLVE>);
+  } else {
+    if (mode == LS_FUSED) go(ldl_kernel<false, LS_FUSED>);
+    else if (mode == LS_FACTOR) go(ldl_kernel<false, LS_FACTOR>);
+    else go(ldl_kernel<false, LS_SOLVE>);
   }
   hip_check(hipGetLastError(), "ldl_kernel launch");
+}
+
+static int ldl_check(cxh_ldl_s* l, int64_t n_inst, int32_t nrhs, int32_t layout) {
+  if (!l) return cxh_fail("null ldl handle");
+  if (n_inst < 0 || nrhs < 0) return cxh_fail("negative size");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  return 0;
+}
+
+int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, int32_t nrhs, int32_t* status,
+                         int32_t layout, void* stream) {
+  CXH_TRY
+  if (int rc = ldl_check(l, n_inst, nrhs, layout)) return rc;
+  if (n_inst == 0) return 0;
+  if (!a || (!x && nrhs > 0)) return cxh_fail("null data pointer");
+  const int n = l->lt.ncol;
+  const int64_t nnz_a = l->a.colind.back();
+  l->fact_n = -1;  // the fused call keeps no factors
+  if (l->engine == CXH_ENGINE_JIT) {
+    auto& J = l->jit[nrhs];
+    if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs);
+    const bool soa = layout == CXH_SOA;
+    return jit_linsol_run(*J, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x, soa ? n_inst : 1,
+                       soa ? 1 : static_cast<int64_t>(n) * nrhs, status, static_cast<hipStream_t>(stream));
+  }
+  ldl_generic(l, LS_FUSED, n_inst, a, x, nrhs, status, layout, static_cast<hipStream_t>(stream));
   return 0;
   CXH_CATCH
 }
+
+int cxh_ldl_nfact(cxh_ldl l, int64_t n_inst, const double* a, int32_t* status, int32_t layout, void* stream) {
+  CXH_TRY
+  if (int rc = ldl_check(l, n_inst, 0, layout)) return rc;
+  l->fact_n = -1;
+  if (n_inst == 0) {
+    l->fact_n = 0;
+    return 0;
+  }
+  if (!a) return cxh_fail("null data pointer");
+  const int n = l->lt.ncol;
+  const int64_t nnz_a = l->a.colind.back(), nnz_lt = l->lt.colind.back();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (l->engine == CXH_ENGINE_JIT) {
+    if (!l->jit_fact) l->jit_fact = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, 0,
+                                                   LS_FACTOR);
+    const int64_t ld = round_up(n_inst, 64);
+    l->lt_vals.reserve(std::max<size_t>(nnz_lt, 1) * ld * sizeof(double));
+    l->d_vals.reserve(std::max<size_t>(n, 1) * ld * sizeof(double));
+    double* fac[2] = {l->lt_vals.as<double>(), l->d_vals.as<double>()};
+    const bool soa = layout == CXH_SOA;
+    if (int rc = jit_linsol_factor(*l->jit_fact, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, fac, 2, ld, status, s))
+      return rc;
+  } else {
+    ldl_generic(l, LS_FACTOR, n_inst, a, nullptr, 0, status, layout, s);
+  }
+  l->fact_n = n_inst;
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_ldl_solve(cxh_ldl l, int64_t n_inst, double* x, int32_t nrhs, int32_t tr, int32_t layout, void* stream) {
+  CXH_TRY
+  (void)tr;  // A is symmetric: the transposed solve is the same (LinsolLdl::solve ignores tr)
+  if (int rc = ldl_check(l, n_inst, nrhs, layout)) return rc;
+  if (l->fact_n < 0) return cxh_fail("cxh_ldl_solve: the linear system has not been factorized (call cxh_ldl_nfact)");
+  if (n_inst != l->fact_n)
+    return cxh_fail("cxh_ldl_solve: " + std::to_string(n_inst) + " instances, the last nfact factored " +
+                    std::to_string(l->fact_n));
+  if (n_inst == 0 || nrhs == 0) return 0;
+  if (!x) return cxh_fail("null data pointer");
+  const int n = l->lt.ncol;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (l->engine == CXH_ENGINE_JIT) {
+    auto& J = l->jit_solve[nrhs];
+    if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs, LS_SOLVE);
+    const int64_t ld = round_up(n_inst, 64);
+    const double* fac[2] = {l->lt_vals.as<double>(), l->d_vals.as<double>()};
+    const bool soa = layout == CXH_SOA;
+    return jit_linsol_solve(*J, n_inst, fac, 2, ld, x, soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(n) * nrhs,
+                            s);
+  }
+  ldl_generic(l, LS_SOLVE, n_inst, nullptr, x, nrhs, nullptr, layout, s);
+  return 0;
+  CXH_CATCH
+}
+
+static int ldl_count(cxh_ldl l, int32_t* out, int what, void* stream) {
+  CXH_TRY
+  if (!l) return cxh_fail("null ldl handle");
+  if (l->fact_n < 0) return cxh_fail("the linear system has not been factorized (call cxh_ldl_nfact)");
+  if (l->fact_n == 0) return 0;
+  if (!out) return cxh_fail("null output array");
+  const int64_t ld = round_up(l->fact_n, 64);
+  hipLaunchKernelGGL(ldl_count_kernel, dim3(static_cast<unsigned>((l->fact_n + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), l->d_vals.as<double>(), l->lt.ncol, ld, l->fact_n, what, out);
+  hip_check(hipGetLastError(), "ldl_count_kernel launch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_ldl_neig(cxh_ldl l, int32_t* neig, void* stream) { return ldl_count(l, neig, 0, stream); }
+int cxh_ldl_rank(cxh_ldl l, int32_t* rank, void* stream) { return ldl_count(l, rank, 1, stream); }
 
 int cxh_ldl_engine(cxh_ldl l, int32_t engine) {
   CXH_TRY
@@ -510,6 +700,7 @@ int cxh_qr_create(const int64_t* sp_a, const int64_t* sp_v, const int64_t* sp_r,
   q->prinv = to_perm(prinv, nrow_ext, nrow_ext);
   q->pc = to_perm(pc, ncol, ncol);
   q->eps = eps;
+  q->pivot_sign = permutation_sign(q->prinv) * permutation_sign(q->pc);
   *out = q.release();
   return 0;
   CXH_CATCH
@@ -520,22 +711,9 @@ int cxh_qr_destroy(cxh_qr q) {
   return 0;
 }
 
-int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, int32_t nrhs, int32_t tr,
-                        int32_t* status, int32_t layout, void* stream) {
-  CXH_TRY
-  if (!q) return cxh_fail("null qr handle");
-  if (n_inst < 0 || nrhs < 0) return cxh_fail("negative size");
-  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
-  if (n_inst == 0) return 0;
-  if (!a || (!x && nrhs > 0)) return cxh_fail("null data pointer");
-  const int ncol = q->a.ncol, nrow_ext = q->v.nrow;
-  const int64_t nnz_a = q->a.colind.back();
-  if (q->engine == CXH_ENGINE_JIT) {
-    const bool soa = layout == CXH_SOA;
-    return jit_linsol_run(qr_program(q, nrhs, tr != 0), n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x,
-                          soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(ncol) * nrhs, status,
-                          static_cast<hipStream_t>(stream));
-  }
+// The generic kernel in one of its modes (LS_FUSED, LS_FACTOR, LS_SOLVE).
+static void qr_generic(cxh_qr_s* q, int mode, int64_t n_inst, const double* a, double* x, int32_t nrhs, int32_t tr,
+                       int32_t* status, int32_t layout, hipStream_t s) {
   if (!q->uploaded) {  // the generic kernel reads the pattern from device memory
     upload_ints(q->d_a_colind, q->a.colind);
     upload_ints(q->d_a_row, q->a.row);
@@ -547,6 +725,8 @@ int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, in
     upload_ints(q->d_pc, q->pc);
     q->uploaded = true;
   }
+  const int ncol = q->a.ncol, nrow_ext = q->v.nrow;
+  const int64_t nnz_a = q->a.colind.back();
   const int bs = 64;
   const int64_t grid = (n_inst + bs - 1) / bs;
   QrArgs A{};
@@ -570,27 +750,246 @@ int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, in
   } else {
     A.a_s = 1; A.a_t = nnz_a; A.x_s = 1; A.x_t = static_cast<int64_t>(ncol) * nrhs;
   }
-  A.lanes = grid * bs;
-  q->v_vals.reserve(std::max<size_t>(q->v.colind.back(), 1) * A.lanes * sizeof(double));
-  q->r_vals.reserve(std::max<size_t>(q->r.colind.back(), 1) * A.lanes * sizeof(double));
-  q->beta.reserve(std::max<size_t>(ncol, 1) * A.lanes * sizeof(double));
+  A.lanes = grid * bs;  // = n_inst rounded up to 64
+  if (mode != LS_SOLVE) {
+    q->v_vals.reserve(std::max<size_t>(q->v.colind.back(), 1) * A.lanes * sizeof(double));
+    q->r_vals.reserve(std::max<size_t>(q->r.colind.back(), 1) * A.lanes * sizeof(double));
+    q->beta.reserve(std::max<size_t>(ncol, 1) * A.lanes * sizeof(double));
+  }
   A.v = q->v_vals.as<double>();
   A.r = q->r_vals.as<double>();
   A.beta = q->beta.as<double>();
   A.status = status;
   A.n_inst = n_inst;
-  hipStream_t s = static_cast<hipStream_t>(stream);
   const int wlen = std::max(nrow_ext, ncol);
   const size_t lds = static_cast<size_t>(std::max(wlen, 1)) * bs * sizeof(double);
-  if (lds <= static_cast<size_t>(device_lds_bytes())) {
-    allow_lds(reinterpret_cast<const void*>(qr_kernel<true>), lds);
-    hipLaunchKernelGGL(qr_kernel<true>, dim3(grid), dim3(bs), lds, s, A);
-  } else {
+  const bool in_lds = lds <= static_cast<size_t>(device_lds_bytes());
+  if (!in_lds) {
     q->w.reserve(std::max<size_t>(wlen, 1) * A.lanes * sizeof(double));
     A.wg = q->w.as<double>();
-    hipLaunchKernelGGL(qr_kernel<false>, dim3(grid), dim3(bs), 0, s, A);
+  }
+  auto go = [&](auto kernel) {
+    if (in_lds) allow_lds(reinterpret_cast<const void*>(kernel), lds);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(bs), in_lds ? lds : 0, s, A);
+  };
+  if (in_lds) {
+    if (mode == LS_FUSED) go(qr_kernel<true, LS_FUSED>);
+    else if (mode == LS_FACTOR) go(qr_kernel<true, LS_FACTOR>);
+    else go(qr_kernel<true, LS_SOLVE>);
+  } else {
+    if (mode == LS_FUSED) go(qr_kernel<false, LS_FUSED>);
+    else if (mode == LS_FACTOR) go(qr_kernel<false, LS_FACTOR>);
+    else go(qr_kernel<false, LS_SOLVE>);
   }
   hip_check(hipGetLastError(), "qr_kernel launch");
+}
+
+static int qr_check(cxh_qr_s* q, int64_t n_inst, int32_t nrhs, int32_t layout) {
+  if (!q) return cxh_fail("null qr handle");
+  if (n_inst < 0 || nrhs < 0) return cxh_fail("negative size");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  return 0;
+}
+
+int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, int32_t nrhs, int32_t tr,
+                        int32_t* status, int32_t layout, void* stream) {
+  CXH_TRY
+  if (int rc = qr_check(q, n_inst, nrhs, layout)) return rc;
+  if (n_inst == 0) return 0;
+  if (!a || (!x && nrhs > 0)) return cxh_fail("null data pointer");
+  const int ncol = q->a.ncol;
+  const int64_t nnz_a = q->a.colind.back();
+  q->fact_n = -1;  // the fused call keeps no factors
+  if (q->engine == CXH_ENGINE_JIT) {
+    const bool soa = layout == CXH_SOA;
+    return jit_linsol_run(qr_program(q, nrhs, tr != 0), n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, x,
+                          soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(ncol) * nrhs, status,
+                          static_cast<hipStream_t>(stream));
+  }
+  qr_generic(q, LS_FUSED, n_inst, a, x, nrhs, tr, status, layout, static_cast<hipStream_t>(stream));
+  return 0;
+  CXH_CATCH
+}
+
+// casadi_cache_check (casadi/core/runtime/casadi_cache.hpp) over whole
+// batches: the entries are walked most recently used first; a never-filled
+// entry ends the walk (a miss that stores there), an entry whose batch of A
+// values equals a (same count, same layout, every value ==) is a hit and moves
+// to the front.  A miss with every entry filled stores in the last entry
+// walked, and the order is left as it is, as the reference does.
+static cxh_qr_s::CacheEntry* qr_cache_check(cxh_qr_s* q, int64_t n_inst, const double* a, int32_t layout,
+                                            hipStream_t s, bool* hit) {
+  *hit = false;
+  const int sz = static_cast<int>(q->cache.size());
+  if (sz == 0) return nullptr;
+  const int64_t na = q->a.colind.back() * n_inst;
+  q->cmp_flag.reserve(sizeof(int));
+  int* flag = q->cmp_flag.as<int>();
+  cxh_qr_s::CacheEntry* e = nullptr;
+  for (int i = 0; i < sz; ++i) {
+    if (q->cache_loc[i] < 0) {
+      q->cache_loc[i] = i;
+      return &q->cache[i];
+    }
+    e = &q->cache[q->cache_loc[i]];
+    bool match = e->n_inst == n_inst && e->layout == layout;
+    if (match && na > 0) {
+      int h = 0;
+      hip_check(hipMemcpyAsync(flag, &h, sizeof h, hipMemcpyHostToDevice, s), "cache flag");
+      const unsigned grid = static_cast<unsigned>(std::min<int64_t>((na + 255) / 256, 4096));
+      hipLaunchKernelGGL(differ_kernel, dim3(grid), dim3(256), 0, s, a, e->a.as<double>(), na, flag);
+      hip_check(hipGetLastError(), "differ_kernel launch");
+      hip_check(hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, s), "cache flag");
+      hip_check(hipStreamSynchronize(s), "cache check");
+      match = h == 0;
+    }
+    if (match) {
+      const int c = q->cache_loc[i];
+      for (int k = i; k > 0; --k) q->cache_loc[k] = q->cache_loc[k - 1];
+      q->cache_loc[0] = c;
+      *hit = true;
+      return e;
+    }
+  }
+  return e;
+}
+
+int cxh_qr_nfact(cxh_qr q, int64_t n_inst, const double* a, int32_t* status, int32_t layout, void* stream) {
+  CXH_TRY
+  if (int rc = qr_check(q, n_inst, 0, layout)) return rc;
+  q->fact_n = -1;
+  if (n_inst == 0) {
+    q->fact_n = 0;
+    return 0;
+  }
+  if (!a) return cxh_fail("null data pointer");
+  const int ncol = q->a.ncol;
+  const int64_t nnz_a = q->a.colind.back(), nnz_v = q->v.colind.back(), nnz_r = q->r.colind.back();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t ld = round_up(n_inst, 64);
+  const size_t bv = static_cast<size_t>(std::max<int64_t>(nnz_v, 1)) * ld * sizeof(double),
+               br = static_cast<size_t>(std::max<int64_t>(nnz_r, 1)) * ld * sizeof(double),
+               bb = static_cast<size_t>(std::max(ncol, 1)) * ld * sizeof(double);
+  bool hit = false;
+  cxh_qr_s::CacheEntry* e = qr_cache_check(q, n_inst, a, layout, s, &hit);
+  if (hit) {  // the remembered factors; status is the nullity they were stored with (zero)
+    q->v_vals.reserve(bv);
+    q->r_vals.reserve(br);
+    q->beta.reserve(bb);
+    hip_check(hipMemcpyAsync(q->v_vals.as<double>(), e->v.as<double>(), bv, hipMemcpyDeviceToDevice, s), "cache");
+    hip_check(hipMemcpyAsync(q->r_vals.as<double>(), e->r.as<double>(), br, hipMemcpyDeviceToDevice, s), "cache");
+    hip_check(hipMemcpyAsync(q->beta.as<double>(), e->beta.as<double>(), bb, hipMemcpyDeviceToDevice, s), "cache");
+    if (status) hip_check(hipMemsetAsync(status, 0, static_cast<size_t>(n_inst) * sizeof(int32_t), s), "status");
+    q->fact_n = n_inst;
+    return 0;
+  }
+  DeviceBuffer nullity;  // the cache stores only factorisations of nonsingular batches
+  int32_t* st = status;
+  if (e && !st) {
+    nullity.reserve(static_cast<size_t>(n_inst) * sizeof(int32_t));
+    st = nullity.as<int32_t>();
+  }
+  if (q->engine == CXH_ENGINE_JIT) {
+    if (!q->jit_fact)
+      q->jit_fact = jit_qr_create(q->a.ncol, q->v.nrow, q->a.colind, q->a.row, q->v.colind, q->v.row, q->r.colind,
+                                  q->r.row, q->prinv, q->pc, q->eps, 0, false, LS_FACTOR);
+    q->v_vals.reserve(bv);
+    q->r_vals.reserve(br);
+    q->beta.reserve(bb);
+    double* fac[3] = {q->v_vals.as<double>(), q->r_vals.as<double>(), q->beta.as<double>()};
+    const bool soa = layout == CXH_SOA;
+    if (int rc = jit_linsol_factor(*q->jit_fact, n_inst, a, soa ? n_inst : 1, soa ? 1 : nnz_a, fac, 3, ld, st, s))
+      return rc;
+  } else {
+    qr_generic(q, LS_FACTOR, n_inst, a, nullptr, 0, 0, st, layout, s);
+  }
+  q->fact_n = n_inst;
+  if (e) {  // remember this factorisation unless an instance is singular (LinsolQr::nfact returns 1 first)
+    std::vector<int32_t> h(static_cast<size_t>(n_inst));
+    hip_check(hipMemcpyAsync(h.data(), st, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
+    hip_check(hipStreamSynchronize(s), "status");
+    if (std::all_of(h.begin(), h.end(), [](int32_t v) { return v == 0; })) {
+      e->a.reserve(static_cast<size_t>(std::max<int64_t>(nnz_a * n_inst, 1)) * sizeof(double));
+      e->v.reserve(bv);
+      e->r.reserve(br);
+      e->beta.reserve(bb);
+      hip_check(hipMemcpyAsync(e->a.as<double>(), a, static_cast<size_t>(nnz_a * n_inst) * sizeof(double),
+                               hipMemcpyDeviceToDevice, s), "cache");
+      hip_check(hipMemcpyAsync(e->v.as<double>(), q->v_vals.as<double>(), bv, hipMemcpyDeviceToDevice, s), "cache");
+      hip_check(hipMemcpyAsync(e->r.as<double>(), q->r_vals.as<double>(), br, hipMemcpyDeviceToDevice, s), "cache");
+      hip_check(hipMemcpyAsync(e->beta.as<double>(), q->beta.as<double>(), bb, hipMemcpyDeviceToDevice, s), "cache");
+      e->n_inst = n_inst;
+      e->layout = layout;
+      hip_check(hipStreamSynchronize(s), "cache");
+    } else {
+      e->n_inst = -1;  // holds nothing that can match
+    }
+  }
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_qr_solve(cxh_qr q, int64_t n_inst, double* x, int32_t nrhs, int32_t tr, int32_t layout, void* stream) {
+  CXH_TRY
+  if (int rc = qr_check(q, n_inst, nrhs, layout)) return rc;
+  if (q->fact_n < 0) return cxh_fail("cxh_qr_solve: the linear system has not been factorized (call cxh_qr_nfact)");
+  if (n_inst != q->fact_n)
+    return cxh_fail("cxh_qr_solve: " + std::to_string(n_inst) + " instances, the last nfact factored " +
+                    std::to_string(q->fact_n));
+  if (n_inst == 0 || nrhs == 0) return 0;
+  if (!x) return cxh_fail("null data pointer");
+  const int ncol = q->a.ncol;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (q->engine == CXH_ENGINE_JIT) {
+    auto& J = q->jit_solve[2 * nrhs + (tr ? 1 : 0)];
+    if (!J)
+      J = jit_qr_create(q->a.ncol, q->v.nrow, q->a.colind, q->a.row, q->v.colind, q->v.row, q->r.colind, q->r.row,
+                        q->prinv, q->pc, q->eps, nrhs, tr != 0, LS_SOLVE);
+    const int64_t ld = round_up(n_inst, 64);
+    const double* fac[3] = {q->v_vals.as<double>(), q->r_vals.as<double>(), q->beta.as<double>()};
+    const bool soa = layout == CXH_SOA;
+    return jit_linsol_solve(*J, n_inst, fac, 3, ld, x, soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(ncol) * nrhs,
+                            s);
+  }
+  qr_generic(q, LS_SOLVE, n_inst, nullptr, x, nrhs, tr, nullptr, layout, s);
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_qr_det(cxh_qr q, double* det, void* stream) {
+  CXH_TRY
+  if (!q) return cxh_fail("null qr handle");
+  if (q->fact_n < 0) return cxh_fail("cxh_qr_det: the linear system has not been factorized (call cxh_qr_nfact)");
+  if (q->fact_n == 0) return 0;
+  if (!det) return cxh_fail("null output array");
+  if (!q->uploaded) {
+    upload_ints(q->d_a_colind, q->a.colind);
+    upload_ints(q->d_a_row, q->a.row);
+    upload_ints(q->d_v_colind, q->v.colind);
+    upload_ints(q->d_v_row, q->v.row);
+    upload_ints(q->d_r_colind, q->r.colind);
+    upload_ints(q->d_r_row, q->r.row);
+    upload_ints(q->d_prinv, q->prinv);
+    upload_ints(q->d_pc, q->pc);
+    q->uploaded = true;
+  }
+  const int64_t ld = round_up(q->fact_n, 64);
+  hipLaunchKernelGGL(qr_det_kernel, dim3(static_cast<unsigned>((q->fact_n + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), q->v_vals.as<double>(), q->r_vals.as<double>(),
+                     q->beta.as<double>(), q->d_v_colind.as<int>(), q->d_r_colind.as<int>(), q->a.ncol, ld,
+                     q->fact_n, q->pivot_sign, det);
+  hip_check(hipGetLastError(), "qr_det_kernel launch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_qr_cache(cxh_qr q, int32_t n_cache) {
+  CXH_TRY
+  if (!q) return cxh_fail("null qr handle");
+  if (n_cache < 0) return cxh_fail("cxh_qr_cache: negative size");
+  q->cache.clear();
+  q->cache.resize(static_cast<size_t>(n_cache));
+  q->cache_loc.assign(static_cast<size_t>(n_cache), -1);
   return 0;
   CXH_CATCH
 }

@@ -120,17 +120,29 @@ int jit_map_jac(cxh_jac_s* j, int64_t n, const IOTable& arg, const IOTable& res,
                 int64_t jac_s_inst, hipStream_t s);
 
 // Batched LDL / QR factor + solve specialised to one sparsity pattern (jit.hip).
+// part: the fused factor + solve program (inputs A, B; outputs X, count), the
+// factor alone (input A; outputs the factor blocks -- L^T, D for LDL; V, R,
+// beta for QR -- then the count) or the solve alone (inputs the factor
+// blocks, then B; output X).
+enum LinsolPart { LS_FUSED = 0, LS_FACTOR = 1, LS_SOLVE = 2 };
 struct LinsolJit;
 std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colind, const std::vector<int>& a_row,
                                           const std::vector<int>& lt_colind, const std::vector<int>& lt_row,
-                                          const std::vector<int>& p, int nrhs);
+                                          const std::vector<int>& p, int nrhs, int part = LS_FUSED);
 std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<int>& a_colind,
                                          const std::vector<int>& a_row, const std::vector<int>& v_colind,
                                          const std::vector<int>& v_row, const std::vector<int>& r_colind,
                                          const std::vector<int>& r_row, const std::vector<int>& prinv,
-                                         const std::vector<int>& pc, double eps, int nrhs, bool tr);
+                                         const std::vector<int>& pc, double eps, int nrhs, bool tr,
+                                         int part = LS_FUSED);
 int jit_linsol_run(LinsolJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* x,
                    int64_t x_s, int64_t x_t, int32_t* status, hipStream_t s);
+// The factor program: fac[b] receives factor block b ([rows][ld]); status the count.
+int jit_linsol_factor(LinsolJit& L, int64_t n_inst, const double* a, int64_t a_s, int64_t a_t, double* const* fac,
+                      int n_fac, int64_t ld, int32_t* status, hipStream_t s);
+// The solve program: factor blocks fac[b] ([rows][ld]), x overwritten in place.
+int jit_linsol_solve(LinsolJit& L, int64_t n_inst, const double* const* fac, int n_fac, int64_t ld, double* x,
+                     int64_t x_s, int64_t x_t, hipStream_t s);
 std::string jit_linsol_source(LinsolJit& L, int segment);
 int jit_linsol_segments(LinsolJit& L);
 }

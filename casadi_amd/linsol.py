@@ -2,15 +2,18 @@
 batched over instances on the GPU.
 
 ``Linsol(name, solver, sp)`` (casadi/core/linsol.cpp:35-39) computes the
-symbolic factorisation once -- ``Sparsity::ldl`` with AMD for "ldl"
-(casadi/solvers/linsol_ldl.cpp:67-100), ``Sparsity::qr_sparse`` with AMD for
-"qr" (casadi/solvers/linsol_qr.cpp:67-83) -- then ``nfact`` / ``solve`` run the
-batched kernels (LinsolLdl::nfact/solve, casadi/solvers/linsol_ldl.cpp:119-132;
-LinsolQr::nfact/solve, casadi/solvers/linsol_qr.cpp:126-178).
+symbolic factorisation once -- ``Sparsity::ldl`` with AMD for "ldl", or with
+the "incomplete" option the pattern of the permuted matrix itself (no fill-in;
+casadi/solvers/linsol_ldl.cpp:67-100), ``Sparsity::qr_sparse`` with AMD for
+"qr" (casadi/solvers/linsol_qr.cpp:67-83).
 
-The numeric state follows CasADi's contract: ``nfact`` must precede ``solve``
-(casadi/core/linsol.cpp:211-218).  Factor and solve run fused in one launch
-per solve, on the values handed to the last ``nfact``.
+The numeric state follows CasADi's contract (casadi/core/linsol.cpp:104-218):
+``nfact`` factors at once and the factors stay on the device
+(``cxh_ldl_nfact`` / ``cxh_qr_nfact``), so later changes to A do not reach
+them; ``solve`` needs a preceding ``nfact`` and may be called any number of
+times on its factors (``cxh_ldl_solve`` / ``cxh_qr_solve``).  ``neig`` /
+``rank`` (ldl) and ``det`` (qr) read the factors of the last ``nfact``.
+``solve_batch`` runs factor and solve fused in one launch and keeps nothing.
 """
 import ctypes as C
 
@@ -20,6 +23,20 @@ from . import _lib as L
 from .sx import Sparsity
 
 PLUGINS = ("ldl", "qr")
+
+
+def _incomplete_ldl_pattern(sp, amd):
+    """LinsolLdl's symbolic step with "incomplete": p = AMD (or the identity)
+    and the strictly upper triangle of A(p, p), no fill-in
+    (casadi/solvers/linsol_ldl.cpp:85-95)."""
+    n = sp.size1()
+    p = np.asarray(sp.amd(), np.int64) if amd else np.arange(n, dtype=np.int64)
+    pinv = np.empty(n, np.int64)
+    pinv[p] = np.arange(n)
+    rows, cols = np.asarray(sp.row, np.int64), np.asarray(sp.get_col(), np.int64)
+    pr, pc = pinv[rows], pinv[cols]
+    keep = pr < pc
+    return Sparsity.triplet(n, n, pr[keep], pc[keep]), p
 
 
 class Linsol:
@@ -35,10 +52,15 @@ class Linsol:
             raise ValueError("Linsol requires a square matrix")
         self._h = None
         self._A = None
+        self._n_fact = None      # instances factored by the last nfact
+        self._status = None
         if solver == "ldl":
             if not sp.is_symmetric():
                 raise ValueError("LDL factorization requires a symmetric matrix")
-            self.sp_lt, self.p = sp.ldl(amd=opts.get("amd", True))
+            if opts.get("incomplete", False):
+                self.sp_lt, self.p = _incomplete_ldl_pattern(sp, opts.get("amd", True))
+            else:
+                self.sp_lt, self.p = sp.ldl(amd=opts.get("amd", True))
             spa, splt, p = (np.ascontiguousarray(v, np.int64) for v in (sp.compressed(), self.sp_lt.compressed(), self.p))
             h = L.P()
             L.check(L.cxh_ldl_create(spa.ctypes.data_as(L.PI64), splt.ctypes.data_as(L.PI64),
@@ -59,6 +81,7 @@ class Linsol:
             if engine not in ("jit", "interp"):
                 raise ValueError(f"unknown engine '{engine}' (jit, interp)")
             L.check(L.cxh_qr_engine(h, 1 if engine == "jit" else 0), "cxh_qr_engine")
+            L.check(L.cxh_qr_cache(h, int(opts.get("cache", 0))), "cxh_qr_cache")
         self._h = h
 
     def __del__(self):
@@ -95,25 +118,122 @@ class Linsol:
 
     # -- CasADi's sfact / nfact / solve contract -------------------------------
     def sfact(self, A=None):
-        """Symbolic factorisation: already done at construction (fixed pattern)."""
+        """Symbolic factorisation: done at construction (the pattern is fixed);
+        a new sfact invalidates the numeric factors (casadi/core/linsol.cpp:112-127)."""
+        self._n_fact = None
         return 0
+
+    @staticmethod
+    def _device(a):
+        import torch
+        if isinstance(a, torch.Tensor) and a.is_cuda:
+            return a
+        return torch.as_tensor(np.asarray(a, dtype=float), dtype=torch.float64, device="cuda")
 
     def nfact(self, A):
-        """Store the numeric values (nnz,) or (nnz, n_inst) for the next solve."""
-        self._A = A
+        """Numeric factorisation of one system (A: (nnz,)) or a batch (A:
+        (nnz, n_inst), SoA), factored now: the factors stay on the device
+        until the next nfact.  For "qr" a singular instance (|R_ii| < eps)
+        makes it fail, as LinsolQr::nfact does; "ldl" only counts zero pivots
+        (status())."""
+        import torch
+        At = self._device(A)
+        single = At.dim() == 1
+        At = (At.reshape(-1, 1) if single else At).contiguous()
+        if At.shape[0] != self.sp.nnz():
+            raise ValueError(f"A has {At.shape[0]} nonzeros, the pattern has {self.sp.nnz()}")
+        n_inst = At.shape[1]
+        status = torch.zeros(n_inst, dtype=torch.int32, device=At.device)
+        stream = torch.cuda.current_stream().cuda_stream
+        fn = L.cxh_ldl_nfact if self.solver == "ldl" else L.cxh_qr_nfact
+        self._n_fact = None
+        L.check(fn(self._h, n_inst, At.data_ptr(), status.data_ptr(), 0, stream), fn.__name__)
+        self._n_fact, self._single, self._status = n_inst, single, status
+        if self.solver == "qr" and int(status.max()) > 0:
+            self._n_fact = None
+            raise RuntimeError(f"'nfact' failed: {int((status > 0).sum())} of {n_inst} instances are singular")
         return 0
 
-    def solve(self, A, B=None, tr=False):
-        """solve(A, B) or, after nfact(A), solve(B): X = A^-1 B.
+    def status(self):
+        """Per instance of the last nfact: zero or non-finite pivots (ldl),
+        |R_ii| < eps entries (qr)."""
+        if self._status is None:
+            raise RuntimeError("Linear system has not been factorized")
+        return self._status.cpu().numpy()
 
-        Host arrays: A (nnz,) + B (n,) / (n, nrhs) for one system, or
-        A (nnz, n_inst) + B (n, n_inst) for a batch.  CUDA tensors in the same
-        [nonzero][instance] layout stay on the device."""
-        if B is None:
-            if self._A is None:
-                raise RuntimeError("Linsol::solve: 'nfact' must be called before 'solve'")
-            A, B = self._A, A
-        return self.solve_batch(A, B, tr=tr)[0]
+    def _require_nfact(self):
+        if self._n_fact is None:
+            raise RuntimeError("Linsol::solve: 'nfact' must be called before 'solve'")
+
+    def solve(self, A, B=None, tr=False):
+        """solve(A, B, tr): sfact, nfact and solve (Linsol::solve(DM, DM, bool),
+        casadi/core/linsol.cpp:73-99); solve(B, tr) after nfact: B (n,) or
+        (n, nrhs) for one system, (n, n_inst) or (n*nrhs, n_inst) for the
+        batch of the last nfact.  Device tensors stay on the device."""
+        import torch
+        if B is None or isinstance(B, bool):
+            tr = bool(B) if isinstance(B, bool) else tr
+            B = A
+        else:
+            self.sfact(A)
+            self.nfact(A)
+        self._require_nfact()
+        dev_in = isinstance(B, torch.Tensor) and B.is_cuda
+        Bt = self._device(B)
+        n_inst = self._n_fact
+        if self._single:
+            shape = tuple(Bt.shape)
+            X = (Bt.reshape(self.n, -1).T.reshape(-1, 1) if Bt.dim() == 2 else Bt.reshape(-1, 1)).clone()
+            nrhs = X.shape[0] // self.n
+        else:
+            X = Bt.clone().contiguous()
+            nrhs = X.shape[0] // self.n
+            if X.shape[1] != n_inst:
+                raise ValueError(f"B has {X.shape[1]} instances, the last nfact factored {n_inst}")
+            if X.shape[0] != self.n * nrhs:
+                raise ValueError(f"B must have {self.n} * nrhs rows")
+        X = X.contiguous()
+        stream = torch.cuda.current_stream().cuda_stream
+        fn = L.cxh_ldl_solve if self.solver == "ldl" else L.cxh_qr_solve
+        L.check(fn(self._h, n_inst, X.data_ptr(), nrhs, int(bool(tr)), 0, stream), fn.__name__)
+        if self._single:
+            X = X.reshape(nrhs, self.n).T.reshape(shape)
+        return X if dev_in else X.cpu().numpy()
+
+    def _counts(self, fn, what):
+        import torch
+        self._require_nfact()
+        out = torch.zeros(max(self._n_fact, 1), dtype=torch.int32, device="cuda")
+        L.check(fn(self._h, out.data_ptr(), torch.cuda.current_stream().cuda_stream), what)
+        v = out[: self._n_fact].cpu().numpy()
+        return int(v[0]) if self._single else v
+
+    def neig(self, A=None):
+        """Number of negative eigenvalues (negative D entries) of the last
+        nfact's factorisation, per instance (LinsolLdl::neig)."""
+        if self.solver != "ldl":
+            raise NotImplementedError(f"'neig' not defined for {self.solver}")
+        return self._counts(L.cxh_ldl_neig, "cxh_ldl_neig")
+
+    def rank(self, A=None):
+        """Number of nonzero D entries of the last nfact's factorisation, per
+        instance (LinsolLdl::rank)."""
+        if self.solver != "ldl":
+            raise NotImplementedError(f"'rank' not defined for {self.solver}")
+        return self._counts(L.cxh_ldl_rank, "cxh_ldl_rank")
+
+    def det(self, A):
+        """Determinant: sfact, nfact, then the determinant from the QR factors
+        (Linsol::det(DM), casadi/core/linsol.cpp:195-201; LinsolQr::det)."""
+        import torch
+        if self.solver != "qr":
+            raise NotImplementedError(f"'det' not defined for {self.solver}")
+        self.sfact(A)
+        self.nfact(A)
+        out = torch.zeros(max(self._n_fact, 1), dtype=torch.float64, device="cuda")
+        L.check(L.cxh_qr_det(self._h, out.data_ptr(), torch.cuda.current_stream().cuda_stream), "cxh_qr_det")
+        v = out[: self._n_fact].cpu().numpy()
+        return float(v[0]) if self._single else v
 
     def solve_batch(self, A, B, tr=False, nrhs=1):
         """Factor + solve a batch; returns (X, status) with status[k] the
@@ -150,9 +270,3 @@ class Linsol:
         if single:
             return Xh.reshape(nrhs, self.n).T.reshape(np.shape(B)), st
         return Xh, st
-
-    def rank(self, A):
-        if self.solver != "qr":
-            raise NotImplementedError("rank is provided by the qr plugin")
-        _, st = self.solve_batch(A, np.zeros(self.n))
-        return self.n - int(np.asarray(st).reshape(-1)[0])

@@ -315,6 +315,28 @@ int cxh_ldl_destroy(cxh_ldl l);
  * Returns 0; use cxh_ldl_count_failures for the host-side count.           */
 int cxh_ldl_factor_solve(cxh_ldl l, int64_t n_inst, const double* a, double* x, int32_t nrhs,
                          int32_t* status, int32_t layout, void* stream);
+/* Numeric factorisation alone: L^T and D of n_inst instances stay in device
+ * memory owned by the handle until the next nfact (or factor + solve), so one
+ * factorisation serves any number of cxh_ldl_solve calls.
+ * status (device, may be NULL): per instance, the number of zero or
+ * non-finite pivots.
+ * Replaces: LinsolLdl::nfact (casadi/solvers/linsol_ldl.cpp:119-126,
+ *   casadi_ldl) behind Linsol::nfact (casadi/core/linsol.cpp:138-169).      */
+int cxh_ldl_nfact(cxh_ldl l, int64_t n_inst, const double* a, int32_t* status, int32_t layout, void* stream);
+/* Solve with the factors of the last cxh_ldl_nfact (same n_inst; an error if
+ * there is none, as Linsol::solve asserts is_nfact): x holds nrhs right-hand
+ * sides per instance and is overwritten by the solutions.  tr is accepted and
+ * ignored (A is symmetric), as LinsolLdl::solve does.
+ * Replaces: LinsolLdl::solve (casadi/solvers/linsol_ldl.cpp:128-132,
+ *   casadi_ldl_solve) behind Linsol::solve (casadi/core/linsol.cpp:211-218). */
+int cxh_ldl_solve(cxh_ldl l, int64_t n_inst, double* x, int32_t nrhs, int32_t tr, int32_t layout, void* stream);
+/* Per instance (device int32 arrays of the last nfact's n_inst), from the D
+ * of the last nfact: the number of negative entries (the inertia an
+ * interior-point method checks) and of nonzero entries.
+ * Replaces: LinsolLdl::neig / LinsolLdl::rank
+ *   (casadi/solvers/linsol_ldl.cpp:134-150).                                */
+int cxh_ldl_neig(cxh_ldl l, int32_t* neig, void* stream);
+int cxh_ldl_rank(cxh_ldl l, int32_t* rank, void* stream);
 /* Engine of cxh_ldl_factor_solve: CXH_ENGINE_JIT (default) compiles, at the
  * first call per nrhs, the factor + solve of this pattern into one
  * straight-line kernel (every operation of casadi_ldl / casadi_ldl_solve in
@@ -334,6 +356,27 @@ int cxh_qr_create(const int64_t* sp_a, const int64_t* sp_v, const int64_t* sp_r,
 int cxh_qr_destroy(cxh_qr q);
 int cxh_qr_factor_solve(cxh_qr q, int64_t n_inst, const double* a, double* x, int32_t nrhs,
                         int32_t tr, int32_t* status, int32_t layout, void* stream);
+/* Numeric factorisation alone, as cxh_ldl_nfact: V, R and beta stay in the
+ * handle's device memory.  status: per instance, the number of |R_ii| < eps
+ * (casadi_qr_singular; LinsolQr::nfact fails on a nonzero count).
+ * Replaces: LinsolQr::nfact (casadi/solvers/linsol_qr.cpp:126-170).         */
+int cxh_qr_nfact(cxh_qr q, int64_t n_inst, const double* a, int32_t* status, int32_t layout, void* stream);
+/* Solve with the factors of the last cxh_qr_nfact (tr: with A^T).
+ * Replaces: LinsolQr::solve (casadi/solvers/linsol_qr.cpp:172-178,
+ *   casadi_qr_solve).                                                        */
+int cxh_qr_solve(cxh_qr q, int64_t n_inst, double* x, int32_t nrhs, int32_t tr, int32_t layout, void* stream);
+/* Per instance (device doubles): the determinant of A from the factors of the
+ * last nfact, casadi_det times the sign of the row and column permutations.
+ * Replaces: LinsolQr::det (casadi/solvers/linsol_qr.cpp, casadi_det in
+ *   casadi/core/runtime/casadi_det.hpp:26-45).                               */
+int cxh_qr_det(cxh_qr q, double* det, void* stream);
+/* Remember the factorisations of the last n_cache batches (0 = none, the
+ * default): an nfact whose batch of A values equals a remembered one (same
+ * instance count and layout, every value ==) copies its factors back instead
+ * of factoring.  Only batches without a singular instance are remembered.
+ * Replaces: LinsolQr's "cache" option (casadi/solvers/linsol_qr.cpp:61-78,
+ *   126-170, casadi_cache_check), per batch instead of per matrix.           */
+int cxh_qr_cache(cxh_qr q, int32_t n_cache);
 /* Engine of cxh_qr_factor_solve, as cxh_ldl_engine: CXH_ENGINE_JIT (default)
  * compiles the factor + solve of this pattern per (nrhs, tr) into straight-
  * line kernels (every operation of casadi_qr / casadi_qr_solve in its order:

@@ -23,6 +23,9 @@
  *                       casadi_house / casadi_qr / casadi_qr_mv / casadi_qr_trs /
  *                       casadi_qr_solve / casadi_qr_singular
  *                       (casadi/core/runtime/casadi_qr.hpp:24-230)
+ *   or_qr_det            casadi_det (casadi/core/runtime/casadi_det.hpp:26-45)
+ *                       times the sign of the row and column permutations
+ *                       (LinsolQr::det, casadi/solvers/linsol_qr.cpp)
  * How each function is pinned is listed in tests/test_oracle_*.py.
  */
 #include <math.h>
@@ -677,4 +680,34 @@ int64_t or_ldl_batch(const int64_t* sp_a, const int64_t* sp_lt, const int64_t* p
     or_ldl_solve(x + k * n, 1, sp_lt, lt, d, p, w);
   }
   return fails;
+}
+
+/* Sign of a permutation: (-1)^(n - number of cycles). */
+static double or_perm_sign(const int64_t* p, int64_t n) {
+  char* seen = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
+  int64_t i, j, cycles = 0;
+  for (i = 0; i < n; ++i) {
+    if (seen[i]) continue;
+    ++cycles;
+    for (j = i; !seen[j]; j = p[j]) seen[j] = 1;
+  }
+  free(seen);
+  return (n - cycles) % 2 ? -1.0 : 1.0;
+}
+
+/* det(A) from its QR factors: the product of R's diagonal, then of
+ * 1 - beta_c * v_c'v_c over the Householder reflectors, in that order, times
+ * the sign of the row (prinv, nrow_ext entries) and column (pc) permutations. */
+double or_qr_det(const int64_t* sp_v, const double* nz_v, const int64_t* sp_r, const double* nz_r,
+                 const double* beta, const int64_t* prinv, const int64_t* pc) {
+  int64_t ncol = sp_v[1], nrow_ext = sp_v[0], c, k;
+  const int64_t *v_colind = sp_v + 2, *r_colind = sp_r + 2;
+  double det = 1, vtv;
+  for (c = 0; c < ncol; ++c) det *= nz_r[r_colind[c + 1] - 1];
+  for (c = 0; c < ncol; ++c) {
+    vtv = 0;
+    for (k = v_colind[c]; k < v_colind[c + 1]; ++k) vtv += nz_v[k] * nz_v[k];
+    det *= 1 - beta[c] * vtv;
+  }
+  return or_perm_sign(prinv, nrow_ext) * or_perm_sign(pc, ncol) * det;
 }

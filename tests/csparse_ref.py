@@ -54,6 +54,8 @@ def lib():
         PD = C.POINTER(C.c_double)
         _ldl.ref_ldl.argtypes = [PI, PD, PI, PD, PD, PI, PD]
         _ldl.ref_ldl_solve.argtypes = [PD, C.c_longlong, PI, PD, PD, PI, PD]
+        _ldl.ref_det.restype = C.c_double
+        _ldl.ref_det.argtypes = [PI, PD, PI, PD, PD]
     return _lib
 
 
@@ -147,3 +149,15 @@ def ref_ldl_solve(b, sp_lt, lt, d, p):
     _ldl.ref_ldl_solve(x.ctypes.data_as(PD), x.size // n, sp_lt.ctypes.data_as(PI), lt.ctypes.data_as(PD),
                        d.ctypes.data_as(PD), p.ctypes.data_as(PI), w.ctypes.data_as(PD))
     return x
+
+
+def ref_det(sp_v, v, sp_r, r, beta):
+    """The reference's casadi_det (casadi/core/runtime/casadi_det.hpp), without
+    the permutation sign LinsolQr::det multiplies in."""
+    lib()
+    PI = C.POINTER(C.c_longlong)
+    PD = C.POINTER(C.c_double)
+    sp_v, sp_r = np.ascontiguousarray(sp_v, np.int64), np.ascontiguousarray(sp_r, np.int64)
+    v, r, beta = (np.ascontiguousarray(x, float) for x in (v, r, beta))
+    return _ldl.ref_det(sp_v.ctypes.data_as(PI), v.ctypes.data_as(PD), sp_r.ctypes.data_as(PI), r.ctypes.data_as(PD),
+                        beta.ctypes.data_as(PD))

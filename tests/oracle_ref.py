@@ -52,6 +52,8 @@ def _load():
     lib.or_sp_forward.argtypes = [P, I64, P, P, P]
     lib.or_sp_reverse.restype = C.c_int
     lib.or_sp_reverse.argtypes = [P, I64, I64, P, P, P]
+    lib.or_qr_det.restype = C.c_double
+    lib.or_qr_det.argtypes = [PI64, PD, PI64, PD, PD, PI64, PI64]
     lib.or_ldl_batch.restype = I64
     lib.or_ldl_batch.argtypes = [PI64, PI64, PI64, I64, PD, PD, PD, C.c_int]
     return lib
@@ -268,6 +270,13 @@ def qr_singular(r, sp_r, pc, eps):
                              _pi(np.ascontiguousarray(sp_r, np.int64)), _pi(np.ascontiguousarray(pc, np.int64)),
                              eps)
     return nul, rmin.value, irmin.value
+
+
+def qr_det(sp_v, v, sp_r, r, beta, prinv, pc):
+    return lib.or_qr_det(_pi(np.ascontiguousarray(sp_v, np.int64)), _pd(np.ascontiguousarray(v, float)),
+                         _pi(np.ascontiguousarray(sp_r, np.int64)), _pd(np.ascontiguousarray(r, float)),
+                         _pd(np.ascontiguousarray(beta, float)), _pi(np.ascontiguousarray(prinv, np.int64)),
+                         _pi(np.ascontiguousarray(pc, np.int64)))
 
 
 class perturbed_transcendentals:

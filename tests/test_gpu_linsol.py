@@ -127,3 +127,167 @@ def test_linsol_host_single_solve_and_contract(cuda):
     berr = np.linalg.norm(K @ x - b) / (np.linalg.norm(K, 2) * np.linalg.norm(x) + np.linalg.norm(b))
     assert berr < 1e-11
     assert np.allclose(x, np.linalg.solve(K, b), rtol=1e-6, atol=1e-6 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("engine", ["jit", "interp"])
+def test_ldl_nfact_then_solves_bit_exact(cuda, batch, engine):
+    """cxh_ldl_nfact factors once, the factors stay on the device, and every
+    cxh_ldl_solve after it (several right-hand-side counts, repeated) gives the
+    bits of the fused factor + solve and of the oracle's casadi_ldl_solve on
+    the oracle's L^T and D.  neig and rank count the oracle's D."""
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    n_inst = 8192 if engine == "jit" else 2051
+    v = vals[:, :n_inst].copy()
+    v[:, 9] = 0.0                        # zero pivots everywhere in one instance
+    A = torch.tensor(v, dtype=torch.float64, device="cuda")
+    print(f"[case] ldl nfact/solve engine={engine} instances={n_inst}", flush=True)
+    ls = Linsol("kkt", "ldl", sp, {"engine": engine})
+    ls.nfact(A)
+    A.fill_(1.0)                         # the factors are kept, not the values
+    st = ls.status()
+    assert st[9] == sp.nrow and np.all(np.delete(st, 9) == 0)
+    rng = np.random.default_rng(seed_of(4) + 5)
+    spc, ltc = sp.compressed(), ls.sp_lt.compressed()
+    sample = rng.choice(np.delete(np.arange(n_inst), 9), 64, replace=False)
+    ref = {k: O.ldl(spc, v[:, k], ltc, ls.p) for k in sample}
+    for nrhs in (1, 3, 1):
+        B = rng.standard_normal((sp.nrow * nrhs, n_inst))
+        X = ls.solve(torch.tensor(B, dtype=torch.float64, device="cuda")).cpu().numpy()
+        Xf, _ = Linsol("f", "ldl", sp, {"engine": engine}).solve_batch(
+            torch.tensor(v, dtype=torch.float64, device="cuda"), torch.tensor(B, dtype=torch.float64, device="cuda"),
+            nrhs=nrhs)
+        assert np.array_equal(X, Xf.cpu().numpy(), equal_nan=True)
+        for k in sample[:16]:
+            lt, d = ref[k]
+            assert np.array_equal(X[:, k], O.ldl_solve(B[:, k], ltc, lt, d, ls.p)), (nrhs, k)
+    neig, rank = ls.neig(), ls.rank()
+    for k in sample:
+        d = ref[k][1]
+        assert neig[k] == int((d < 0).sum()) and rank[k] == int((d != 0).sum())
+    assert rank[9] == 0
+    # inertia of [H J'; J -1e-8 I] with H positive definite and J of full row
+    # rank: one negative eigenvalue per dual row
+    from casadi_amd import workloads as W
+    _, nw, nl = W.ocp_kkt_sparsity()
+    assert nw + nl == sp.nrow and np.all(neig[sample] == nl)
+
+
+def test_ldl_neig_is_the_inertia(cuda):
+    """Sylvester's law of inertia: the number of negative D entries equals the
+    number of negative eigenvalues of A (checked with numpy's eigenvalues)."""
+    import torch
+    from casadi_amd.linsol import Linsol
+    from casadi_amd.sx import Sparsity
+    rng = np.random.default_rng(seed_of(4) + 6)
+    n, n_inst = 10, 70
+    mats = []
+    for k in range(n_inst):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        ev = rng.uniform(0.5, 2, n) * np.where(rng.random(n) < (k % 11) / 10, -1, 1)
+        mats.append((Q * ev) @ Q.T)
+    sp = Sparsity.dense(n, n)
+    vals = np.stack([M.T.reshape(-1) for M in mats], axis=1)
+    ls = Linsol("d", "ldl", sp)
+    ls.nfact(torch.tensor(vals, dtype=torch.float64, device="cuda"))
+    neig, rank = ls.neig(), ls.rank()
+    for k, M in enumerate(mats):
+        assert neig[k] == int((np.linalg.eigvalsh(M) < 0).sum()) and rank[k] == n
+
+
+@pytest.mark.parametrize("engine", ["jit", "interp"])
+def test_qr_nfact_solve_det(cuda, batch, engine):
+    """cxh_qr_nfact / cxh_qr_solve (both directions) give the fused call's
+    bits; cxh_qr_det gives the oracle's casadi_det times the permutation
+    sign bit for bit."""
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    n_inst = 8192 if engine == "jit" else 1031
+    A = torch.tensor(vals[:, :n_inst], dtype=torch.float64, device="cuda")
+    print(f"[case] qr nfact/solve/det engine={engine} instances={n_inst}", flush=True)
+    ls = Linsol("kkt", "qr", sp, {"engine": engine})
+    ls.nfact(A)
+    spc, svc, src = sp.compressed(), ls.sp_v.compressed(), ls.sp_r.compressed()
+    sample = np.random.default_rng(2).choice(n_inst, 16, replace=False)
+    for tr in (False, True, False):
+        B = torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda")
+        X = ls.solve(B, tr).cpu().numpy()
+        Xf, _ = Linsol("f", "qr", sp, {"engine": engine}).solve_batch(A, B, tr=tr)
+        assert np.array_equal(X, Xf.cpu().numpy())
+    det = ls.det(A)
+    for k in sample:
+        v, r, beta = O.qr(spc, vals[:, k], svc, src, ls.prinv, ls.pc)
+        assert det[k] == O.qr_det(svc, v, src, r, beta, ls.prinv, ls.pc), k
+        x = O.qr_solve(rhs[:, k], 0, svc, v, src, r, beta, ls.prinv, ls.pc)
+        assert np.array_equal(ls.solve(torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda"))
+                              .cpu().numpy()[:, k], x)
+
+
+def test_qr_nfact_fails_on_singular_and_cache(cuda, batch):
+    """A singular instance makes nfact fail (LinsolQr::nfact); with the
+    "cache" option, factorisations of earlier batches come back unchanged
+    (the reference's test_cache pattern, test/python/linearsolver.py:506-532)."""
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    n_inst = 640
+    ls = Linsol("kkt", "qr", sp, {"cache": 2})
+    sing = vals[:, :n_inst].copy()
+    sing[:, 3] = 0.0
+    with pytest.raises(RuntimeError, match="nfact"):
+        ls.nfact(torch.tensor(sing, dtype=torch.float64, device="cuda"))
+    with pytest.raises(RuntimeError, match="nfact"):
+        ls.solve(torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda"))
+    batches = [vals[:, i * n_inst:(i + 1) * n_inst] for i in range(3)]
+    B = torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda")
+
+    def run(i):
+        ls.nfact(torch.tensor(batches[i], dtype=torch.float64, device="cuda"))
+        return ls.solve(B).cpu().numpy()
+    S = [None] * 3
+    S[0] = run(0)
+    assert np.array_equal(run(0), S[0])
+    S[1] = run(1)
+    assert np.array_equal(run(1), S[1]) and np.array_equal(run(0), S[0])
+    S[2] = run(2)
+    assert np.array_equal(run(2), S[2]) and np.array_equal(run(0), S[0]) and np.array_equal(run(1), S[1])
+    plain = Linsol("p", "qr", sp)
+    for i in range(3):
+        Xf, _ = plain.solve_batch(torch.tensor(batches[i], dtype=torch.float64, device="cuda"), B)
+        assert np.array_equal(S[i], Xf.cpu().numpy())
+
+
+def test_incomplete_ldl_matches_oracle(cuda, batch):
+    """The "incomplete" option (no fill-in) runs casadi_ldl on the strictly
+    upper pattern of A(p, p): bit for bit the oracle's casadi_ldl on that
+    pattern."""
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    n_inst = 700
+    ls = Linsol("inc", "ldl", sp, {"incomplete": True})
+    assert ls.sp_lt.nnz() < sp.ldl()[0].nnz()
+    ls.nfact(torch.tensor(vals[:, :n_inst], dtype=torch.float64, device="cuda"))
+    X = ls.solve(torch.tensor(rhs[:, :n_inst], dtype=torch.float64, device="cuda")).cpu().numpy()
+    spc, ltc = sp.compressed(), ls.sp_lt.compressed()
+    for k in np.random.default_rng(3).choice(n_inst, 16, replace=False):
+        lt, d = O.ldl(spc, vals[:, k], ltc, ls.p)
+        assert np.array_equal(X[:, k], O.ldl_solve(rhs[:, k], ltc, lt, d, ls.p), equal_nan=True), k
+
+
+def test_linsol_solve_contract_errors(cuda, batch):
+    import torch
+    from casadi_amd.linsol import Linsol
+    sp, vals, rhs = batch
+    ls = Linsol("kkt", "ldl", sp)
+    with pytest.raises(RuntimeError, match="nfact"):
+        ls.neig()
+    ls.nfact(torch.tensor(vals[:, :100], dtype=torch.float64, device="cuda"))
+    with pytest.raises(Exception, match="instances"):
+        ls.solve(torch.tensor(rhs[:, :99], dtype=torch.float64, device="cuda"))
+    with pytest.raises(NotImplementedError, match="det"):
+        ls.det(vals[:, 0])
+    with pytest.raises(NotImplementedError, match="neig"):
+        Linsol("q", "qr", sp).neig()
