@@ -64,6 +64,9 @@ cxh_tape_config = _fn("cxh_tape_config", I32, P, I32, I32, I32)
 cxh_map_eval = _fn("cxh_map_eval", I32, P, I64, PP, PP, I32, I64, P)
 cxh_map_eval_host = _fn("cxh_map_eval_host", I32, P, I64, PP, PP)
 cxh_map_fwd_adj = _fn("cxh_map_fwd_adj", I32, P, I64, PP, PP, PP, PP, I32, I64, I64, P)
+cxh_tape_regularity_check = _fn("cxh_tape_regularity_check", I32, P, I32, P, I64)
+cxh_tape_io = _fn("cxh_tape_io", I32, P, C.POINTER(I32), C.POINTER(I32), PI64, PI64)
+cxh_tape_deserialize = _fn("cxh_tape_deserialize", I32, C.c_char_p, I64, C.POINTER(P))
 cxh_map_eval_sum = _fn("cxh_map_eval_sum", I32, P, I64, PP, PP, C.POINTER(I32), C.POINTER(I32), I64, P)
 cxh_sp_forward = _fn("cxh_sp_forward", I32, P, I64, PP, PP, I64, P)
 cxh_tape_create_nested = _fn("cxh_tape_create_nested", I32, P, I64, I64, I32, PI64, I32, PI64, I32, PP, PP, PP,
@@ -84,6 +87,8 @@ cxh_tape_jit_prepare = _fn("cxh_tape_jit_prepare", I32, P, I64, I32, I32)
 cxh_tape_jit_source = _fn("cxh_tape_jit_source", I32, P, I64, I32, I32, C.c_char_p, I64, PI64,
                           C.POINTER(I32))
 cxh_tape_jit_plan = _fn("cxh_tape_jit_plan", I32, P, I64, PI64, PI64)
+cxh_tape_roll_plan = _fn("cxh_tape_roll_plan", I32, P, I64, PI64)
+cxh_tape_roll_source = _fn("cxh_tape_roll_source", I32, P, I64, I32, C.c_char_p, I64, PI64)
 cxh_jit_compile_check = _fn("cxh_jit_compile_check", I32, CSTR, CSTR)
 cxh_jac_jit_source = _fn("cxh_jac_jit_source", I32, P, I32, C.c_char_p, I64, PI64, C.POINTER(I32),
                          C.POINTER(I32))

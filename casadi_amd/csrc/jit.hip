@@ -34,6 +34,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -53,27 +54,15 @@
 #include "capi_common.hpp"
 #include "sx_ops.h"
 #include "tape_internal.hpp"
+#include "jit_common.hpp"
 
 namespace sxh {
 
-static const char kSxOpsSource[] =
+extern const char kSxOpsSource[] =
 #include "sx_ops_rtc.inc"
     ;
 
-// Kernel argument block; the generated source declares the same layout.
-struct CxjArgs {
-  IOTable arg, res, aseed, asens;
-  const double* st_in;
-  double* st_out;
-  const double* adj_in;
-  double* adj_out;
-  long long n, lds;
-  double* spill;
-};
-static_assert(sizeof(IOArg) == 24, "IOArg layout");
-static_assert(sizeof(CxjArgs) == 4 * CXH_MAX_IO * 24 + 7 * 8, "CxjArgs layout");
-
-static const char kPrologue[] = R"CXJ(
+extern const char kPrologue[] = R"CXJ(
 struct IOArg { const double* p; long long s_nz; long long s_inst; };
 struct IOTable { IOArg a[16]; };
 struct CxjArgs {
@@ -82,7 +71,13 @@ struct CxjArgs {
   long long n, lds;
   double* spill;
 };
+#ifdef CXJ_SOA32
+// every block of the launch is SoA (instance stride 1) and n * 8 < 2^32: the
+// row start is wave-uniform and the instance is a 32-bit byte offset
+#define CXJ_AT(t, i, j) ((const double*)((const char*)((t).a[i].p + (long long)(j) * (t).a[i].s_nz) + (unsigned)(k * 8)))
+#else
 #define CXJ_AT(t, i, j) ((t).a[i].p + (long long)(j) * (t).a[i].s_nz + k * (t).a[i].s_inst)
+#endif
 #define CXJ_IN(i, j) ((A.arg.a[i].p) ? *CXJ_AT(A.arg, i, j) : 0.0)
 #define CXJ_OUT(i, j, v) do { if (A.res.a[i].p) *const_cast<double*>(CXJ_AT(A.res, i, j)) = (v); } while (0)
 #define CXJ_SEED(i, j) ((A.aseed.a[i].p) ? *CXJ_AT(A.aseed, i, j) : 0.0)
@@ -116,7 +111,7 @@ static int trig_call_sites() {
 
 // The finished program: switches its sin/cos pairs to calls when it has more
 // than trig_call_sites() of them.
-static std::string finish_source(std::string o) {
+std::string finish_source(std::string o) {
   const int lim = trig_call_sites();
   if (lim < 0) return o;
   int64_t sites = 0;  // "  CXJ_SINCOS(" starts an emitted line; the definitions have one space
@@ -138,6 +133,7 @@ struct JitPlan {
   int64_t snap_rows = 0;  // sub-block boundary states of one reverse kernel (max over segments)
   int ipt = 1;            // instances per thread of the forward kernels
   std::vector<hipFunction_t> fwd, rev;
+  std::vector<hipFunction_t> fwd_soa;          // forward kernels for all-SoA launches (32-bit instance offsets)
   bool trig_cache = false;                     // pass 1 stores sin/cos pairs for the recomputation
   std::vector<int> n_sub;                      // sub-blocks of each segment's reverse kernel
   std::vector<hipFunction_t> snap, back, pair;  // a reverse kernel split: first pass, the rest, and
@@ -160,7 +156,7 @@ struct JitTape {
 
 // Whether reverse kernels run split in two, the halves of consecutive
 // segments in one launch (CXH_JIT_SPLIT=0 keeps one kernel per segment).
-static bool split_reverse() {
+bool split_reverse() {
   if (const char* e = std::getenv("CXH_JIT_SPLIT")) return std::atoi(e) != 0;
   return true;
 }
@@ -284,6 +280,7 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
     P.n_sub[s] = static_cast<int>(c.size()) - 1;
   }
   P.fwd.assign(P.n_seg, nullptr);
+  P.fwd_soa.assign(P.n_seg, nullptr);
   P.rev.assign(P.n_seg, nullptr);
   P.snap.assign(P.n_seg, nullptr);
   P.back.assign(P.n_seg, nullptr);
@@ -294,7 +291,7 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
 
 // ------------------------------------------------------------ codegen ---
 
-static std::string lit(double d) {
+std::string lit(double d) {
   if (std::isnan(d)) return "__builtin_nan(\"\")";
   if (std::isinf(d)) return d > 0 ? "__builtin_huge_val()" : "(-__builtin_huge_val())";
   char b[64];
@@ -603,10 +600,10 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
           }
           if (e.op == OP_IF_ELSE_ZERO) {
             acc(oy[k - t0], "sxh::sxh_if_else_zero(d" + ks + "[1], " + an + ")");
-          } else if (op_is_binary(e.op)) {
-            acc(ox[k - t0], "d" + ks + "[0] * " + an);
-            acc(oy[k - t0], "d" + ks + "[1] * " + an);
-          } else {
+          } else if (op_is_binary(e.op)) {  // a constant-zero partial makes no contribution
+            if (!op_zero_partial(e.op, 0)) acc(ox[k - t0], "d" + ks + "[0] * " + an);
+            if (!op_zero_partial(e.op, 1)) acc(oy[k - t0], "d" + ks + "[1] * " + an);
+          } else if (!op_zero_partial(e.op, 0)) {
             acc(ox[k - t0], "d" + ks + "[0] * " + an);
           }
         }
@@ -629,8 +626,10 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
 
 // One segment kernel: one lane per instance, the arguments by value.  phase
 // (reverse kernels): 0 whole, 1 first pass (cxj_snap), 2 the rest.
-static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, int phase = 0) {
-  std::string o = kSxOpsSource;
+static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, int phase = 0,
+                               bool soa32 = false) {
+  std::string o = soa32 ? "#define CXJ_SOA32\n" : "";
+  o += kSxOpsSource;
   o += kPrologue;
   o += gen_body(t, P, s, rev, "cxj_body", true, rev ? phase : 0);
   if (rev || P.ipt == 1) {
@@ -693,7 +692,7 @@ static std::vector<char> rtc_compile(const std::string& src, const std::string& 
 static std::mutex g_jit_mu;
 static std::unordered_map<std::string, hipFunction_t> g_jit_cache;
 
-static hipFunction_t jit_function(const std::string& src, const char* name) {
+hipFunction_t jit_function(const std::string& src, const char* name) {
   std::lock_guard<std::mutex> lk(g_jit_mu);
   const std::string key = std::string(name) + "\n" + src;
   auto it = g_jit_cache.find(key);
@@ -705,17 +704,26 @@ static hipFunction_t jit_function(const std::string& src, const char* name) {
       std::fclose(f);
     }
   }
+  // CXH_JIT_LOG=1: one line per compiled kernel on stderr (name, source
+  // size, compile time, scratch), for programs that take long to compile
+  const bool log = std::getenv("CXH_JIT_LOG") && std::atoi(std::getenv("CXH_JIT_LOG")) != 0;
+  if (log) std::fprintf(stderr, "[jit] compiling %s (%zu bytes of source)\n", name, src.size());
+  const auto t0 = std::chrono::steady_clock::now();
   const std::vector<char> code = rtc_compile(src, device_arch());
   hipModule_t mod;
   hip_check(hipModuleLoadData(&mod, code.data()), "hipModuleLoadData");
   hipFunction_t fn;
   hip_check(hipModuleGetFunction(&fn, mod, name), "hipModuleGetFunction");
+  if (log) {
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "[jit] %s compiled in %.1f s, scratch %d B/lane\n", name, sec, scratch_bytes(fn));
+  }
   g_jit_cache.emplace(key, fn);
   return fn;
 }
 
 // Bytes of scratch (private segment) per lane of a loaded kernel.
-static int scratch_bytes(hipFunction_t f) {
+int scratch_bytes(hipFunction_t f) {
   int v = 0;
   hip_check(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f), "hipFuncGetAttribute");
   return v;
@@ -729,7 +737,7 @@ static int scratch_bytes(hipFunction_t f) {
 // not found.  The generators keep every kernel in use free of scratch: the
 // tangent kernels lower their register budget until the compiler needs none,
 // the others raise this error with the setting to change.
-static hipFunction_t jit_function_checked(const std::string& src, const char* name, const char* remedy) {
+hipFunction_t jit_function_checked(const std::string& src, const char* name, const char* remedy) {
   hipFunction_t f = jit_function(src, name);
   const int sb = scratch_bytes(f);
   if (sb > 0)
@@ -746,6 +754,26 @@ static hipFunction_t segment_kernel(cxh_tape_s* t, JitPlan& P, int s, bool rev) 
                                    "checkpoint interval or the interpreter engine"
                                  : "use a shorter checkpoint interval or the interpreter engine");
   return f;
+}
+
+// The forward kernel of segment s for launches whose blocks are all SoA.
+static hipFunction_t segment_kernel_soa(cxh_tape_s* t, JitPlan& P, int s) {
+  hipFunction_t& f = P.fwd_soa[s];
+  if (!f)
+    f = jit_function_checked(gen_segment(t, P, s, false, 0, true), "cxj_fwd",
+                             "use a shorter checkpoint interval or the interpreter engine");
+  return f;
+}
+
+// Whether every block of a launch over n instances is SoA (instance stride 1)
+// with byte offsets below 2^32: the launch can take the 32-bit-offset kernels.
+static bool soa32_launch(const IOTable& a, size_t na, const IOTable& b, size_t nb, int64_t n) {
+  if (n <= 0 || static_cast<uint64_t>(n) * 8u >= (uint64_t(1) << 32)) return false;
+  for (size_t i = 0; i < na; ++i)
+    if (a.a[i].p && a.a[i].s_inst != 1) return false;
+  for (size_t i = 0; i < nb; ++i)
+    if (b.a[i].p && b.a[i].s_inst != 1) return false;
+  return true;
 }
 
 // The two halves of a split reverse kernel (phase 1 or 2).
@@ -785,13 +813,13 @@ static hipFunction_t pair_kernel(cxh_tape_s* t, JitPlan& P, int g) {
   return f;
 }
 
-static void launch(hipFunction_t f, int64_t n, int block, CxjArgs& A, hipStream_t s) {
+void launch(hipFunction_t f, int64_t n, int block, CxjArgs& A, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>((n + block - 1) / block);
   void* params[] = {&A};
   hip_check(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, s, params, nullptr), "jit kernel launch");
 }
 
-static void launch_pair(hipFunction_t f, int64_t n, int block, CxjArgs& A, CxjArgs& B, hipStream_t s) {
+void launch_pair(hipFunction_t f, int64_t n, int block, CxjArgs& A, CxjArgs& B, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>((n + block - 1) / block);
   void* params[] = {&A, &B};
   hip_check(hipModuleLaunchKernel(f, grid, 2, 1, block, 1, 1, 0, s, params, nullptr), "jit kernel launch");
@@ -803,6 +831,7 @@ static void launch_pair(hipFunction_t f, int64_t n, int block, CxjArgs& A, CxjAr
 static bool split_plan(const JitPlan& P) { return split_reverse() && P.n_seg > 1 && P.snap_rows > 0; }
 
 void jit_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
+  if (roll_active(t)) return roll_prepare(t, every, fwd, rev);
   JitPlan& P = plan_for(t, every);
   const bool split = split_plan(P);
   for (int s = 0; s < P.n_seg; ++s) {
@@ -818,7 +847,9 @@ void jit_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
   }
 }
 
-int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every) { return plan_for(t, every).ck_rows; }
+int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every) {
+  return roll_active(t) ? roll_ckpt_rows(t, every) : plan_for(t, every).ck_rows;
+}
 
 // Forward-only evaluation chooses its own segment length: the default one,
 // halved until no forward segment kernel needs scratch.
@@ -845,6 +876,7 @@ static JitPlan& eval_plan(cxh_tape_s* t) {
 }
 
 int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s) {
+  if (roll_active(t)) return roll_map_eval(t, n, arg, res, s);
   JitPlan& P = eval_plan(t);
   const int64_t lds = round_up(n, 64);
   CxjArgs A{};
@@ -853,8 +885,9 @@ int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& re
   A.lds = lds;
   if (P.n_seg > 1) t->jit->st.reserve(2 * static_cast<size_t>(std::max(P.max_live, 1)) * lds * sizeof(double));
   double* st = t->jit->st.as<double>();
+  const bool soa = soa32_launch(arg, t->nnz_in.size(), res, t->nnz_out.size(), n);
   for (int g = 0; g < P.n_seg; ++g) {
-    hipFunction_t f = segment_kernel(t, P, g, false);
+    hipFunction_t f = soa ? segment_kernel_soa(t, P, g) : segment_kernel(t, P, g, false);
     A.res = res;
     A.st_in = st ? st + static_cast<int64_t>(g % 2) * P.max_live * lds : nullptr;
     A.st_out = st ? st + static_cast<int64_t>((g + 1) % 2) * P.max_live * lds : nullptr;
@@ -867,6 +900,7 @@ int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& re
 // 2: reverse from the checkpoints in ckpt_ext.
 int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
             const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s) {
+  if (roll_active(t)) return roll_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
   JitPlan& P = plan_for(t, every);
   const int64_t lds = round_up(n, 64);
   double* ck = ckpt_ext;
@@ -967,7 +1001,7 @@ void jit_plan_info(cxh_tape_s* t, int64_t every, int64_t* info, int64_t* ck_off)
 
 // Operand codes (instruction indices) of the values whose sin and cos are
 // both taken in the tape: computed by one sincos call.
-static std::unordered_set<int64_t> step_trig_pairs(const cxh_tape_s* t) {
+std::unordered_set<int64_t> step_trig_pairs(const cxh_tape_s* t) {
   std::vector<int64_t> cur(t->ws, 0);
   std::unordered_map<int64_t, int> seen;
   for (size_t k = 0; k < t->alg.size(); ++k) {
@@ -986,9 +1020,9 @@ static std::unordered_set<int64_t> step_trig_pairs(const cxh_tape_s* t) {
 // Emits `const double v<k> = sin/cos(v<x>)` through sincos (one call per
 // operand, names remembered in trig) and returns true, for SIN / COS when
 // all_trig or the operand is in pairs.
-static bool emit_trig(std::string& o, const Instr& e, int64_t k, int64_t x, const std::string& xname, bool all_trig,
-                      const std::unordered_set<int64_t>& pairs,
-                      std::unordered_map<int64_t, std::pair<std::string, std::string>>& trig) {
+bool emit_trig(std::string& o, const Instr& e, int64_t k, int64_t x, const std::string& xname, bool all_trig,
+               const std::unordered_set<int64_t>& pairs,
+               std::unordered_map<int64_t, std::pair<std::string, std::string>>& trig) {
   if ((e.op != OP_SIN && e.op != OP_COS) || !(all_trig || pairs.count(x))) return false;
   auto it = trig.find(x);
   if (it == trig.end()) {
@@ -1748,7 +1782,23 @@ namespace {
 struct TapeEmit {
   std::vector<Instr> alg;
   int ws = 0;
+  // Deferred inputs: a negative handle names an input that is read afresh
+  // at every use.  A solve program that named every factor entry at its top,
+  // or kept one for its second use, would hold them live across thousands of
+  // instructions (and across segment boundaries, through memory).
+  std::vector<std::pair<int, int>> pend;  // (block, nonzero) of each handle
+  int deferred(int blk, int nz) {
+    pend.push_back({blk, nz});
+    return -static_cast<int>(pend.size());
+  }
+  int use(int x) {
+    if (x >= 0) return x;
+    const auto& b = pend[static_cast<size_t>(-x - 1)];
+    return input(b.first, b.second);
+  }
   int op(int code, int x, int y) {
+    x = use(x);
+    y = use(y);
     Instr e{};
     e.op = code;
     e.i0 = ws;
@@ -1786,6 +1836,7 @@ struct TapeEmit {
     return ws++;
   }
   void output(int blk, int nz, int x) {
+    x = use(x);
     Instr e{};
     e.op = OP_OUTPUT;
     e.i0 = blk;
@@ -1866,8 +1917,8 @@ std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colin
       e.output(1, 0, cnt);
     }
   } else {
-    for (int k = 0; k < nnz_lt; ++k) lt[k] = e.input(0, k);
-    for (int c = 0; c < n; ++c) d[c] = e.input(1, c);
+    for (int k = 0; k < nnz_lt; ++k) lt[k] = e.deferred(0, k);
+    for (int c = 0; c < n; ++c) d[c] = e.deferred(1, c);
   }
   // x = P' L'^-1 D^-1 L^-1 P b for every right-hand side
   const int rhs_blk = part == LS_SOLVE ? 2 : 1;
@@ -1906,10 +1957,13 @@ std::shared_ptr<LinsolJit> jit_ldl_create(int n, const std::vector<int>& a_colin
   L->jac.tape = &L->tape;
   L->jac.ndir = 0;
   set_io_offsets(*L);
-  L->jac.seg_len = static_cast<int64_t>(L->tape.alg.size());
   // measured on the 274 x 274 KKT batch (8192 instances = 128 wavefronts):
   // budget 160 with reloads 64 instructions ahead, 0.34 ms against 0.80 ms
-  // at the tangent sweeps' 200 / 8 (whose kernel also needed scratch)
+  // at the tangent sweeps' 200 / 8 (whose kernel also needed scratch).  The
+  // solve alone runs in stretches of 2048 operations: as one straight line
+  // its kernel took the compiler close to four minutes (register allocation
+  // over one basic block), in 2048s a few seconds each.
+  L->jac.seg_len = part == LS_SOLVE ? 2048 : static_cast<int64_t>(L->tape.alg.size());
   L->jac.reg_budget = 160;
   L->jac.prefetch = 64;
   L->jac.kernel = part == LS_FUSED ? "cxj_ldl" : part == LS_FACTOR ? "cxj_ldl_fact" : "cxj_ldl_solve";
@@ -2043,9 +2097,9 @@ std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<i
   } else {
     // the factor leaves its work vector all zero (every entry it fills is
     // cleared again), which is the state the solve starts from
-    for (int k = 0; k < nnz_v; ++k) v[k] = e.input(0, k);
-    for (int k = 0; k < nnz_r; ++k) r[k] = e.input(1, k);
-    for (int c = 0; c < ncol; ++c) beta[c] = e.input(2, c);
+    for (int k = 0; k < nnz_v; ++k) v[k] = e.deferred(0, k);
+    for (int k = 0; k < nnz_r; ++k) r[k] = e.deferred(1, k);
+    for (int c = 0; c < ncol; ++c) beta[c] = e.deferred(2, c);
   }
   std::vector<int>& w = x;  // the solve reuses the factor's work vector, as the kernel does
   const int rhs_blk = part == LS_SOLVE ? 3 : 1;
@@ -2094,7 +2148,9 @@ std::shared_ptr<LinsolJit> jit_qr_create(int ncol, int nrow, const std::vector<i
   L->jac.tape = &L->tape;
   L->jac.ndir = 0;
   set_io_offsets(*L);
-  L->jac.seg_len = 16384;
+  // the solve alone: 4096 operations a segment compile in seconds, 16384 took
+  // the compiler close to four minutes (register allocation over one block)
+  L->jac.seg_len = part == LS_SOLVE ? 4096 : 16384;
   L->jac.reg_budget = 160;
   L->jac.prefetch = 64;
   L->jac.kernel = part == LS_FUSED ? "cxj_qr" : part == LS_FACTOR ? "cxj_qr_fact" : "cxj_qr_solve";
@@ -2285,10 +2341,10 @@ static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
         }
         if (e.op == OP_IF_ELSE_ZERO) {
           acc(oy[k], "sxh::sxh_if_else_zero(d" + ks + "[1], a" + ks + ")");
-        } else if (op_is_binary(e.op)) {
-          acc(ox[k], "d" + ks + "[0] * a" + ks);
-          acc(oy[k], "d" + ks + "[1] * a" + ks);
-        } else {
+        } else if (op_is_binary(e.op)) {  // a constant-zero partial makes no contribution
+          if (!op_zero_partial(e.op, 0)) acc(ox[k], "d" + ks + "[0] * a" + ks);
+          if (!op_zero_partial(e.op, 1)) acc(oy[k], "d" + ks + "[1] * a" + ks);
+        } else if (!op_zero_partial(e.op, 0)) {
           acc(ox[k], "d" + ks + "[0] * a" + ks);
         }
       }

@@ -398,7 +398,7 @@ struct cxh_ldl_s {
   int engine = CXH_ENGINE_JIT;                          // specialised program or the generic kernel
   std::map<int, std::shared_ptr<sxh::LinsolJit>> jit;  // specialised factor + solve per nrhs
   std::shared_ptr<sxh::LinsolJit> jit_fact;            // specialised factor alone
-  std::map<int, std::shared_ptr<sxh::LinsolJit>> jit_solve;  // specialised solve alone per nrhs
+  std::shared_ptr<sxh::LinsolJit> jit_solve;  // specialised solve alone, one right-hand side
   int64_t fact_n = -1;                                 // instances factored by the last nfact (-1: none)
 };
 
@@ -413,7 +413,7 @@ struct cxh_qr_s {
   int engine = CXH_ENGINE_JIT;                         // specialised program or the generic kernel
   std::map<int, std::shared_ptr<sxh::LinsolJit>> jit;  // specialised factor + solve per (nrhs, tr)
   std::shared_ptr<sxh::LinsolJit> jit_fact;            // specialised factor alone
-  std::map<int, std::shared_ptr<sxh::LinsolJit>> jit_solve;  // specialised solve alone per (nrhs, tr)
+  std::shared_ptr<sxh::LinsolJit> jit_solve[2];  // specialised solve alone, one right-hand side, per tr
   int64_t fact_n = -1;                                 // instances factored by the last nfact (-1: none)
   // factorisations remembered (LinsolQr's "cache" option): each holds a copy
   // of the batch of A values it was computed from, and its V, R, beta
@@ -596,13 +596,18 @@ int cxh_ldl_solve(cxh_ldl l, int64_t n_inst, double* x, int32_t nrhs, int32_t tr
   const int n = l->lt.ncol;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (l->engine == CXH_ENGINE_JIT) {
-    auto& J = l->jit_solve[nrhs];
-    if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, nrhs, LS_SOLVE);
+    auto& J = l->jit_solve;
+    if (!J) J = jit_ldl_create(n, l->a.colind, l->a.row, l->lt.colind, l->lt.row, l->p, 1, LS_SOLVE);
     const int64_t ld = round_up(n_inst, 64);
     const double* fac[2] = {l->lt_vals.as<double>(), l->d_vals.as<double>()};
     const bool soa = layout == CXH_SOA;
-    return jit_linsol_solve(*J, n_inst, fac, 2, ld, x, soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(n) * nrhs,
-                            s);
+    // the right-hand sides one after another: each is solved on its own, in
+    // the order the fused program solves it, so the bits are the same
+    for (int q = 0; q < nrhs; ++q)
+      if (int rc = jit_linsol_solve(*J, n_inst, fac, 2, ld, x + static_cast<int64_t>(q) * n * (soa ? n_inst : 1),
+                                    soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(n) * nrhs, s))
+        return rc;
+    return 0;
   }
   ldl_generic(l, LS_SOLVE, n_inst, nullptr, x, nrhs, nullptr, layout, s);
   return 0;
@@ -941,15 +946,18 @@ int cxh_qr_solve(cxh_qr q, int64_t n_inst, double* x, int32_t nrhs, int32_t tr, 
   const int ncol = q->a.ncol;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (q->engine == CXH_ENGINE_JIT) {
-    auto& J = q->jit_solve[2 * nrhs + (tr ? 1 : 0)];
+    auto& J = q->jit_solve[tr ? 1 : 0];
     if (!J)
       J = jit_qr_create(q->a.ncol, q->v.nrow, q->a.colind, q->a.row, q->v.colind, q->v.row, q->r.colind, q->r.row,
-                        q->prinv, q->pc, q->eps, nrhs, tr != 0, LS_SOLVE);
+                        q->prinv, q->pc, q->eps, 1, tr != 0, LS_SOLVE);
     const int64_t ld = round_up(n_inst, 64);
     const double* fac[3] = {q->v_vals.as<double>(), q->r_vals.as<double>(), q->beta.as<double>()};
     const bool soa = layout == CXH_SOA;
-    return jit_linsol_solve(*J, n_inst, fac, 3, ld, x, soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(ncol) * nrhs,
-                            s);
+    for (int k = 0; k < nrhs; ++k)  // one right-hand side at a time, as the fused program orders them
+      if (int rc = jit_linsol_solve(*J, n_inst, fac, 3, ld, x + static_cast<int64_t>(k) * ncol * (soa ? n_inst : 1),
+                                    soa ? n_inst : 1, soa ? 1 : static_cast<int64_t>(ncol) * nrhs, s))
+        return rc;
+    return 0;
   }
   qr_generic(q, LS_SOLVE, n_inst, nullptr, x, nrhs, tr, nullptr, layout, s);
   return 0;

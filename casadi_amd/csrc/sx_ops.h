@@ -103,11 +103,43 @@ SXH_HD bool op_is_unary(int op) {
   }
 }
 
+// Whether the partial of op with respect to operand `which` (0 or 1) is the
+// constant zero in casadi_math<SXElem>::der (casadi/core/calculus.hpp:
+// 690-1018): floor, ceil, sign, not and the comparisons and logic ops in
+// every operand, the exponent of constpow, the sign operand of copysign, the
+// second operand of printme and lift, and the condition of if_else_zero.  A
+// symbolic reverse or forward sweep multiplies by that zero SXElem, which
+// simplifies away (x * 0 -> 0, w + 0 -> w), so no contribution is made: the
+// numeric sweeps skip it too (an infinite or NaN seed times it would give
+// NaN where CasADi's derivative Functions give nothing).
+SXH_HD bool op_zero_partial(int op, int which) {
+  switch (op) {
+    case OP_FLOOR: case OP_CEIL: case OP_SIGN: case OP_NOT:
+    case OP_LT: case OP_LE: case OP_EQ: case OP_NE: case OP_AND: case OP_OR:
+      return true;
+    case OP_IF_ELSE_ZERO:
+      return which == 0;
+    case OP_CONSTPOW: case OP_COPYSIGN: case OP_PRINTME: case OP_LIFT:
+      return which == 1;
+    default:
+      return false;
+  }
+}
+
 // True for the operations the numeric VM can execute (everything but the
 // structural CONST/INPUT/OUTPUT/PARAMETER/CALL codes).
 SXH_HD bool op_is_arith(int op) { return op_is_unary(op) || op_is_binary(op); }
 
 SXH_HD double sxh_sign(double x) { return x < 0 ? -1 : x > 0 ? 1 : x; }
+
+// std::fmin / std::fmax as the reference's host library computes them: a NaN
+// operand gives the other one; otherwise the first operand when it is
+// strictly smaller (larger), else the second -- so a tie of -0 and +0 gives
+// the second operand.  IEEE 754 leaves that tie open, and the GPU's own
+// min / max instructions order -0 below +0; spelling the rule out keeps the
+// bits of the CPU evaluation.
+SXH_HD double sxh_fmin(double x, double y) { return x != x ? y : y != y ? x : (x < y ? x : y); }
+SXH_HD double sxh_fmax(double x, double y) { return x != x ? y : y != y ? x : (x > y ? x : y); }
 SXH_HD double sxh_if_else_zero(double c, double y) { return c == 0 ? 0 : y; }
 
 // Inverse error function: CasADi's own rational approximation followed by two
@@ -191,8 +223,8 @@ SXH_HD double op_fun(int op, double x, double y) {
     case OP_EQ: return x == y;
     case OP_NE: return x != y;
     case OP_IF_ELSE_ZERO: return sxh_if_else_zero(x, y);
-    case OP_FMIN: return fmin(x, y);
-    case OP_FMAX: return fmax(x, y);
+    case OP_FMIN: return sxh_fmin(x, y);
+    case OP_FMAX: return sxh_fmax(x, y);
     default: return op_fun_all(op, x, y);
   }
 }
@@ -252,8 +284,8 @@ SXH_HD double op_fun_full(int op, double x, double y) {
     case OP_COPYSIGN: return copysign(x, y);
     case OP_IF_ELSE_ZERO: return sxh_if_else_zero(x, y);
     case OP_ERF: return erf(x);
-    case OP_FMIN: return fmin(x, y);
-    case OP_FMAX: return fmax(x, y);
+    case OP_FMIN: return sxh_fmin(x, y);
+    case OP_FMAX: return sxh_fmax(x, y);
     case OP_INV: return 1. / x;
     case OP_SINH: return sinh(x);
     case OP_COSH: return cosh(x);

@@ -55,6 +55,7 @@ void validate_tape(const std::vector<Instr>& a, int64_t ws, const std::vector<in
                    const std::vector<int64_t>& nnz_out);
 
 struct JitTape;
+struct RollTape;
 
 }  // namespace sxh
 
@@ -64,15 +65,22 @@ struct cxh_tape_s {
   std::vector<int64_t> nnz_in, nnz_out;
   std::vector<int2> unused_in;
   sxh::DeviceBuffer d_tape, d_unused;  // uploaded at first interpreter launch
+  sxh::DeviceBuffer d_live;            // adjoint_live bits, uploaded at the first interpreter adjoint
   bool uploaded = false;
   sxh::DeviceBuffer scratch, trail, ckpt;
   sxh::DeviceBuffer red, partial;  // summed blocks of a reduced map (mapsum.hip)
   sxh::DeviceBuffer spw;           // work words of a sparsity sweep too large for LDS (sparsity_kernels.hip)
+  // Regularity check (cxh_tape_regularity_check): the caller's per-instance
+  // status array and its capacity, and the flagged-instance counter.
+  int32_t* status = nullptr;
+  int64_t status_cap = 0;
+  sxh::DeviceBuffer status_count;
   int block = 64;
   int w_lds = 1;
   int ipw = 64;  // instances per wavefront (8, 16, 32, 64)
   int engine = 0;  // CXH_ENGINE_INTERP or CXH_ENGINE_JIT
   std::shared_ptr<sxh::JitTape> jit;  // specialised kernels (jit.hip)
+  std::shared_ptr<sxh::RollTape> roll;  // the tape rerolled into a loop, when it repeats (jit_roll.hip)
 };
 
 namespace sxh {
@@ -86,6 +94,15 @@ const Instr* device_tape(cxh_tape_s* t);
 // Evaluation and fused forward + adjoint over explicit I/O tables (any
 // per-block strides: s_inst = 0 broadcasts a block to every instance).
 int map_eval_io(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);
+
+// Per instruction (bit k of word k / 32), whether its adjoint is
+// structurally nonzero in the reverse sweep CasADi builds symbolically
+// (SXFunction::ad_reverse, casadi/core/sx_function.cpp:1454-1560, every output
+// seeded): an OUTPUT marks its operand's slot, an operation whose adjoint is
+// marked marks its operands through every partial that is not the constant
+// zero (op_zero_partial), and the mark of a slot ends where its value is
+// defined.  The reverse sweeps make no contribution from unmarked values.
+std::vector<uint32_t> adjoint_live(const std::vector<Instr>& alg, int64_t ws);
 int adj_io(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
            const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);
 
@@ -94,6 +111,14 @@ int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& re
 int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
             const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);
 int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every);
+
+// The same over a tape whose middle repeats, as one loop body (jit_roll.hip).
+bool roll_active(cxh_tape_s* t);
+int64_t roll_ckpt_rows(cxh_tape_s* t, int64_t every);
+void roll_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev);
+int roll_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);
+int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
+             const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);
 }  // namespace sxh
 
 namespace sxh {

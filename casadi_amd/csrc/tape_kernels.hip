@@ -252,6 +252,7 @@ struct AdjArgs {
   int64_t lanes;         // stride of the HBM buffers (instance slots of the grid)
   int mode;              // 0 fwd+adj, 1 fwd + all checkpoints, 2 adj from checkpoints
   int ipw;               // instances per wavefront
+  const uint32_t* live;  // bit t: instruction t's adjoint is structurally nonzero (adjoint_live)
 };
 
 // lane: index into the per-lane scratch (trail); inst: instance for I/O
@@ -284,7 +285,7 @@ __device__ __forceinline__ void rev_segment(const AdjArgs& A, W& w, W& aw, WLds 
         const double* src = A.trail + static_cast<int64_t>(c0 - t0) * A.lanes + lane;
         for (int j = 0; j < c1 - c0; ++j) tr[j] = src[static_cast<int64_t>(j) * A.lanes];
       },
-      [&](const Instr& e, int, int j) {
+      [&](const Instr& e, int t, int j) {
         if (e.op == OP_OUTPUT) {
           const IOArg s = A.aseed.a[e.i0];
           if (s.p) aw[e.i1] += *io_addr(s, e.i2, k);
@@ -308,15 +309,14 @@ __device__ __forceinline__ void rev_segment(const AdjArgs& A, W& w, W& aw, WLds 
         const double x = w[e.i1], y = w[e.i2];
         const double seed = aw[e.i0];
         aw[e.i0] = 0.0;
+        if (!((A.live[t >> 5] >> (t & 31)) & 1u)) return;  // no output reaches this value
         double d[2];
         op_der(e.op, x, y, f, d);
         if (e.op == OP_IF_ELSE_ZERO) {
           aw[e.i2] += sxh_if_else_zero(d[1], seed);
-        } else if (op_is_binary(e.op)) {
-          aw[e.i1] += d[0] * seed;
-          aw[e.i2] += d[1] * seed;
-        } else {
-          aw[e.i1] += d[0] * seed;
+        } else {  // a constant-zero partial makes no contribution
+          if (!op_zero_partial(e.op, 0)) aw[e.i1] += d[0] * seed;
+          if (op_is_binary(e.op) && !op_zero_partial(e.op, 1)) aw[e.i2] += d[1] * seed;
         }
       });
 }
@@ -567,6 +567,70 @@ GridShape grid_shape(const cxh_tape_s* t, int64_t n) {
 
 }  // namespace sxh
 
+namespace sxh {
+
+// Blocks scanned by the regularity check: what a call wrote (outputs,
+// sensitivities), with their nonzero counts.
+struct StatusIO {
+  IOTable t[2];
+  int64_t nnz[2][CXH_MAX_IO];
+  int nb[2];
+};
+
+__global__ void status_clear(int* count) { *count = 0; }
+
+// status[k]: 1 when instance k wrote a NaN, | 2 when it wrote an infinity.
+__global__ void __launch_bounds__(256) status_kernel(StatusIO io, int64_t n, int32_t* status, int* count) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int flag = 0;
+  for (int b = 0; b < 2; ++b)
+    for (int i = 0; i < io.nb[b]; ++i) {
+      const IOArg& a = io.t[b].a[i];
+      if (!a.p) continue;
+      for (int64_t j = 0; j < io.nnz[b][i]; ++j) {
+        const double v = a.p[j * a.s_nz + k * a.s_inst];
+        if (isnan(v)) flag |= 1;
+        else if (isinf(v)) flag |= 2;
+      }
+    }
+  status[k] = flag;
+  if (flag) atomicAdd(count, 1);
+}
+
+void check_status_cap(const cxh_tape_s* t, int64_t n) {
+  if (t->status && n > t->status_cap)
+    throw std::runtime_error("regularity check: the status array holds " + std::to_string(t->status_cap) +
+                             " instances, the call has " + std::to_string(n));
+}
+
+// With the regularity check on: scan the blocks, write the per-instance
+// status, synchronize and return the number of flagged instances.
+int regularity(cxh_tape_s* t, int64_t n, const IOTable& a, const std::vector<int64_t>& nnz_a, const IOTable& b,
+               const std::vector<int64_t>& nnz_b, hipStream_t s) {
+  if (!t->status || n == 0) return 0;
+  StatusIO io{};
+  io.t[0] = a;
+  io.t[1] = b;
+  io.nb[0] = static_cast<int>(nnz_a.size());
+  io.nb[1] = static_cast<int>(nnz_b.size());
+  for (size_t i = 0; i < nnz_a.size(); ++i) io.nnz[0][i] = nnz_a[i];
+  for (size_t i = 0; i < nnz_b.size(); ++i) io.nnz[1][i] = nnz_b[i];
+  t->status_count.reserve(sizeof(int));
+  int* count = t->status_count.as<int>();
+  hipLaunchKernelGGL(status_clear, dim3(1), dim3(1), 0, s, count);
+  hip_check(hipGetLastError(), "status_clear launch");
+  hipLaunchKernelGGL(status_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, io, n, t->status,
+                     count);
+  hip_check(hipGetLastError(), "status_kernel launch");
+  int h = 0;
+  hip_check(hipMemcpyAsync(&h, count, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync status count");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return h;
+}
+
+}  // namespace sxh
+
 extern "C" {
 
 int cxh_tape_create(const void* instr, int64_t n_instr, int64_t worksize, int32_t n_in,
@@ -708,6 +772,23 @@ int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw) {
   return 0;
 }
 
+int cxh_tape_io(cxh_tape t, int32_t* n_in, int32_t* n_out, int64_t* nnz_in, int64_t* nnz_out) {
+  if (!t) return cxh_fail("null tape");
+  if (n_in) *n_in = static_cast<int32_t>(t->nnz_in.size());
+  if (n_out) *n_out = static_cast<int32_t>(t->nnz_out.size());
+  for (size_t i = 0; nnz_in && i < t->nnz_in.size(); ++i) nnz_in[i] = t->nnz_in[i];
+  for (size_t i = 0; nnz_out && i < t->nnz_out.size(); ++i) nnz_out[i] = t->nnz_out[i];
+  return 0;
+}
+
+int cxh_tape_regularity_check(cxh_tape t, int32_t on, int32_t* status, int64_t cap) {
+  if (!t) return cxh_fail("null tape");
+  if (on && (!status || cap <= 0)) return cxh_fail("cxh_tape_regularity_check: a status array is needed");
+  t->status = on ? status : nullptr;
+  t->status_cap = on ? cap : 0;
+  return 0;
+}
+
 int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const* res, int32_t layout,
                  int64_t ld, void* stream) {
   CXH_TRY
@@ -715,8 +796,11 @@ int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const*
   if (n < 0) return cxh_fail("negative n");
   if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
-  return map_eval_io(t, n, make_io(arg, t->nnz_in, layout, ld), make_io(res, t->nnz_out, layout, ld),
-                     static_cast<hipStream_t>(stream));
+  check_status_cap(t, n);
+  const IOTable R = make_io(res, t->nnz_out, layout, ld);
+  const int rc = map_eval_io(t, n, make_io(arg, t->nnz_in, layout, ld), R, static_cast<hipStream_t>(stream));
+  if (rc) return rc;
+  return regularity(t, n, R, t->nnz_out, IOTable{}, {}, static_cast<hipStream_t>(stream));
   CXH_CATCH
 }
 
@@ -803,15 +887,48 @@ static int launch_adj(cxh_tape t, int mode, int64_t n, const double* const* arg,
   if (n < 0) return cxh_fail("negative n");
   if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
   if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
-  return adj_io(t, mode, n, make_io(arg, t->nnz_in, layout, ld), make_io(aseed, t->nnz_out, layout, ld),
-                make_io(res, t->nnz_out, layout, ld), make_io(asens, t->nnz_in, layout, ld), every, ckpt_ext, s);
+  check_status_cap(t, n);
+  const IOTable R = make_io(res, t->nnz_out, layout, ld), G = make_io(asens, t->nnz_in, layout, ld);
+  const int rc = adj_io(t, mode, n, make_io(arg, t->nnz_in, layout, ld), make_io(aseed, t->nnz_out, layout, ld), R,
+                        G, every, ckpt_ext, s);
+  if (rc) return rc;
+  // what the call wrote: outputs (modes 0 and 1), sensitivities (0 and 2)
+  return regularity(t, n, mode == 2 ? IOTable{} : R, t->nnz_out, mode == 1 ? IOTable{} : G, t->nnz_in, s);
+}
+
+std::vector<uint32_t> adjoint_live(const std::vector<Instr>& alg, int64_t ws) {
+  std::vector<uint32_t> bits((alg.size() + 31) / 32, 0u);
+  std::vector<char> act(static_cast<size_t>(std::max<int64_t>(ws, 1)), 0);
+  for (int64_t k = static_cast<int64_t>(alg.size()) - 1; k >= 0; --k) {
+    const Instr& e = alg[k];
+    if (e.op == OP_OUTPUT) {
+      act[e.i1] = 1;
+      continue;
+    }
+    const bool live = act[e.i0] != 0;
+    act[e.i0] = 0;
+    if (live) bits[k >> 5] |= 1u << (k & 31);
+    if (!live || e.op == OP_CONST || e.op == OP_INPUT) continue;
+    if (e.op == OP_IF_ELSE_ZERO) {
+      act[e.i2] = 1;
+    } else {
+      if (!op_zero_partial(e.op, 0)) act[e.i1] = 1;
+      if (op_is_binary(e.op) && !op_zero_partial(e.op, 1)) act[e.i2] = 1;
+    }
+  }
+  return bits;
 }
 
 int adj_io(cxh_tape t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
            const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s) {
   if (n == 0 || t->alg.empty()) return 0;
   if (t->engine == CXH_ENGINE_JIT) return jit_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
+  if (!t->d_live.ptr) {
+    const std::vector<uint32_t> bits = adjoint_live(t->alg, t->ws);
+    t->d_live.upload(bits.data(), bits.size() * sizeof(uint32_t));
+  }
   AdjArgs A{};
+  A.live = t->d_live.as<uint32_t>();
   A.tape = device_tape(t);
   A.n_instr = static_cast<int>(t->alg.size());
   A.ws = static_cast<int>(t->ws);

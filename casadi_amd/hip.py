@@ -97,6 +97,24 @@ class HipTape:
         self.set_engine(engine)
         return self
 
+    @classmethod
+    def deserialize(cls, text, block=64, w_in_lds=True, engine="interp"):
+        """A tape from CasADi's serialized SXFunction (the string
+        Function.serialize() returns), read by cxh_tape_deserialize."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        self = cls.__new__(cls)
+        self.f = None
+        h = L.P()
+        L.check(L.cxh_tape_deserialize(data, len(data), C.byref(h)), "cxh_tape_deserialize")
+        self._h = h
+        ni, no = L.I32(), L.I32()
+        a, b = (L.I64 * 16)(), (L.I64 * 16)()
+        L.check(L.cxh_tape_io(h, C.byref(ni), C.byref(no), a, b), "cxh_tape_io")
+        self.nnz_in, self.nnz_out = [int(a[i]) for i in range(ni.value)], [int(b[i]) for i in range(no.value)]
+        self.config(block, w_in_lds)
+        self.set_engine(engine)
+        return self
+
     def instructions(self):
         """The tape the kernels run (calls inlined) and its work-vector size."""
         from .function import INSTR_DTYPE
@@ -107,6 +125,35 @@ class HipTape:
             L.check(L.cxh_tape_instructions(self._h, a.ctypes.data, n.value, C.byref(n), C.byref(ws)),
                     "cxh_tape_instructions")
         return a, ws.value
+
+    def regularity_check(self, capacity=None, on=True):
+        """Per-instance regularity status (cxh_tape_regularity_check): later
+        eval / fwd_adj / fwd_ckpt / rev_ckpt calls scan what they wrote,
+        instance by instance, into ``self.status`` (int32 on the GPU: 1 NaN,
+        | 2 infinity), and raise CasadiHipError naming the flagged instances
+        when any is, as CasADi's regularity_check option raises on a NaN or
+        Inf output.  capacity: the largest batch to be checked."""
+        torch = _torch()
+        if not on:
+            L.check(L.cxh_tape_regularity_check(self._h, 0, None, 0), "cxh_tape_regularity_check")
+            self.status = None
+            return None
+        self.status = torch.zeros(int(capacity), dtype=torch.int32, device="cuda")
+        L.check(L.cxh_tape_regularity_check(self._h, 1, self.status.data_ptr(), int(capacity)),
+                "cxh_tape_regularity_check")
+        return self.status
+
+    def _flagged(self, rc, n, what):
+        """A positive return code is the count of instances the regularity
+        check flagged."""
+        if rc <= 0:
+            return
+        st = self.status[:n].cpu().numpy()
+        bad = np.flatnonzero(st)
+        kinds = {1: "NaN", 2: "Inf", 3: "NaN and Inf"}
+        head = ", ".join(f"{k} ({kinds[int(st[k])]})" for k in bad[:8])
+        raise L.CasadiHipError(f"{what}: {rc} of {n} instances wrote NaN or Inf: {head}"
+                               + (" ..." if len(bad) > 8 else ""))
 
     def set_engine(self, engine):
         """'interp': tape interpreter kernels; 'jit': straight-line kernels
@@ -130,6 +177,28 @@ class HipTape:
         L.check(L.cxh_tape_jit_source(self._h, int(checkpoint_every), int(segment), int(reverse), buf,
                                       n.value + 1, C.byref(n), C.byref(ns)), "cxh_tape_jit_source")
         return buf.value.decode(), ns.value
+
+    def roll_plan(self, checkpoint_every=0):
+        """The loop structure the specialiser found in the tape: dict of ok,
+        prelude (instructions before the loop), period (body length),
+        iterations, state (values crossing an iteration), K (iterations per
+        checkpoint segment), segments, enabled (the tape runs as the loop: CXH_JIT_ROLL
+        when the tape was first analysed, and a loop was found)."""
+        info = (L.I64 * 8)()
+        L.check(L.cxh_tape_roll_plan(self._h, int(checkpoint_every), info), "cxh_tape_roll_plan")
+        keys = ("ok", "prelude", "period", "iterations", "state", "K", "segments", "enabled")
+        return {k: int(v) for k, v in zip(keys, info)}
+
+    def roll_source(self, checkpoint_every=0, mode=0):
+        """Generated source of the rerolled kernels: mode 0 the forward chain,
+        2 the forward chain storing checkpoints, 4 the paired reverse."""
+        n = L.I64()
+        L.check(L.cxh_tape_roll_source(self._h, int(checkpoint_every), int(mode), None, 0, C.byref(n)),
+                "cxh_tape_roll_source")
+        buf = C.create_string_buffer(n.value + 1)
+        L.check(L.cxh_tape_roll_source(self._h, int(checkpoint_every), int(mode), buf, n.value + 1, C.byref(n)),
+                "cxh_tape_roll_source")
+        return buf.value.decode()
 
     def jit_plan(self, checkpoint_every=0):
         """The plan of a checkpoint interval as the drivers size their buffers
@@ -180,8 +249,9 @@ class HipTape:
         ld_o = _check_soa(out, self.nnz_out, n, "res")
         if ld_o != ld and any(k > 1 for k in self.nnz_out):
             raise ValueError("inputs and outputs must share the leading dimension")
-        L.check(L.cxh_map_eval(self._h, n, _ptrs(args), _ptrs(out), CXH_SOA, ld, _stream_ptr(stream)),
-                "cxh_map_eval")
+        rc = L.cxh_map_eval(self._h, n, _ptrs(args), _ptrs(out), CXH_SOA, ld, _stream_ptr(stream))
+        L.check(rc, "cxh_map_eval")
+        self._flagged(rc, n, "cxh_map_eval")
         return out
 
     def eval_host(self, args_aos, n):
@@ -202,8 +272,10 @@ class HipTape:
         _check_soa(seeds, self.nnz_out, n, "adj_out")
         outs = self._alloc(self.nnz_out, n, like) if res else [None] * len(self.nnz_out)
         adj = self._alloc(self.nnz_in, n, like)
-        L.check(L.cxh_map_fwd_adj(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adj), CXH_SOA, ld,
-                                  int(checkpoint_every), _stream_ptr(stream)), "cxh_map_fwd_adj")
+        rc = L.cxh_map_fwd_adj(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(outs), _ptrs(adj), CXH_SOA, ld,
+                               int(checkpoint_every), _stream_ptr(stream))
+        L.check(rc, "cxh_map_fwd_adj")
+        self._flagged(rc, n, "cxh_map_fwd_adj")
         return outs, adj
 
     # -- reduced maps (shared inputs, outputs summed over instances) ------------
@@ -378,8 +450,10 @@ class HipTape:
         ld = _check_soa(args, self.nnz_in, n, "arg")
         outs = self._alloc(self.nnz_out, n, like)
         ckpt = torch.empty(max(self.ckpt_size(n, checkpoint_every), 1), dtype=torch.float64, device=like.device)
-        L.check(L.cxh_map_fwd_ckpt(self._h, n, _ptrs(args), _ptrs(outs), CXH_SOA, ld, int(checkpoint_every),
-                                   ckpt.data_ptr(), _stream_ptr(stream)), "cxh_map_fwd_ckpt")
+        rc = L.cxh_map_fwd_ckpt(self._h, n, _ptrs(args), _ptrs(outs), CXH_SOA, ld, int(checkpoint_every),
+                                ckpt.data_ptr(), _stream_ptr(stream))
+        L.check(rc, "cxh_map_fwd_ckpt")
+        self._flagged(rc, n, "cxh_map_fwd_ckpt")
         return outs, ckpt
 
     def rev_ckpt(self, args, seeds, n, checkpoint_every, ckpt, stream=None):
@@ -387,9 +461,10 @@ class HipTape:
         ld = _check_soa(args, self.nnz_in, n, "arg")
         _check_soa(seeds, self.nnz_out, n, "adj_out")
         adj = self._alloc(self.nnz_in, n, like)
-        L.check(L.cxh_map_rev_ckpt(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(adj), CXH_SOA, ld,
-                                   int(checkpoint_every), ckpt.data_ptr(), _stream_ptr(stream)),
-                "cxh_map_rev_ckpt")
+        rc = L.cxh_map_rev_ckpt(self._h, n, _ptrs(args), _ptrs(seeds), _ptrs(adj), CXH_SOA, ld,
+                                int(checkpoint_every), ckpt.data_ptr(), _stream_ptr(stream))
+        L.check(rc, "cxh_map_rev_ckpt")
+        self._flagged(rc, n, "cxh_map_rev_ckpt")
         return adj
 
 

@@ -42,6 +42,10 @@ class MapFunction:
             self._tape = HipTape(self.f, block=self.opts.get("block", 64),
                                  w_in_lds=self.opts.get("w_in_lds", True),
                                  engine=self.opts.get("engine", "jit"))
+            if self.opts.get("regularity_check", False):
+                # CasADi's option: NaN / Inf in an output raises; here checked
+                # per instance on the GPU (cxh_tape_regularity_check)
+                self._tape.regularity_check(self.n)
         return self._tape
 
     def _aos(self, i, a):

@@ -87,6 +87,23 @@ int cxh_tape_create_nested(const void* instr, int64_t n_instr, int64_t worksize,
  * Replaces: SXFunction::instructions_sx / instruction_* accessors
  *   (casadi/core/sx_function.hpp:187-231) for a tape held by the library.   */
 int cxh_tape_instructions(cxh_tape t, void* instr, int64_t cap, int64_t* n_instr, int64_t* worksize);
+/* The tape's input and output blocks: their counts and nonzeros (arrays of
+ * at least *n_in / *n_out entries, at most 16 each; NULL skips).
+ * Replaces: FunctionInternal::n_in / n_out / nnz_in / nnz_out
+ *   (casadi/core/function_internal.hpp) for a tape held by the library.     */
+int cxh_tape_io(cxh_tape t, int32_t* n_in, int32_t* n_out, int64_t* nnz_in, int64_t* nnz_out);
+/* A tape from CasADi's serialized SXFunction: the text Function::serialize()
+ * returns (len characters, debug mode or not).  Every field of the
+ * ProtoFunction / FunctionInternal / XFunction / SXFunction bodies is read in
+ * the reference's order with its version branches, the SX graphs of the
+ * inputs and outputs are parsed, and the algorithm, worksize and the
+ * nonzeros of the inputs and outputs go to cxh_tape_create.  Refused: other
+ * Function classes, free variables, OP_CALL records, jit libraries, and any
+ * stream that does not parse to its end.
+ * Replaces: Function::deserialize for an SXFunction
+ *   (casadi/core/function.cpp:1491-1560, casadi/core/sx_function.cpp:1926-1990,
+ *   casadi/core/serializing_stream.cpp), feeding the GPU boundary.          */
+int cxh_tape_deserialize(const char* data, int64_t len, cxh_tape* out);
 /* Kernel configuration: threads per block (64/128/256), whether the
  * interpreter's work vector lives in LDS (1) or in HBM scratch (0), and
  * instances per wavefront of the interpreter (8/16/32/64; the 64/ipw lanes of
@@ -94,6 +111,21 @@ int cxh_tape_instructions(cxh_tape t, void* instr, int64_t cap, int64_t* n_instr
  * current value.  Replaces nothing in the reference: launch tuning of the
  * "hip" Map.                                                                */
 int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw);
+/* Regularity check of batched evaluation, per instance.  With on = 1, every
+ * later cxh_map_eval, cxh_map_fwd_adj, cxh_map_fwd_ckpt and cxh_map_rev_ckpt
+ * call on this tape scans what it wrote (the outputs; the input
+ * sensitivities of the adjoint calls), instance by instance, into status (a
+ * device array of cap int32): status[k] = 1 when instance k wrote a NaN,
+ * | 2 when it wrote an infinity, 0 otherwise.  The call then synchronizes its
+ * stream and returns the number of flagged instances: positive is the ABI's
+ * numerical failure count, 0 means every instance is regular.  A call over
+ * more than cap instances fails.  on = 0 turns the check off (status and
+ * cap ignored).
+ * Replaces: the error flag OmpMap::eval ORs over its instances
+ *   (casadi/core/map.cpp:371-380) and the NaN / Inf scan of the outputs under
+ *   the regularity_check option (casadi/core/function_internal.cpp:1016-1030),
+ *   here reported per instance.                                            */
+int cxh_tape_regularity_check(cxh_tape t, int32_t on, int32_t* status, int64_t cap);
 
 /* Execution engine of a tape for cxh_map_eval / cxh_map_fwd_adj /
  * cxh_ckpt_size / cxh_map_fwd_ckpt / cxh_map_rev_ckpt:
@@ -125,6 +157,23 @@ int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, i
  * gives the source of the launch pairing segment `segment`'s reverse half
  * with the first pass of the segment before it.                            */
 int cxh_tape_jit_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info, int64_t* ck_off);
+/* Introspection: the loop structure of a tape whose middle repeats one body
+ * (a recurrence unrolled into the graph: rollouts, shooting problems), which
+ * the specialised engine runs as a loop instead of straight-line segments.
+ * info[8] = {found (1/0), prelude length, body length, iterations, state
+ * entries crossing an iteration, iterations per checkpoint segment for
+ * checkpoint_every, checkpoint segments, whether the tape runs as the loop
+ * (CXH_JIT_ROLL when the tape was first analysed, and a loop was found)}.
+ * Reference: the graph order of SXFunction::init
+ *   (casadi/core/sx_function.cpp:476-841) is what makes such tapes periodic;
+ *   evaluation as in SXFunction::eval (casadi/core/sx_function.cpp:72-127).   */
+int cxh_tape_roll_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info);
+/* Introspection: generated source of the loop kernels; mode 0 the forward
+ * chain, 2 the forward chain storing checkpoints, 4 the paired reverse
+ * launch (one segment's reverse iterations beside the recompute of the
+ * segment before it).                                                       */
+int cxh_tape_roll_source(cxh_tape t, int64_t checkpoint_every, int32_t mode, char* buf, int64_t cap,
+                         int64_t* len);
 /* Introspection: generated source of the specialised tangent kernel of one
  * segment of a Jacobian plan (the bodies of all direction groups, selected by
  * blockIdx.y), with the group / segment counts.  cxh_map_jac runs these
@@ -388,8 +437,9 @@ int cxh_qr_engine(cxh_qr q, int32_t engine);
 int cxh_qr_jit_source(cxh_qr q, int32_t nrhs, int32_t tr, int32_t segment, char* buf, int64_t cap,
                       int64_t* len, int32_t* n_segments);
 
-/* Launch duration in ms of the last kernel issued through a handle-less
- * timing probe: event-timed average of `reps` launches of cxh_map_eval.    */
+/* Facts of the current device: compute units, LDS per compute unit in KiB,
+ * the device name (truncated to name_len).  Replaces nothing in the
+ * reference (launch shaping of the batched kernels).                       */
 int cxh_device_info(int32_t* n_cu, int32_t* lds_per_cu_kb, char* name, int32_t name_len);
 
 /* ------------------------------------------------------------------------ */

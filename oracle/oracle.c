@@ -392,9 +392,28 @@ int or_map_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const 
  * seed = aw[i0], aw[i0] = 0, aw[i1] += d0*seed, aw[i2] += d1*seed
  * (if_else_zero: aw[i2] += if_else_zero(d1, seed)).
  * asens[i] (nnz_in[i] each) is zeroed first; res may hold NULL entries. */
+/* Forward sweep, then the numeric reverse sweep of SXFunction::ad_reverse
+ * (casadi/core/sx_function.cpp:1454-1560).  The symbolic sweep the reference
+ * builds multiplies by constant-zero partials and by adjoints that no output
+ * reaches, and those products simplify away (x * 0 -> 0, w + 0 -> w): so
+ * here an operation contributes only while its adjoint is structurally
+ * nonzero (act, one flag per slot, set by an OUTPUT or by a contribution,
+ * cleared where the slot's value is defined), and only through partials that
+ * are not the constant zero (or_zero_partial, the reference's der). */
+static int or_zero_partial(int op, int which) {
+  switch (op) {
+    case O_FLOOR: case O_CEIL: case O_SIGN: case O_NOT:
+    case O_LT: case O_LE: case O_EQ: case O_NE: case O_AND: case O_OR:
+      return 1;
+    case O_IF_ELSE_ZERO: return which == 0;
+    case O_CONSTPOW: case O_COPYSIGN: case O_PRINTME: case O_LIFT: return which == 1;
+    default: return 0;
+  }
+}
+
 int or_adj_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const int64_t* nnz_in,
                 const double** arg, const double** aseed, double** res, double** asens,
-                double* w, double* aw, double* trail) {
+                double* w, double* aw, double* trail, char* act) {
   const or_instr* a = (const or_instr*)instr;
   int64_t k, i;
   double d[2], f, x, y, seed;
@@ -403,6 +422,7 @@ int or_adj_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const 
   for (i = 0; i < ws; ++i) {
     w[i] = 0;
     aw[i] = 0;
+    act[i] = 0;
   }
   for (k = 0; k < n_instr; ++k) {
     const or_instr* e = a + k;
@@ -419,17 +439,21 @@ int or_adj_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const 
   }
   for (k = n_instr - 1; k >= 0; --k) {
     const or_instr* e = a + k;
+    int live;
     switch (e->op) {
       case O_OUTPUT:
         if (aseed[e->i0]) aw[e->u.s.i1] += aseed[e->i0][e->u.s.i2];
+        act[e->u.s.i1] = 1;
         break;
       case O_INPUT:
         if (asens[e->u.s.i1]) asens[e->u.s.i1][e->u.s.i2] = aw[e->i0];
         aw[e->i0] = 0;
+        act[e->i0] = 0;
         w[e->i0] = trail[k];
         break;
       case O_CONST:
         aw[e->i0] = 0;
+        act[e->i0] = 0;
         w[e->i0] = trail[k];
         break;
       default:
@@ -439,14 +463,22 @@ int or_adj_eval(const void* instr, int64_t n_instr, int64_t ws, int n_in, const 
         y = w[e->u.s.i2];
         seed = aw[e->i0];
         aw[e->i0] = 0;
+        live = act[e->i0];
+        act[e->i0] = 0;
+        if (!live) break; /* no output reaches this value */
         or_der(e->op, x, y, f, d);
         if (e->op == O_IF_ELSE_ZERO) {
           aw[e->u.s.i2] += d[1] == 0 ? 0 : seed;
-        } else if (or_is_binary(e->op)) {
-          aw[e->u.s.i1] += d[0] * seed;
-          aw[e->u.s.i2] += d[1] * seed;
+          act[e->u.s.i2] = 1;
         } else {
-          aw[e->u.s.i1] += d[0] * seed; /* unary: i2 mirrors i1 */
+          if (!or_zero_partial(e->op, 0)) {
+            aw[e->u.s.i1] += d[0] * seed;
+            act[e->u.s.i1] = 1;
+          }
+          if (or_is_binary(e->op) && !or_zero_partial(e->op, 1)) {
+            aw[e->u.s.i2] += d[1] * seed;
+            act[e->u.s.i2] = 1;
+          }
         }
     }
   }
@@ -467,7 +499,7 @@ int or_adj_map(const void* instr, int64_t n_instr, int64_t ws, int n_in, const i
     const double* s[64];
     double* r[64];
     double* g[64];
-    double* buf = (double*)malloc(sizeof(double) * (size_t)(2 * ws + n_instr + 2));
+    double* buf = (double*)malloc(sizeof(double) * (size_t)(3 * ws + n_instr + 2));
     int i;
     if (!buf || n_in > 64 || n_out > 64) {
       flag = 1;
@@ -484,7 +516,8 @@ int or_adj_map(const void* instr, int64_t n_instr, int64_t ws, int n_in, const i
           s[i] = aseed[i] ? aseed[i] + k * nnz_out[i] : 0;
           r[i] = res && res[i] ? res[i] + k * nnz_out[i] : 0;
         }
-        if (or_adj_eval(instr, n_instr, ws, n_in, nnz_in, a, s, r, g, buf, buf + ws, buf + 2 * ws))
+        if (or_adj_eval(instr, n_instr, ws, n_in, nnz_in, a, s, r, g, buf, buf + ws, buf + 2 * ws,
+                        (char*)(buf + 2 * ws + n_instr + 2)))
           flag = 1;
       }
     }

@@ -120,9 +120,12 @@ def test_jit_aos_layout_and_null_blocks(ca, cuda):
     assert O.rel_err(ax.cpu().numpy(), rx.T) <= TOL and O.rel_err(au.cpu().numpy(), ru.T) <= TOL
 
 
-def test_jit_rollout_segments_match_interpreter(ca, cuda):
+@pytest.mark.parametrize("roll", ["1", "0"])
+def test_jit_rollout_segments_match_interpreter(ca, cuda, monkeypatch, roll):
     """Checkpointed fused adjoint, split forward/reverse and forward-only
-    evaluation of a 20-step rollout cut into 4 segments."""
+    evaluation of a 20-step rollout cut into 4 segments: as one loop body
+    (roll = 1, the default) and as straight-line segment kernels."""
+    monkeypatch.setenv("CXH_JIT_ROLL", roll)
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     f = W.cartpole_rollout(20)
@@ -131,8 +134,9 @@ def test_jit_rollout_segments_match_interpreter(ca, cuda):
     x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (20, n))
     seed = np.ones((1, n))
     every = (f.n_instructions() + 3) // 4
-    print(f"[case] jit rollout20 n={n} every={every}", flush=True)
+    print(f"[case] jit rollout20 n={n} every={every} roll={roll}", flush=True)
     ti, tj = HipTape(f), HipTape(f, engine="jit")
+    assert tj.roll_plan(every)["ok"] == 1 and tj.roll_plan(every)["enabled"] == int(roll)
     (Ji,), gi = ti.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
     (Jj,), gj = tj.fwd_adj(_dev(x0, us), _dev(seed), checkpoint_every=every)
     assert np.array_equal(Ji.cpu().numpy(), Jj.cpu().numpy())
@@ -186,6 +190,7 @@ def test_jit_split_reverse_ragged_segments(ca, cuda, monkeypatch, every):
     instructions) or segments of one or two sub-blocks (every = 257): the
     bits of the unsplit kernels and of the interpreter, fused and from stored
     checkpoints, on a ragged batch."""
+    monkeypatch.setenv("CXH_JIT_ROLL", "0")   # the straight-line segment kernels
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     f = W.cartpole_rollout(20)
@@ -216,6 +221,7 @@ def test_jit_instances_per_thread(ca, cuda, monkeypatch, ipt):
     instances, one block width apart.  On a ragged batch (1000 = 15 blocks of
     64 + 40) the forward evaluation, the fused adjoint and the split
     checkpointed sweep give the bits of one instance per thread."""
+    monkeypatch.setenv("CXH_JIT_ROLL", "0")   # the straight-line segment kernels
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     f = W.cartpole_rollout(20)
@@ -249,6 +255,7 @@ def test_jit_settings_keep_the_bits(ca, cuda, monkeypatch, setting):
     sin/cos recomputed instead of loaded from the first pass, and sin/cos as
     calls at every site or inline at every site (the programs compile with
     -ffp-contract=off) give the default's bits, fused and split."""
+    monkeypatch.setenv("CXH_JIT_ROLL", "0")   # the straight-line segment kernels
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     f = W.cartpole_rollout(20)

@@ -43,7 +43,8 @@ def _live_at(f, bounds):
     return out
 
 
-def test_segment_state_is_the_live_set(ca):
+def test_segment_state_is_the_live_set(ca, monkeypatch):
+    monkeypatch.setenv("CXH_JIT_ROLL", "0")   # the straight-line segments, not the loop the rollout rerolls into
     from casadi_amd import workloads as W
     from casadi_amd.hip import HipTape
     f = W.cartpole_rollout(5)

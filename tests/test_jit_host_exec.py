@@ -398,3 +398,42 @@ def test_specialised_qr_runs_on_host(tr):
         assert np.array_equal(x[:, k], xr, equal_nan=True), k
         assert cnt[0, k] == O.qr_singular(r, src_, ls.pc, ls.eps)[0]
     assert cnt[0, n - 1] == nn
+
+
+def test_soa32_forward_segments_run_on_host(monkeypatch):
+    """The forward segment kernels built for all-SoA launches (CXJ_SOA32:
+    the row start wave-uniform, the instance a 32-bit byte offset) compute
+    the same outputs and boundary states as the general ones, on a cart-pole
+    step map and the segments of a short rollout."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    monkeypatch.setenv("CXH_JIT_ROLL", "0")
+    f = W.cartpole_rollout(3)
+    every = 150
+    tape = HipTape(f, engine="jit")
+    nseg = (f.n_instructions() + every - 1) // every
+    src = [tape.jit_source(every, s, False)[0] for s in range(nseg)]
+    n = 77
+    rng = np.random.default_rng(12)
+    x0, us = rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (3, n))
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for soa in (False, True):
+            sub = os.path.join(d, "soa32" if soa else "general")
+            os.makedirs(sub)
+            libs = _compile([("#define CXJ_SOA32\n" if soa else "") + x for x in src], sub)
+            J = np.full((1, n), np.nan)
+            state = [np.zeros((64, n)) for _ in range(nseg + 1)]
+            A = CxjArgs()
+            A.n, A.lds = n, n
+            A.arg = _table([x0, us], n)
+            A.res = _table([J], n)
+            for s, (_, fn) in enumerate(libs):
+                A.st_in = state[s].ctypes.data
+                A.st_out = state[s + 1].ctypes.data
+                fn(C.byref(A), n)
+            out[soa] = (J.copy(), [x.copy() for x in state])
+    assert np.array_equal(out[True][0], out[False][0])
+    assert all(np.array_equal(a, b) for a, b in zip(out[True][1], out[False][1]))
+    (Jo,) = O.map_eval(f, n, [x0.T, us.T])
+    assert np.array_equal(out[True][0], Jo.T)

@@ -56,10 +56,12 @@ def _emu_source(src):
     src = src.replace("CXJ_GLOBAL double*", "cxj_emu::Ptr<double>")
     m = _KERNEL.search(src)
     assert m, "no kernel entry point"
-    call = f"{m.group(1)}(*A)" if m.group(2).count("CxjArgs") == 1 else f"{m.group(1)}(*A, *B)"
+    params = m.group(2)
+    call = (f"{m.group(1)}(*A" + (", *B" if params.count("CxjArgs") == 2 else "") +
+            (", seg" if "long long seg" in params else "") + ")")
     return ('#include "jit_launch_emu.h"\n' + src +
-            '\nextern "C" void emu_launch(const CxjArgs* A, const CxjArgs* B, unsigned gx, unsigned gy, '
-            f"unsigned bx) {{\n  CXJ_EMU_LAUNCH_LOOP({call});\n}}\n")
+            '\nextern "C" void emu_launch(const CxjArgs* A, const CxjArgs* B, long long seg, unsigned gx, '
+            f"unsigned gy, unsigned bx) {{\n  (void)seg;\n  CXJ_EMU_LAUNCH_LOOP({call});\n}}\n")
 
 
 class Emu:
@@ -79,7 +81,8 @@ class Emu:
         self.libs = []
         for so in sos:
             lib = C.CDLL(so)
-            lib.emu_launch.argtypes = [C.POINTER(CxjArgs), C.POINTER(CxjArgs), C.c_uint, C.c_uint, C.c_uint]
+            lib.emu_launch.argtypes = [C.POINTER(CxjArgs), C.POINTER(CxjArgs), C.c_longlong, C.c_uint, C.c_uint,
+                                       C.c_uint]
             lib.emu_add_region.argtypes = [C.c_void_p, C.c_longlong, C.c_int]
             lib.emu_errors.restype = C.c_longlong
             lib.emu_accesses.restype = C.c_longlong
@@ -95,13 +98,13 @@ class Emu:
         self.buffers.append(b)
         return b
 
-    def launch(self, i, A, n, B=None, gy=1):
+    def launch(self, i, A, n, B=None, gy=1, seg=0):
         lib = self.libs[i]
         lib.emu_clear_regions()
         for r, b in enumerate(self.buffers):
             lib.emu_add_region(b.ctypes.data, b.nbytes, r)
         lib.emu_reset_errors()
-        lib.emu_launch(C.byref(A), C.byref(B if B is not None else A), (n + BLOCK - 1) // BLOCK, gy, BLOCK)
+        lib.emu_launch(C.byref(A), C.byref(B if B is not None else A), seg, (n + BLOCK - 1) // BLOCK, gy, BLOCK)
         assert lib.emu_accesses() > 0
         return lib.emu_errors()
 

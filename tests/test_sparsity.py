@@ -160,3 +160,65 @@ def test_qr_patterns_match_reference_csparse():
         for c in range(sp.ncol):
             assert set(V.row[V.colind[c]:V.colind[c + 1]]) == set(vr[vc[c]:vc[c + 1]])
             assert set(R.row[R.colind[c]:R.colind[c + 1]]) == set(rr[rc[c]:rc[c + 1]])
+
+
+# -- colourings traced by hand through the reference's steps -------------------
+def _sym(n, edges):
+    r = list(range(n)) + [i for i, j in edges] + [j for i, j in edges]
+    c = list(range(n)) + [j for i, j in edges] + [i for i, j in edges]
+    return Sparsity.triplet(n, n, r, c)
+
+
+def _cols_of(D):
+    return [D.row[D.colind[c]:D.colind[c + 1]].tolist() for c in range(D.ncol)]
+
+
+def test_largest_first_ties_hand_traced():
+    """largest_first (casadi/core/sparsity_internal.cpp:3511-3544): a bucket
+    sort by degree filled from the last column down, then reversed, so that
+    columns come by decreasing degree and, within one degree, by increasing
+    index.  Arrow 4x4 with hub 1: degrees [2, 4, 2, 2] -> [1, 0, 2, 3].
+    Path 0-1-2-3: degrees [2, 3, 3, 2] -> [1, 2, 0, 3]."""
+    assert _sym(4, [(1, 0), (1, 2), (1, 3)]).largest_first().tolist() == [1, 0, 2, 3]
+    assert _sym(4, [(0, 1), (1, 2), (2, 3)]).largest_first().tolist() == [1, 2, 0, 3]
+
+
+def test_star_coloring_hand_traced():
+    """star_coloring with ordering 1 (casadi/core/sparsity_internal.cpp:
+    3398-3509): colour the pattern permuted by largest_first with GMP
+    Alg. 4.1, then map the colours back.
+
+    Arrow with hub 1: the permuted hub (vertex 0) takes colour 0, every leaf
+    is forbidden colour 0 only and takes colour 1, so D = [[1], [0, 2, 3]].
+    (Without the ordering the hub would be coloured second and get colour 1.)
+
+    Path 0-1-2-3, permuted by p_inv = [2, 0, 1, 3] into the path 2-0-1-3:
+    vertex 0 -> colour 0; 1 -> 1 (0 forbidden by its coloured neighbour);
+    2 -> 1 (only 0 forbidden); 3: its neighbour 1 has colour 1, and 1's
+    neighbour 0 has a neighbour (2) of colour 1, so colours 0 and 1 are both
+    forbidden (lines 12-16) -> a new colour 2.  Back in the original order:
+    columns [0, 1, 2, 3] have colours [1, 0, 1, 2], D = [[1], [0, 2], [3]]."""
+    D = _sym(4, [(1, 0), (1, 2), (1, 3)]).star_coloring()
+    assert (D.nrow, D.ncol) == (4, 2) and _cols_of(D) == [[1], [0, 2, 3]]
+    D = _sym(4, [(0, 1), (1, 2), (2, 3)]).star_coloring()
+    assert (D.nrow, D.ncol) == (4, 3) and _cols_of(D) == [[1], [0, 2], [3]]
+    # without the ordering (ordering 0), GMP Alg. 4.1 in the natural order:
+    # the hub of the arrow comes second and gets colour 1
+    D = _sym(4, [(1, 0), (1, 2), (1, 3)]).star_coloring(0)
+    assert _cols_of(D) == [[0, 2, 3], [1]]
+
+
+def test_uni_coloring_hand_traced():
+    """uni_coloring (casadi/core/sparsity_internal.cpp:2959-3046): column i
+    takes the first colour no earlier column sharing a row with it holds.
+    A = 3x4 staircase (column rows {0}, {0,1}, {1,2}, {2}): colours
+    [0, 1, 0, 1]: column 3 shares row 2 with column 2 (colour 0) only, so
+    colour 0 is forbidden and colour 1 is free.  D lists each colour's
+    columns in increasing order."""
+    A = Sparsity.triplet(3, 4, [0, 0, 1, 1, 2, 2], [0, 1, 1, 2, 2, 3])
+    D = A.uni_coloring()
+    assert (D.nrow, D.ncol) == (4, 2) and _cols_of(D) == [[0, 2], [1, 3]]
+    # columns 0 and 1 share no row: both colour 0; column 2 shares a row with
+    # each of them, so colour 0 is forbidden and it opens colour 1
+    B = Sparsity.triplet(2, 3, [0, 1, 0, 1], [0, 1, 2, 2])
+    assert _cols_of(B.uni_coloring()) == [[0, 1], [2]]

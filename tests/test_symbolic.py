@@ -47,6 +47,102 @@ def test_simplification_rules():
     assert op_of(e) == OP.IF_ELSE_ZERO
 
 
+def _deps(e):
+    """(op, constant value, dep0 node, dep1 node) of the node of scalar e."""
+    from casadi_amd import _lib as L
+    op, v, d0, d1 = L.I32(), L.D(), L.U64(), L.U64()
+    L.check(L.cxs_node_info(node(e), L.C.byref(op), L.C.byref(v), L.C.byref(d0), L.C.byref(d1)))
+    return op.value, v.value, d0.value, d1.value
+
+
+def test_more_rewrite_rules():
+    """Rules of common_simp_unary / common_simp_binary
+    (casadi/core/calculus.hpp:1812-2139) and SXElem::binary / unary
+    (casadi/core/sx_elem.cpp:189-263), each checked on the node it must
+    build: the expression the reference returns, with its operation and
+    operands.  A rule applied in the wrong branch order or not at all builds
+    a different node."""
+    x, y, z, c = (ca.SX.sym(s) for s in "xyzc")
+    nx, ny, nz = node(x), node(y), node(z)
+    # unary
+    assert _deps(ca.sq(-x))[0] == OP.SQ and _deps(ca.sq(-x))[2] == nx                  # (-x)^2 -> x^2
+    assert _deps(ca.sq(ca.fabs(x)))[0] == OP.SQ and _deps(ca.sq(ca.fabs(x)))[2] == nx   # |x|^2 -> x^2
+    assert _deps(ca.fabs(-x))[0] == OP.FABS and _deps(ca.fabs(-x))[2] == nx             # fabs(-x) -> fabs(x)
+    # fabs of a nonnegative unary result is that result (sqrt, sq, exp, not,
+    # fabs); the binary ones of the checker table (lt, hypot, ...) are not
+    # taken, since is_nonnegative asks is_unary first (sx_elem.cpp:366-374)
+    for e in (ca.sq(x), ca.sqrt(x), ca.exp(x), ca.logic_not(x)):
+        assert node(ca.fabs(e)) == node(e)
+    assert _deps(ca.fabs(x < y))[0] == OP.FABS and _deps(ca.fabs(ca.hypot(x, y)))[0] == OP.FABS
+    assert node(1 / (1 / x)) == nx                           # 1/(1/x) -> x
+    assert _deps(ca.sqrt(ca.sq(x)))[0] == OP.FABS            # sqrt(x^2) -> fabs(x)
+    assert _deps(ca.cos(-x))[0] == OP.COS and _deps(ca.cos(-x))[2] == nx                # cos(-x) -> cos(x)
+    assert _deps(ca.cos(ca.fabs(x)))[2] == nx                # cos(|x|) -> cos(x)
+    assert node(2 * (0.5 * x)) == nx                         # twice(0.5 x) -> x
+    # addition
+    assert _deps(x + (-y)) [0] == OP.SUB and _deps(x + (-y))[2:] == (nx, ny)             # x + (-y) -> x - y
+    assert _deps((-x) + y)[0] == OP.SUB and _deps((-x) + y)[2:] == (ny, nx)              # (-x) + y -> y - x
+    assert node((x - y) + y) == nx                           # (x - y) + y -> x
+    assert node(y + (x - y)) == nx                           # y + (x - y) -> x
+    assert float(ca.sq(ca.cos(x)) + ca.sq(ca.sin(x))) == 1.0  # cos^2 + sin^2 -> 1
+    op, _, d0, d1 = _deps(2 * x + x)                          # twice(x) + x -> 3*x (is_doubled)
+    assert op == OP.MUL and _node_op(d0, value=True) == (OP.CONST, 3.0) and d1 == nx
+    # le / lt decided when the difference is a nonnegative unary result:
+    # (x + z^2) - x simplifies to z^2 first
+    assert float(x <= x + ca.sq(z)) == 1.0 and float(x + ca.sq(z) < x) == 0.0
+    # subtraction
+    assert _deps(0 - x)[0] == OP.NEG                          # 0 - y -> -y
+    assert _deps(x - (-y))[0] == OP.ADD and _deps(x - (-y))[2:] == (nx, ny)              # x - (-y) -> x + y
+    assert node((x + y) - y) == nx and node((x + y) - x) == ny   # (x + y) - y -> x, (x + y) - x -> y
+    assert _deps(y - (x + y))[0] == OP.NEG and _deps(y - (x + y))[2] == nx             # y - (x + y) -> -x
+    op, _, d0, _ = _deps((-x) - y)                           # (-x) - y -> -(x + y)
+    assert op == OP.NEG
+    assert node(2 * x - x) == nx                             # 2x - x -> x (is_doubled)
+    # multiplication
+    assert _deps(x * 3.0)[0] == OP.MUL and _deps(x * 3.0)[3] == nx                      # x*c -> c*x
+    assert _deps(x * 0)[:2] == (OP.CONST, 0.0)
+    assert _deps(x * -1)[0] == OP.NEG                         # x*(-1) -> -x
+    assert _deps(x * (1 / y))[0] == OP.DIV and _deps(x * (1 / y))[2:] == (nx, ny)        # x * inv(y) -> x / y
+    assert node(5.0 * (0.2 * x)) == nx                        # 5*(0.2 x) -> x
+    assert node(0.5 * (2 * x)) == nx                          # 0.5 * twice(x) -> x
+    assert node(5.0 * (x / 5.0)) == nx                        # 5*(x/5) -> x
+    assert node((y / x) * x) == ny                            # (y/x)*x -> y
+    op, _, d0, _ = _deps((-x) * y)                            # (-x)*y -> -(x*y)
+    assert op == OP.NEG and _node_op(d0) == OP.MUL
+    op, v, d0, d1 = _deps(3.0 * (2.5 * x))                    # a*(b*x) -> (a*b)*x
+    assert op == OP.MUL and _node_op(d0, value=True) == (OP.CONST, 7.5) and d1 == nx
+    # division
+    assert float(x / x) == 1.0
+    assert _deps(x / 0.5)[0] == OP.TWICE                      # x / 0.5 -> 2x
+    assert node((2 * x) / 2.0) == nx                          # twice(x)/2 -> x
+    assert float((2 * x) / x) == 2.0                          # twice(x)/x -> 2
+    assert node((x * y) / x) == ny and node((x * y) / y) == nx
+    assert _deps(1 / x)[0] == OP.INV                          # 1/y -> inv(y)
+    assert float((-x) / x) == -1.0 and float(x / (-x)) == -1.0
+    op, _, d0, _ = _deps((-x) / y)                            # (-x)/y -> -(x/y)
+    assert op == OP.NEG and _node_op(d0) == OP.DIV
+    # powers: integer exponents become products, negative ones an inverse
+    op, _, d0, d1 = _deps(x ** 4)                             # even: (x^2)*(x^2) -> sq rule: sq(x^2)
+    assert op == OP.SQ and _node_op(d0) == OP.SQ
+    assert _deps(x ** -2)[0] == OP.INV                        # 1/pow(x, 2)
+    assert float(x ** 0) == 1.0
+    assert _deps(x ** 101)[0] == OP.CONSTPOW                  # |n| > 100: constpow
+    # comparisons, fmin / fmax, if_else_zero
+    assert float(x.eq(x)) == 1.0 and float(x.ne(x)) == 0.0
+    assert node(ca.fmin(x, x)) == nx and node(ca.fmax(x, x)) == nx
+    assert node(ca.fmin(x, float("inf"))) == nx and node(ca.fmax(float("-inf"), x)) == nx
+    assert float(ca.fmin(x, float("-inf"))) == float("-inf") and float(ca.fmax(x, float("inf"))) == float("inf")
+    assert float(ca.if_else_zero(c, 0)) == 0.0 and node(ca.if_else_zero(1.0, y)) == ny
+    assert float(ca.if_else_zero(0.0, y)) == 0.0
+    # if_else_zero(c, x) + if_else_zero(c, y) -> if_else_zero(c, x + y) (rule 2)
+    op, _, d0, d1 = _deps(ca.if_else_zero(c, x) + ca.if_else_zero(c, y))
+    assert op == OP.IF_ELSE_ZERO and d0 == node(c) and _node_op(d1) == OP.ADD
+    # logic: not(not(x)) -> x; twice of twice -> 4x
+    assert node(ca.logic_not(ca.logic_not(x))) == nx
+    op, _, d0, d1 = _deps(2 * (2 * x))
+    assert op == OP.MUL and _node_op(d0, value=True) == (OP.CONST, 4.0) and d1 == nx
+
+
 def _np_cartpole_ode(s, u):
     mc, mp, l, g = W.CP_MC, W.CP_MP, W.CP_L, W.CP_G
     th, xd, thd = s[1], s[2], s[3]
@@ -122,6 +218,44 @@ def test_tape_structure():
     # unary instructions carry i2 == i1
     un = np.isin(a["op"], [OP.SIN, OP.COS, OP.SQ, OP.NEG, OP.TWICE])
     assert np.array_equal(a["i1"][un], a["i2"][un])
+
+
+def _tape(f):
+    inv = {v: k for k, v in vars(OP).items() if not k.startswith("_")}
+    return [(inv[int(e["op"])], int(e["i0"]), int(e["i1"]), int(e["i2"])) for e in f.instructions()], f.sz_w()
+
+
+def test_tape_hand_traced():
+    """The tape SXFunction::init emits (casadi/core/sx_function.cpp:476-841),
+    traced by hand for two small Functions.
+
+    Nodes come in the depth-first order of sort_depth_first
+    (casadi/core/x_function.hpp:412-438: dependencies left to right, each
+    node once), one OUTPUT per output nonzero.  Work-vector slots are handed
+    out in that order from a LIFO stack of freed slots: an instruction first
+    releases the slots of the operands it reads for the last time, the last
+    operand first (so the first operand's slot is on top), then takes the top
+    of the stack, or a new slot when the stack is empty (lines 689-753).
+    Symbolic nodes become INPUT records after the slots are placed.
+
+    f(x, y) = [(x + y) sin(x) - y, x + y]: nodes x, y, a = x+y, b = sin x,
+    c = a b, d = c - y.  b reads x for the last time: x's slot 0 is freed and
+    reused by b; c frees b's slot 0 and takes it; d frees y's slot 1, then
+    c's slot 0 (on top) and takes 0; a keeps slot 2 until the second output.
+
+    g(x[2], y) = [(y + 3) x1]: x0 is never read (no INPUT record); the ADD
+    frees the constant's slot 1 and then y's slot 0 and takes 0; x1 takes the
+    freed slot 1."""
+    x, y = ca.SX.sym("x"), ca.SX.sym("y")
+    a = x + y
+    f = ca.Function("f", [x, y], [a * ca.sin(x) - y, a])
+    assert _tape(f) == ([("INPUT", 0, 0, 0), ("INPUT", 1, 1, 0), ("ADD", 2, 0, 1), ("SIN", 0, 0, 0),
+                         ("MUL", 0, 2, 0), ("SUB", 0, 0, 1), ("OUTPUT", 0, 0, 0), ("OUTPUT", 1, 2, 0)], 3)
+    x, y = ca.SX.sym("x", 2), ca.SX.sym("y")
+    g = ca.Function("g", [x, y], [(y + 3.0) * x[1]])
+    three = np.array([3.0]).view(np.int32)       # the CONST record's i1, i2 hold the double's two words
+    assert _tape(g) == ([("INPUT", 0, 1, 0), ("CONST", 1, int(three[0]), int(three[1])), ("ADD", 0, 0, 1),
+                         ("INPUT", 1, 0, 1), ("MUL", 0, 0, 1), ("OUTPUT", 0, 0, 0)], 2)
 
 
 def test_free_variable_error():
