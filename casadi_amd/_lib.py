@@ -68,6 +68,8 @@ cxh_tape_regularity_check = _fn("cxh_tape_regularity_check", I32, P, I32, P, I64
 cxh_tape_io = _fn("cxh_tape_io", I32, P, C.POINTER(I32), C.POINTER(I32), PI64, PI64)
 cxh_tape_deserialize = _fn("cxh_tape_deserialize", I32, C.c_char_p, I64, C.POINTER(P))
 cxh_map_eval_sum = _fn("cxh_map_eval_sum", I32, P, I64, PP, PP, C.POINTER(I32), C.POINTER(I32), I64, P)
+cxh_map_eval_sum_noalloc = _fn("cxh_map_eval_sum_noalloc", I32, P, I64, PP, PP, C.POINTER(I32), C.POINTER(I32), I64,
+                               P)
 cxh_sp_forward = _fn("cxh_sp_forward", I32, P, I64, PP, PP, I64, P)
 cxh_tape_create_nested = _fn("cxh_tape_create_nested", I32, P, I64, I64, I32, PI64, I32, PI64, I32, PP, PP, PP,
                              PP)

@@ -697,6 +697,9 @@ hipFunction_t jit_function(const std::string& src, const char* name) {
   const std::string key = std::string(name) + "\n" + src;
   auto it = g_jit_cache.find(key);
   if (it != g_jit_cache.end()) return it->second;
+  if (no_alloc_active())
+    throw std::runtime_error(std::string("no-allocation call: kernel ") + name +
+                             " is not compiled and loaded yet; make the allocating call once first");
   if (const char* dir = std::getenv("CXH_JIT_DUMP")) {
     const std::string path = std::string(dir) + "/cxj_" + std::to_string(g_jit_cache.size()) + ".hip";
     if (FILE* f = std::fopen(path.c_str(), "w")) {

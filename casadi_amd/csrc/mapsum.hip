@@ -22,17 +22,47 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "../../include/casadi_hip.h"
 #include "capi_common.hpp"
 #include "tape_internal.hpp"
 
+// Scratch of the no-allocation reduced map (cxh_map_eval_sum_noalloc): the
+// summed instance outputs ([row][round_up(n, 64)]) and the first pass's
+// partial sums.  64 MiB, held by every GPU the library runs on until the
+// process ends.  Host code never uses the array's name as an address: the
+// device address comes from hipGetSymbolAddress, per device.
+constexpr int64_t kScratchDoubles = int64_t(8) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace sxh {
 namespace {
 
 constexpr int kRedBlock = 256;        // threads per reduction block (4 wavefronts)
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+double* scratch_ptr() {
+  constexpr int kMaxDev = 64;
+  static std::mutex mu;
+  static double* ptr[kMaxDev] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDev) throw std::runtime_error("cxh_map_eval_sum_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) gave a null address");
+    ptr[dev] = static_cast<double*>(p);
+  }
+  return ptr[dev];
+}
 constexpr int64_t kRedChunk = 2048;   // instances per first-pass block
 
 // Sum over the block in a fixed order; the result is valid in thread 0.
@@ -76,14 +106,16 @@ struct Reduced {
   int64_t rows;   // nnz of the block
 };
 
-// Sums rows [0, rows) of y ([row][ld], n instances) into the caller's blocks.
-void sum_rows(cxh_tape_s* t, const double* y, int64_t ld, int64_t n, int64_t rows, const std::vector<Reduced>& blocks,
+// First-pass blocks per summed row: one per kRedChunk instances, at most 1024.
+int64_t n_chunks(int64_t n) { return std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedChunk - 1) / kRedChunk)); }
+
+// Sums rows [0, rows) of y ([row][ld], n instances) into the caller's blocks,
+// the first pass's partial sums in part (rows * n_chunks(n) doubles).
+void sum_rows(const double* y, int64_t ld, int64_t n, int64_t rows, const std::vector<Reduced>& blocks, double* part,
               hipStream_t s) {
   if (rows == 0) return;
-  const int64_t nchunk = std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedChunk - 1) / kRedChunk));
+  const int64_t nchunk = n_chunks(n);
   const int64_t chunk = std::max<int64_t>(1, (n + nchunk - 1) / nchunk);
-  t->partial.reserve(static_cast<size_t>(rows * nchunk) * sizeof(double));
-  double* part = t->partial.as<double>();
   hipLaunchKernelGGL(sum_rows_pass1, dim3(static_cast<unsigned>(nchunk), static_cast<unsigned>(rows)),
                      dim3(kRedBlock), 0, s, y, ld, n, chunk, part, nchunk);
   hip_check(hipGetLastError(), "sum_rows_pass1 launch");
@@ -131,6 +163,46 @@ void broadcast(IOTable& tab, const std::vector<int64_t>& nnz, const int32_t* fla
     }
 }
 
+// Reduced map evaluation.  With no_alloc the summed instance outputs and the
+// partial sums live in the library's static device array instead of the
+// tape's own buffers, and nothing is allocated or compiled (NoAlloc).
+int eval_sum(cxh_tape t, int64_t n, const double* const* arg, double* const* res, const int32_t* reduce_in,
+             const int32_t* reduce_out, int64_t ld, hipStream_t s, bool no_alloc) {
+  if (!t) return cxh_fail("null tape");
+  if (n < 0) return cxh_fail("negative n");
+  if (ld < n) return cxh_fail("ld < n");
+  IOTable A = make_io(arg, t->nnz_in, CXH_SOA, ld);
+  IOTable R = make_io(res, t->nnz_out, CXH_SOA, ld);
+  broadcast(A, t->nnz_in, reduce_in);
+  const int64_t ldr = round_up(std::max<int64_t>(n, 1), 64);
+  const int64_t rows = count_rows(res, t->nnz_out, reduce_out);
+  check_rows(rows);
+  double* red = nullptr;
+  double* part = nullptr;
+  if (no_alloc) {
+    const int64_t need = rows * (ldr + n_chunks(n));
+    if (need > kScratchDoubles)
+      return cxh_fail("cxh_map_eval_sum_noalloc: " + std::to_string(rows) + " summed nonzeros over " +
+                      std::to_string(n) + " instances need " + std::to_string(need) +
+                      " doubles of scratch; the static array holds " + std::to_string(kScratchDoubles));
+    double* base = scratch_ptr();
+    red = base;
+    part = base + rows * ldr;
+  } else if (rows > 0) {
+    t->red.reserve(static_cast<size_t>(rows * ldr) * sizeof(double));
+    t->partial.reserve(static_cast<size_t>(rows * n_chunks(n)) * sizeof(double));
+    red = t->red.as<double>();
+    part = t->partial.as<double>();
+  }
+  std::vector<Reduced> blocks;
+  redirect(R, res, t->nnz_out, reduce_out, red, ldr, 0, blocks);
+  std::unique_ptr<NoAlloc> guard(no_alloc ? new NoAlloc : nullptr);
+  const int rc = map_eval_io(t, n, A, R, s);
+  if (rc) return rc;
+  sum_rows(red, ldr, n, rows, blocks, part, s);
+  return 0;
+}
+
 }  // namespace
 }  // namespace sxh
 
@@ -141,23 +213,14 @@ extern "C" {
 int cxh_map_eval_sum(cxh_tape t, int64_t n, const double* const* arg, double* const* res, const int32_t* reduce_in,
                      const int32_t* reduce_out, int64_t ld, void* stream) {
   CXH_TRY
-  if (!t) return cxh_fail("null tape");
-  if (n < 0) return cxh_fail("negative n");
-  if (ld < n) return cxh_fail("ld < n");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  IOTable A = make_io(arg, t->nnz_in, CXH_SOA, ld);
-  IOTable R = make_io(res, t->nnz_out, CXH_SOA, ld);
-  broadcast(A, t->nnz_in, reduce_in);
-  const int64_t ldr = round_up(std::max<int64_t>(n, 1), 64);
-  const int64_t rows = count_rows(res, t->nnz_out, reduce_out);
-  check_rows(rows);
-  if (rows > 0) t->red.reserve(static_cast<size_t>(rows * ldr) * sizeof(double));
-  std::vector<Reduced> blocks;
-  redirect(R, res, t->nnz_out, reduce_out, t->red.as<double>(), ldr, 0, blocks);
-  const int rc = map_eval_io(t, n, A, R, s);
-  if (rc) return rc;
-  sum_rows(t, t->red.as<double>(), ldr, n, rows, blocks, s);
-  return 0;
+  return eval_sum(t, n, arg, res, reduce_in, reduce_out, ld, static_cast<hipStream_t>(stream), false);
+  CXH_CATCH
+}
+
+int cxh_map_eval_sum_noalloc(cxh_tape t, int64_t n, const double* const* arg, double* const* res,
+                             const int32_t* reduce_in, const int32_t* reduce_out, int64_t ld, void* stream) {
+  CXH_TRY
+  return eval_sum(t, n, arg, res, reduce_in, reduce_out, ld, static_cast<hipStream_t>(stream), true);
   CXH_CATCH
 }
 
@@ -184,7 +247,8 @@ int cxh_map_fwd_adj_sum(cxh_tape t, int64_t n, const double* const* arg, const d
   redirect(G, adj_in, t->nnz_in, reduce_in, t->red.as<double>(), ldr, r0, blocks);
   const int rc = adj_io(t, 0, n, A, S, R, G, checkpoint_every, nullptr, s);
   if (rc) return rc;
-  sum_rows(t, t->red.as<double>(), ldr, n, rows, blocks, s);
+  if (rows > 0) t->partial.reserve(static_cast<size_t>(rows * n_chunks(n)) * sizeof(double));
+  sum_rows(t->red.as<double>(), ldr, n, rows, blocks, t->partial.as<double>(), s);
   return 0;
   CXH_CATCH
 }

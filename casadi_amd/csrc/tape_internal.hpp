@@ -16,6 +16,18 @@ int device_lds_bytes();
 void allow_lds(const void* kernel, size_t bytes);
 int64_t round_up(int64_t a, int64_t b);
 
+// While a NoAlloc guard lives on the calling thread, every device allocation
+// and kernel compile the library would make throws instead: the
+// no-allocation entry points run the same code as their allocating
+// counterparts under it.
+struct NoAlloc {
+  NoAlloc();
+  ~NoAlloc();
+  NoAlloc(const NoAlloc&) = delete;
+  NoAlloc& operator=(const NoAlloc&) = delete;
+};
+bool no_alloc_active();
+
 // Owning device allocation that only grows.
 struct DeviceBuffer {
   void* ptr = nullptr;

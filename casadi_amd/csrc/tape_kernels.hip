@@ -48,8 +48,19 @@ void hip_check(hipError_t e, const char* what) {
 DeviceBuffer::~DeviceBuffer() {
   if (ptr) (void)hipFree(ptr);
 }
+namespace {
+thread_local int g_no_alloc = 0;
+}
+NoAlloc::NoAlloc() { ++g_no_alloc; }
+NoAlloc::~NoAlloc() { --g_no_alloc; }
+bool no_alloc_active() { return g_no_alloc > 0; }
+
 void DeviceBuffer::reserve(size_t nbytes) {
   if (nbytes <= bytes) return;
+  if (g_no_alloc)
+    throw std::runtime_error("no-allocation call: a device buffer of the tape would have to grow to " +
+                             std::to_string(nbytes) + " bytes; make the allocating call once first with a batch "
+                             "at least as large");
   if (ptr) hip_check(hipFree(ptr), "hipFree");
   ptr = nullptr;
   bytes = 0;

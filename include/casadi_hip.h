@@ -223,6 +223,24 @@ int cxh_map_fwd_adj(cxh_tape t, int64_t n, const double* const* arg,
 int cxh_map_eval_sum(cxh_tape t, int64_t n, const double* const* arg, double* const* res,
                      const int32_t* reduce_in, const int32_t* reduce_out, int64_t ld, void* stream);
 
+/* cxh_map_eval_sum without any allocation: same arguments, same results.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The summed instance outputs and the
+ * partial sums go to one static device array of 64 MiB (8 Mi doubles) held
+ * by each GPU the library runs on until the process ends, instead of the
+ * tape's own buffers: rows * (round_up(n, 64) + min(1024, ceil(n / 2048)))
+ * doubles must fit (rows = the summed output nonzeros), else the call fails
+ * with a message.  It compiles and allocates nothing: make one
+ * cxh_map_eval_sum call with the same tape and a batch at least as large
+ * first (it loads the kernels and sizes the tape's work buffers), else it
+ * fails saying so.  Every call on one GPU shares that GPU's array and the
+ * array keeps the last call's contents: calls that overlap on one GPU
+ * (other streams or threads) race.  Nothing in it needs clearing, as every
+ * call writes all that it reads.                                           */
+int cxh_map_eval_sum_noalloc(cxh_tape t, int64_t n, const double* const* arg, double* const* res,
+                             const int32_t* reduce_in, const int32_t* reduce_out, int64_t ld, void* stream);
+
 /* Fused forward + adjoint of a reduced map: the seed adj_out[j] of a reduced
  * output is nnz_out(j) doubles applied to every instance, the sensitivity
  * adj_in[i] of a reduced input is the sum over instances (nnz_in(i) doubles).
