@@ -61,6 +61,8 @@ def check(rc, what=""):
 cxh_tape_create = _fn("cxh_tape_create", I32, P, I64, I64, I32, PI64, I32, PI64, PP)
 cxh_tape_destroy = _fn("cxh_tape_destroy", I32, P)
 cxh_tape_config = _fn("cxh_tape_config", I32, P, I32, I32, I32)
+cxh_tape_ipt = _fn("cxh_tape_ipt", I32, P, I32)
+cxh_tape_tune = _fn("cxh_tape_tune", I32, P, I64, PP, PP, I32, I64, I32, P, C.POINTER(I32), C.POINTER(C.c_double))
 cxh_map_eval = _fn("cxh_map_eval", I32, P, I64, PP, PP, I32, I64, P)
 cxh_map_eval_host = _fn("cxh_map_eval_host", I32, P, I64, PP, PP)
 cxh_map_fwd_adj = _fn("cxh_map_fwd_adj", I32, P, I64, PP, PP, PP, PP, I32, I64, I64, P)

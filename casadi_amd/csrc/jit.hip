@@ -148,6 +148,17 @@ static int fwd_ipt() {
   return 1;
 }
 
+// The tape's instances per thread (cxh_tape_ipt) for a plan: forward kernels
+// generated for another count are dropped and generated again (the source
+// cache makes switching back cheap).
+static void sync_ipt(const cxh_tape_s* t, JitPlan& P) {
+  const int want = t->ipt > 0 ? t->ipt : fwd_ipt();
+  if (P.ipt == want) return;
+  P.ipt = want;
+  std::fill(P.fwd.begin(), P.fwd.end(), nullptr);
+  std::fill(P.fwd_soa.begin(), P.fwd_soa.end(), nullptr);
+}
+
 struct JitTape {
   std::map<int64_t, JitPlan> plans;  // keyed by segment length
   int64_t eval_every = 0;            // segment length of forward-only evaluation (0: not chosen yet)
@@ -285,7 +296,7 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
   P.snap.assign(P.n_seg, nullptr);
   P.back.assign(P.n_seg, nullptr);
   P.pair.assign(P.n_seg, nullptr);
-  P.ipt = fwd_ipt();
+  P.ipt = t->ipt > 0 ? t->ipt : fwd_ipt();
   return t->jit->plans.emplace(every, std::move(P)).first->second;
 }
 
@@ -881,6 +892,7 @@ static JitPlan& eval_plan(cxh_tape_s* t) {
 int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s) {
   if (roll_active(t)) return roll_map_eval(t, n, arg, res, s);
   JitPlan& P = eval_plan(t);
+  sync_ipt(t, P);
   const int64_t lds = round_up(n, 64);
   CxjArgs A{};
   A.arg = arg;
@@ -905,6 +917,7 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
             const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s) {
   if (roll_active(t)) return roll_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
   JitPlan& P = plan_for(t, every);
+  sync_ipt(t, P);
   const int64_t lds = round_up(n, 64);
   double* ck = ckpt_ext;
   if (!ck && P.ck_rows) {

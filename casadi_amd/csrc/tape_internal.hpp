@@ -90,6 +90,7 @@ struct cxh_tape_s {
   int block = 64;
   int w_lds = 1;
   int ipw = 64;  // instances per wavefront (8, 16, 32, 64)
+  int ipt = 0;   // instances per thread of the specialised forward kernels (0: CXH_JIT_IPT, else 1)
   int engine = 0;  // CXH_ENGINE_INTERP or CXH_ENGINE_JIT
   std::shared_ptr<sxh::JitTape> jit;  // specialised kernels (jit.hip)
   std::shared_ptr<sxh::RollTape> roll;  // the tape rerolled into a loop, when it repeats (jit_roll.hip)

@@ -783,6 +783,13 @@ int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw) {
   return 0;
 }
 
+int cxh_tape_ipt(cxh_tape t, int32_t ipt) {
+  if (!t) return cxh_fail("null tape");
+  if (ipt != 0 && ipt != 1 && ipt != 2 && ipt != 4 && ipt != 8) return cxh_fail("ipt must be 0, 1, 2, 4 or 8");
+  t->ipt = ipt;
+  return 0;
+}
+
 int cxh_tape_io(cxh_tape t, int32_t* n_in, int32_t* n_out, int64_t* nnz_in, int64_t* nnz_out) {
   if (!t) return cxh_fail("null tape");
   if (n_in) *n_in = static_cast<int32_t>(t->nnz_in.size());
@@ -812,6 +819,87 @@ int cxh_map_eval(cxh_tape t, int64_t n, const double* const* arg, double* const*
   const int rc = map_eval_io(t, n, make_io(arg, t->nnz_in, layout, ld), R, static_cast<hipStream_t>(stream));
   if (rc) return rc;
   return regularity(t, n, R, t->nnz_out, IOTable{}, {}, static_cast<hipStream_t>(stream));
+  CXH_CATCH
+}
+
+int cxh_tape_tune(cxh_tape t, int64_t n, const double* const* arg, double* const* res, int32_t layout, int64_t ld,
+                  int32_t reps, void* stream, int32_t* best, double* best_ms) {
+  CXH_TRY
+  if (!t) return cxh_fail("null tape");
+  if (n <= 0) return cxh_fail("cxh_tape_tune: n must be positive");
+  if (layout != CXH_SOA && layout != CXH_AOS) return cxh_fail("bad layout");
+  if (layout == CXH_SOA && ld < n) return cxh_fail("ld < n");
+  reps = std::max(reps, 1);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const IOTable A = make_io(arg, t->nnz_in, layout, ld), R = make_io(res, t->nnz_out, layout, ld);
+  // the settings each engine reads: block size, and lanes per instance
+  // (instances per wavefront) and the work vector's place for the
+  // interpreter, instances per thread for the specialised kernels
+  struct Cfg {
+    int block, w_lds, ipw, ipt;
+  };
+  std::vector<Cfg> cfgs;
+  for (int blk : {64, 128, 256}) {
+    if (t->engine == CXH_ENGINE_JIT) {
+      for (int ipt : {1, 2, 4}) cfgs.push_back({blk, t->w_lds, t->ipw, ipt});
+    } else {
+      for (int w : {1, 0})
+        for (int ipw : {64, 32, 16, 8}) cfgs.push_back({blk, w, ipw, t->ipt});
+    }
+  }
+  struct Events {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Events() {
+      hip_check(hipEventCreate(&e0), "hipEventCreate");
+      hip_check(hipEventCreate(&e1), "hipEventCreate");
+    }
+    ~Events() {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  } ev;
+  const Cfg keep{t->block, t->w_lds, t->ipw, t->ipt};
+  Cfg pick = keep;
+  float best_t = -1.0f;
+  std::string last_err;
+  for (const Cfg& c : cfgs) {
+    t->block = c.block;
+    t->w_lds = c.w_lds;
+    t->ipw = c.ipw;
+    t->ipt = c.ipt;
+    try {
+      if (int rc = map_eval_io(t, n, A, R, s)) return rc;  // warm-up (generates the kernels)
+      hip_check(hipEventRecord(ev.e0, s), "hipEventRecord");
+      for (int r = 0; r < reps; ++r)
+        if (int rc = map_eval_io(t, n, A, R, s)) return rc;
+      hip_check(hipEventRecord(ev.e1, s), "hipEventRecord");
+      hip_check(hipEventSynchronize(ev.e1), "hipEventSynchronize");
+    } catch (const std::runtime_error& e) {  // a setting whose kernel needs scratch: not a candidate
+      if (std::string(e.what()).find("spills") == std::string::npos) throw;
+      last_err = e.what();
+      continue;
+    }
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev.e0, ev.e1), "hipEventElapsedTime");
+    ms /= static_cast<float>(reps);
+    if (best_t < 0 || ms < best_t) {
+      best_t = ms;
+      pick = c;
+    }
+  }
+  t->block = pick.block;
+  t->w_lds = pick.w_lds;
+  t->ipw = pick.ipw;
+  t->ipt = pick.ipt;
+  if (best_t < 0) return cxh_fail("cxh_tape_tune: no setting could be launched: " + last_err);
+  if (best) {
+    best[0] = pick.block;
+    best[1] = pick.w_lds;
+    best[2] = pick.ipw;
+    best[3] = pick.ipt;
+  }
+  if (best_ms) *best_ms = best_t;
+  return 0;
   CXH_CATCH
 }
 

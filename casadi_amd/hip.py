@@ -228,6 +228,28 @@ class HipTape:
         if ipw != -1:
             self.ipw = int(ipw)
 
+    def ipt(self, ipt):
+        """Instances per thread of the specialised forward kernels (1/2/4/8;
+        0: the CXH_JIT_IPT setting)."""
+        L.check(L.cxh_tape_ipt(self._h, int(ipt)), "cxh_tape_ipt")
+
+    def tune(self, args, n=None, reps=5, stream=None):
+        """Autotune the map launch over these inputs (cxh_tape_tune): the
+        fastest setting stays on the tape.  Returns (setting dict, ms per
+        call).  The outputs of the timed calls are discarded."""
+        like = self._like(args)
+        n = n if n is not None else like.shape[1]
+        ld = _check_soa(args, self.nnz_in, n, "arg")
+        out = self._alloc(self.nnz_out, n, like)
+        best = (L.I32 * 4)()
+        ms = C.c_double()
+        L.check(L.cxh_tape_tune(self._h, n, _ptrs(args), _ptrs(out), CXH_SOA, ld, int(reps), _stream_ptr(stream),
+                                best, C.byref(ms)), "cxh_tape_tune")
+        self.block, self.ipw = int(best[0]), int(best[2])
+        cfg = {"block": int(best[0]), "w_in_lds": bool(best[1]), "lanes_per_instance": 64 // int(best[2]),
+               "instances_per_thread": int(best[3]) or 1}
+        return cfg, ms.value
+
     def _alloc(self, nnz, n, like):
         torch = _torch()
         return [torch.empty((k, n), dtype=torch.float64, device=like.device) if k else None for k in nnz]

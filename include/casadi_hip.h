@@ -111,6 +111,24 @@ int cxh_tape_deserialize(const char* data, int64_t len, cxh_tape* out);
  * current value.  Replaces nothing in the reference: launch tuning of the
  * "hip" Map.                                                                */
 int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw);
+/* Instances per thread of the specialised forward kernels (1/2/4/8; 0: the
+ * CXH_JIT_IPT setting, else 1): a thread evaluates that many instances, the
+ * loads of the later ones issued while the first computes.  Replaces nothing
+ * in the reference: launch tuning of the "hip" Map.                         */
+int cxh_tape_ipt(cxh_tape t, int32_t ipt);
+/* Autotuner of one map evaluation (cxh_map_eval's arguments; the outputs are
+ * written, reps + 1 times per setting).  It times every setting the tape's
+ * engine reads -- threads per block (64/128/256), then for the interpreter
+ * the work vector in LDS or HBM and the lanes per instance (64/ipw for ipw in
+ * 64/32/16/8), for the specialised kernels the instances per thread (1/2/4)
+ * -- with events on `stream`, keeps the fastest on the tape, and returns it
+ * in best[4] = {block, w_in_lds, ipw, ipt} with its time in ms per call.
+ * Every setting gives the same bits.  A specialised setting whose kernel
+ * would need scratch is skipped.  Replaces nothing in the reference: CasADi's
+ * OmpMap leaves the split of instances over threads to OpenMP
+ * (casadi/core/map.cpp:340-386).                                            */
+int cxh_tape_tune(cxh_tape t, int64_t n, const double* const* arg, double* const* res, int32_t layout,
+                  int64_t ld, int32_t reps, void* stream, int32_t* best, double* best_ms);
 /* Regularity check of batched evaluation, per instance.  With on = 1, every
  * later cxh_map_eval, cxh_map_fwd_adj, cxh_map_fwd_ckpt and cxh_map_rev_ckpt
  * call on this tape scans what it wrote (the outputs; the input
