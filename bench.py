@@ -6,7 +6,9 @@ checkpointed reverse sweep (checkpoints every 10 steps) on the specialised
 kernels: the flat tape of the rollout repeats one step, so it runs as one
 loop body per step.  A step of the benchmark = one pass over the batch with
 inputs resident in HBM.  The line also carries the cart-pole step map over
-65536 instances and its dense Jacobian (f + jac), and both CPU baselines.
+65536 instances and its dense Jacobian (f + jac), both CPU baselines, and
+(unless --no-configs) the other workloads at their stated sizes: the
+quadrotor OCP Hessian, the heterogeneous batch and the KKT LDL / QR solves.
 
 "plain" = CasADi's way on a GPU tape interpreter: one launch of the f tape,
 one launch of its reverse Function adj1_f (casadi reverse(1)), each re-reading
@@ -40,6 +42,8 @@ def parse():
     p.add_argument("--n", type=int, default=N_BATCH)
     p.add_argument("--cpu-sample", type=int, default=16384, help="instances timed on the CPU oracle")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-configs", action="store_true",
+                   help="skip the other workloads (Hessian, heterogeneous batch, KKT solves)")
     return p.parse_args()
 
 
@@ -256,6 +260,7 @@ def main():
                    "sample": f"{reps_p} x {mc} instances, same two maps, map openmp, {nt} threads, {dt_p:.1f} s"}
 
     c1 = cartpole_map_figures(timed, args, nt if not args.no_cpu else 1)
+    others = None if args.no_configs else other_config_figures(timed)
 
     line = {
         "metric": "instances/sec fwd+adjoint, fused batched vs plain GPU tape @1e-13 rel err",
@@ -323,6 +328,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_openmp": cpu_omp,
         "cartpole_map": c1,
+        "other_configs": others,
     }
     print(json.dumps(line))
 
@@ -376,6 +382,88 @@ def cartpole_map_figures(timed, args, n_threads):
                     reps += 1
                 out[key] = {"value": reps * n / dt, "unit": "instances/s", "cores": nt, "kind": "port",
                             "sample": f"{reps} x {n} instances, oracle VM map, {dt:.1f} s"}
+    return out
+
+
+def other_config_figures(timed):
+    """The other workloads of the benchmark set, each at its stated size, on
+    the specialised kernels with the generic path beside it: the star-coloured
+    Hessian of the 20-step quadrotor OCP Lagrangian (16384 instances), the
+    heterogeneous batch (40000 cart-pole + 12000 quadrotor + 25000 arm
+    forward + adjoint, autotuned launch) and the batched KKT solve (n = 274,
+    8192 instances, ldl and qr).  first_call_s includes compiling the kernels;
+    ms are HIP-event times after it.  max_abs_diff compares the two engines
+    (the tests hold both to the oracle)."""
+    import torch
+    from casadi_amd import workloads as W
+    from casadi_amd.function import Function
+    from casadi_amd.hip import HipBatch, HipJacobian, HipTape
+    from casadi_amd.linsol import Linsol
+    from casadi_amd.sx import gradient
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    out = {}
+
+    def first(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    # config 2: Hessian of the quadrotor OCP Lagrangian
+    n = 16384
+    w, lam, Lag = W.quadrotor_ocp_lagrangian()
+    G = Function("grad_lag", [w, lam], [gradient(Lag, w)])
+    rng = np.random.default_rng(20261203)
+    args = [t(rng.uniform(-1, 1, (w.nnz(), n))), t(rng.standard_normal((lam.nnz(), n)))]
+    hj = HipJacobian(G, symmetric=True, first_block=True, engine="jit")
+    (_, hs), c_s = first(lambda: hj(args, n))
+    ms = timed(lambda: hj(args, n), 10, 1)
+    hi = HipJacobian(G, symmetric=True, first_block=True)
+    (_, hi_v), _ = first(lambda: hi(args, n))
+    ms_i = timed(lambda: hi(args, n), 2, 0)
+    out["hessian_16384"] = {"n": n, "ms": ms, "instances_per_s": n / (ms * 1e-3), "first_call_s": c_s,
+                            "colours": hj.plan["ndir"], "nnz": hj.nnz_jac, "interpreter_ms": ms_i,
+                            "max_abs_diff": float((hs - hi_v).abs().max())}
+    # config 4: heterogeneous batch
+    fs = [W.cartpole_step(), W.quadrotor_step(), W.arm_dynamics()]
+    counts = [40000, 12000, 25000]
+    rng = np.random.default_rng(20261205)
+    (B, c_s) = first(lambda: HipBatch(fs, counts))
+    data = []
+    for i, (f, m) in enumerate(zip(fs, counts)):
+        A = [t(rng.uniform(-1, 1, (f.nnz_in(j), m))) for j in range(f.n_in())]
+        S = [t(rng.standard_normal((f.nnz_out(j), m))) for j in range(f.n_out())]
+        B.bind(i, A, S)
+        data.append((A, S))
+    cfg, _ = B.tune(reps=3)
+    ms_g = timed(lambda: B.launch(), 50, 5)
+    ms_d = timed(lambda: B.launch(graph=False), 50, 5)
+    tapes = [(HipTape(f), HipTape(f.reverse(1))) for f in fs]
+
+    def plain():
+        for (tf, tr), (A, S), m in zip(tapes, data, counts):
+            tf.eval(A, m)
+            tr.eval(A + [None] * len(S) + S, m)
+    ms_p = timed(plain, 10, 1)
+    out["batch_77000"] = {"instances": sum(counts), "ms_graph": ms_g, "ms_direct": ms_d, "first_call_s": c_s,
+                          "instances_per_s": sum(counts) / (min(ms_g, ms_d) * 1e-3), "tuned": cfg,
+                          "plain_interpreter_ms": ms_p}
+    # config 5: batched KKT solves
+    n = 8192
+    sp, _, _ = W.ocp_kkt_sparsity()
+    rng = np.random.default_rng(20261206)
+    _, A = W.ocp_kkt_batch(rng, n)
+    At, Bt = t(A), t(rng.standard_normal((sp.size1(), n)))
+    for solver in ("ldl", "qr"):
+        Lj = Linsol("kkt", solver, sp, {"engine": "jit"})
+        (xj, _), c_s = first(lambda: Lj.solve_batch(At, Bt))
+        ms = timed(lambda: Lj.solve_batch(At, Bt), 10, 1)
+        Lg = Linsol("kkt", solver, sp, {"engine": "interp"})
+        (xg, _), _ = first(lambda: Lg.solve_batch(At, Bt))
+        ms_g = timed(lambda: Lg.solve_batch(At, Bt), 3, 0)
+        out[f"kkt_{solver}_8192"] = {"n": n, "ms": ms, "instances_per_s": n / (ms * 1e-3), "first_call_s": c_s,
+                                     "generic_ms": ms_g, "max_abs_diff": float((xj - xg).abs().max())}
     return out
 
 

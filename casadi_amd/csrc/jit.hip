@@ -34,6 +34,8 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -46,6 +48,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -175,7 +178,7 @@ bool split_reverse() {
 // Whether the first pass of a reverse kernel stores the sin/cos pairs of its
 // sub-blocks for their recomputation to load (CXH_JIT_TRIG_CACHE=0: computed
 // again).  Fixed per plan when the plan is made.
-static bool trig_cache_enabled() {
+bool trig_cache_enabled() {
   if (const char* e = std::getenv("CXH_JIT_TRIG_CACHE")) return std::atoi(e) != 0;
   return true;
 }
@@ -863,6 +866,141 @@ void jit_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
 
 int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every) {
   return roll_active(t) ? roll_ckpt_rows(t, every) : plan_for(t, every).ck_rows;
+}
+
+static JitPlan& eval_plan(cxh_tape_s* t);
+
+// The kernels of forward-only evaluation, both address forms.
+static void jit_prepare_eval(cxh_tape_s* t) {
+  if (roll_active(t)) return roll_prepare_eval(t);
+  JitPlan& P = eval_plan(t);
+  sync_ipt(t, P);
+  for (int g = 0; g < P.n_seg; ++g) {
+    segment_kernel(t, P, g, false);
+    segment_kernel_soa(t, P, g);
+  }
+}
+
+// ---------------------------------------------------- asynchronous compile ---
+//
+// A worker thread builds the kernels on its own copy of the program (the
+// tape's instructions and launch settings, none of its device state), so the
+// caller's tape is never touched while it runs.  When it is done, the next
+// call on the tape merges what it built (plans, loop kernels) into the tape.
+
+struct JitJob {
+  cxh_tape_s clone;
+  std::array<int64_t, 3> key{};  // (every, fwd, rev); every = -1: forward-only evaluation
+  std::atomic<int> state{1};     // 1 running, 2 done, 3 failed
+  std::string error;
+  std::thread th;
+  ~JitJob() {
+    if (th.joinable()) th.join();
+  }
+};
+
+// Workers of tapes never destroyed are waited for when the library unloads,
+// before the compile cache they write to goes (this registry is created
+// after that cache, so it is torn down first).
+struct JitJobs {
+  std::mutex mu;
+  std::vector<std::weak_ptr<JitJob>> live;
+  ~JitJobs() {
+    for (auto& w : live)
+      if (auto j = w.lock())
+        if (j->th.joinable()) j->th.join();
+  }
+};
+static JitJobs& jit_jobs() {
+  static JitJobs r;
+  return r;
+}
+
+struct JitAsync {
+  bool on = false;
+  std::shared_ptr<JitJob> job;                   // the running or finished worker
+  std::vector<std::array<int64_t, 3>> done, failed;
+  std::string last_error;
+};
+
+static void jit_adopt(cxh_tape_s* t, cxh_tape_s& from) {
+  if (from.jit) {
+    if (!t->jit) {
+      t->jit = from.jit;
+    } else {
+      for (auto& kv : from.jit->plans) t->jit->plans.emplace(kv.first, std::move(kv.second));
+      if (t->jit->eval_every == 0) t->jit->eval_every = from.jit->eval_every;
+    }
+  }
+  roll_adopt(t, from);
+}
+
+// The finished worker's results taken over (or its error recorded).
+static void jit_finish(cxh_tape_s* t) {
+  JitAsync& A = *t->async;
+  std::shared_ptr<JitJob> job = std::move(A.job);
+  if (job->th.joinable()) job->th.join();
+  if (job->state.load() == 2) {
+    jit_adopt(t, job->clone);
+    A.done.push_back(job->key);
+  } else {
+    A.failed.push_back(job->key);
+    A.last_error = job->error;
+  }
+}
+
+void jit_settle(cxh_tape_s* t) {
+  if (t->async && t->async->job) jit_finish(t);
+}
+
+bool jit_ready(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
+  if (!t->async || !t->async->on) return true;
+  JitAsync& A = *t->async;
+  if (A.job) {
+    if (A.job->state.load() == 1) return false;  // still compiling: the interpreter takes this call
+    jit_finish(t);
+  }
+  const std::array<int64_t, 3> key{every, fwd ? 1 : 0, rev ? 1 : 0};
+  auto covers = [&](const std::array<int64_t, 3>& d) {
+    return d[0] == key[0] && d[1] >= key[1] && d[2] >= key[2];
+  };
+  if (std::any_of(A.done.begin(), A.done.end(), covers)) return true;
+  // a failed build is left to the specialised path, which raises its error
+  if (std::any_of(A.failed.begin(), A.failed.end(), covers)) return true;
+  if (no_alloc_active()) return true;
+  auto job = std::make_shared<JitJob>();
+  job->clone.alg = t->alg;
+  job->clone.ws = t->ws;
+  job->clone.nnz_in = t->nnz_in;
+  job->clone.nnz_out = t->nnz_out;
+  job->clone.unused_in = t->unused_in;
+  job->clone.block = t->block;
+  job->clone.ipt = t->ipt;
+  job->clone.engine = t->engine;
+  job->key = key;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  JitJob* j = job.get();  // the tape owns the job and joins the thread before letting it go
+  job->th = std::thread([j, dev, every, fwd, rev]() {
+    try {
+      hip_check(hipSetDevice(dev), "hipSetDevice");
+      if (every < 0) jit_prepare_eval(&j->clone);
+      else jit_prepare(&j->clone, every, fwd, rev);
+      j->state.store(2);
+    } catch (const std::exception& e) {
+      j->error = e.what();
+      j->state.store(3);
+    }
+  });
+  {
+    JitJobs& r = jit_jobs();
+    std::lock_guard<std::mutex> lk(r.mu);
+    r.live.erase(std::remove_if(r.live.begin(), r.live.end(), [](const std::weak_ptr<JitJob>& w) { return w.expired(); }),
+                 r.live.end());
+    r.live.push_back(job);
+  }
+  A.job = std::move(job);
+  return false;
 }
 
 // Forward-only evaluation chooses its own segment length: the default one,
@@ -2970,10 +3108,30 @@ int cxh_tape_engine(cxh_tape t, int32_t engine) {
   return 0;
 }
 
+int cxh_tape_jit_async(cxh_tape t, int32_t on) {
+  CXH_TRY
+  if (!t) return cxh_fail("null tape");
+  if (on) {
+    if (!t->async) t->async = std::make_shared<JitAsync>();
+    t->async->on = true;
+  } else if (t->async) {
+    jit_settle(t);
+    t->async->on = false;
+  }
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_tape_jit_pending(cxh_tape t) {
+  if (!t) return cxh_fail("null tape");
+  return t->async && t->async->job && t->async->job->state.load() == 1 ? 1 : 0;
+}
+
 int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, int32_t reverse) {
   CXH_TRY
   if (!t) return cxh_fail("null tape");
   if (t->alg.empty()) return 0;
+  jit_settle(t);
   jit_prepare(t, checkpoint_every, forward != 0, reverse != 0);
   return 0;
   CXH_CATCH
@@ -2983,6 +3141,7 @@ int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, i
                         int64_t cap, int64_t* len, int32_t* n_segments) {
   CXH_TRY
   if (!t) return cxh_fail("null tape");
+  jit_settle(t);
   if (n_segments) *n_segments = plan_for(t, checkpoint_every).n_seg;
   if (t->alg.empty()) {
     if (len) *len = 0;
@@ -3003,6 +3162,7 @@ int cxh_tape_jit_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info, int64
   CXH_TRY
   if (!t || !info) return cxh_fail("null argument");
   if (t->alg.empty()) return cxh_fail("empty tape");
+  jit_settle(t);
   jit_plan_info(t, checkpoint_every, info, ck_off);
   return 0;
   CXH_CATCH

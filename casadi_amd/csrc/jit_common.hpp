@@ -40,6 +40,9 @@ std::string lit(double d);
 std::string finish_source(std::string o);
 // Whether reverse chains run as paired launches (CXH_JIT_SPLIT=0: not).
 bool split_reverse();
+// Whether reverse sweeps load the sin / cos pairs their forward pass stored
+// (CXH_JIT_TRIG_CACHE, default on) instead of computing them again.
+bool trig_cache_enabled();
 // Operand codes whose sin and cos are both taken in the tape.
 std::unordered_set<int64_t> step_trig_pairs(const cxh_tape_s* t);
 // `const double v<k> = sin/cos(...)` through one sincos per operand.

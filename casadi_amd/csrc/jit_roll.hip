@@ -78,6 +78,16 @@ struct RollPlan {
   std::vector<int64_t> st_pos, st_init;
   std::vector<int64_t> pos_st;                  // body position -> entry (-1)
   std::unordered_map<int64_t, int64_t> inv_st;  // prelude instruction -> entry of a passed-through value
+  // Trig rows: the operands (Emitter keys) whose sin and cos the body both
+  // takes, in order of first use.  The forward passes that store iteration
+  // states store each pair beside them, and the reverse iterations load the
+  // pairs instead of calling sincos again.  Empty when the cache is off or
+  // the body takes only one of the two for some operand.
+  std::vector<int64_t> trig_keys;
+  // Rows per iteration in the iteration-state buffers: the state, then two
+  // per trig key.
+  int64_t state_rows() const { return std::max<int64_t>(ns, 1); }
+  int64_t step_rows() const { return state_rows() + 2 * static_cast<int64_t>(trig_keys.size()); }
   std::vector<uint32_t> live;                   // adjoint_live bits of the flat tape
   bool is_live(int64_t k) const { return (live[k >> 5] >> (k & 31)) & 1u; }
 };
@@ -339,6 +349,11 @@ struct Emitter {
   enum Region { PRE, BODY, EPI } region;
   std::unordered_set<std::string> decl;
   std::unordered_map<int64_t, std::pair<std::string, std::string>> trig;
+  // Trig rows (RollPlan::trig_keys) in the body: 1 stores each pair after
+  // its sincos (when store_trig, a constant at every call site), 2 loads it
+  // instead; pair i sits in rows row0 + 2i (sin) and row0 + 2i + 1 (cos) of
+  // `st`.
+  int trig_io = 0;
   char buf[320];
 
   Emitter(const cxh_tape_s* t_, const RollPlan& L_, std::string& o_, Region r) : t(t_), L(L_), o(o_), region(r) {}
@@ -437,6 +452,7 @@ struct Emitter {
       std::string x, ax, y, ay;
       int64_t kx = 0, ky = 0;
       if (e.op != OP_CONST && e.op != OP_INPUT) operand(k, 1, x, ax, kx);
+      if (trig_io && (e.op == OP_SIN || e.op == OP_COS) && trig_row(e, k, kx, x)) continue;
       if (emit_trig(o, e, k, kx, x, all_trig, pairs, trig)) continue;
       switch (e.op) {
         case OP_CONST:
@@ -456,6 +472,29 @@ struct Emitter {
           o += buf;
       }
     }
+  }
+
+  // Sin or cos of a cached operand: the pair from its rows (trig_io 2), or
+  // from sincos and stored to them (trig_io 1).
+  bool trig_row(const Instr& e, int64_t k, int64_t key, const std::string& xname) {
+    const auto pos = std::find(L.trig_keys.begin(), L.trig_keys.end(), key);
+    if (pos == L.trig_keys.end()) return false;
+    auto it = trig.find(key);
+    if (it == trig.end()) {
+      const std::string id = std::to_string(k), i = std::to_string(2 * (pos - L.trig_keys.begin()));
+      it = trig.emplace(key, std::make_pair("ts" + id, "tc" + id)).first;
+      if (trig_io == 2) {
+        o += "  const double ts" + id + " = CXJ_SNAP(st, row0 + " + i + ");\n  const double tc" + id +
+             " = CXJ_SNAP(st, row0 + " + i + " + 1);\n";
+      } else {
+        o += "  double ts" + id + ", tc" + id + ";\n  CXJ_SINCOS(" + xname + ", ts" + id + ", tc" + id + ");\n";
+        o += "  if (store_trig) {\n    CXJ_SNAP(st, row0 + " + i + ") = ts" + id + ";\n    CXJ_SNAP(st, row0 + " + i +
+             " + 1) = tc" + id + ";\n  }\n";
+      }
+    }
+    o += "  const double v" + std::to_string(k) + " = " + (e.op == OP_SIN ? it->second.first : it->second.second) +
+         ";\n";
+    return true;
   }
 
   void acc(const std::string& target, const std::string& term) {
@@ -513,6 +552,28 @@ struct Emitter {
   }
 };
 
+// The operands whose sin and cos the body both takes, in order of first use
+// (the trig rows); empty if the body takes only one of the two for some
+// operand, as then its forward and reverse would not share the sincos.
+std::vector<int64_t> trig_keys(const cxh_tape_s* t, const RollPlan& L) {
+  std::string unused;
+  const Emitter E(t, L, unused, Emitter::BODY);
+  std::vector<int64_t> order;
+  std::unordered_map<int64_t, int> seen;
+  for (int64_t k = L.a; k < L.a + L.P; ++k) {
+    const Instr& e = t->alg[k];
+    if (e.op != OP_SIN && e.op != OP_COS) continue;
+    std::string v, a;
+    int64_t key = 0;
+    E.operand(k, 1, v, a, key);
+    if (!seen.count(key)) order.push_back(key);
+    seen[key] |= e.op == OP_SIN ? 1 : 2;
+  }
+  for (const auto& kv : seen)
+    if (kv.second != 3) return {};
+  return order;
+}
+
 // The state entries read by the body (carried in or passed through).
 std::string load_state(const RollPlan& L, const char* arr) {
   std::string o;
@@ -536,9 +597,10 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev) {
   }
   {  // one iteration forward
     o += "__device__ __forceinline__ void cxj_step_fwd(const CxjArgs& A, const long long k, const long long step, "
-         "const double* s, double* sn, const bool outs) {\n";
+         "const double* s, double* sn, const bool outs, double* st, const long long row0, const bool store_trig) {\n";
     o += load_state(L, "s");
     Emitter E(t, L, o, Emitter::BODY);
+    E.trig_io = L.trig_keys.empty() ? 0 : 1;
     E.declare_constants(a, a + P);
     E.forward(a, a + P, true, false);
     for (int64_t e = 0; e < L.ns; ++e)
@@ -573,9 +635,10 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev) {
   {  // one iteration reverse: as holds the adjoint of the state the iteration
      // hands on, on return that of the state it received
     o += "__device__ __forceinline__ void cxj_step_rev(const CxjArgs& A, const long long k, const long long step, "
-         "const double* s, double* as) {\n";
+         "const double* s, double* as, const double* st, const long long row0) {\n";
     o += load_state(L, "s");
     Emitter E(t, L, o, Emitter::BODY);
+    E.trig_io = L.trig_keys.empty() ? 0 : 2;
     E.declare_constants(a, a + P);
     E.forward(a, a + P, false, true);
     for (int64_t e = 0; e < L.ns; ++e) {  // adjoints handed over, where they are structurally nonzero
@@ -630,6 +693,9 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
   const int64_t S = (L.R + K - 1) / K;
   const std::string ns = std::to_string(std::max<int64_t>(L.ns, 1)), nsi = std::to_string(L.ns);
   const std::string R = std::to_string(L.R), Ks = std::to_string(K), Ss = std::to_string(S);
+  // iteration-state buffers: rs rows per iteration, the trig pairs from row ns1
+  const std::string rs = std::to_string(L.step_rows()), ns1 = std::to_string(L.state_rows());
+  const bool trig = !L.trig_keys.empty();
   if (mode == 4) {
     o += "__device__ __forceinline__ void cxj_back(const CxjArgs& A, const long long k, const long long seg) {\n"
          "  double s[" + ns + "], as[" + ns + "];\n"
@@ -642,8 +708,8 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
          "  }\n"
          "#pragma unroll 1\n"
          "  for (long long step = t1 - 1; step >= t0; --step) {\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j);\n"
-         "    cxj_step_rev(A, k, step, s, as);\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + rs + " + j);\n"
+         "    cxj_step_rev(A, k, step, s, as, A.st_out, (step - t0) * " + rs + " + " + ns1 + ");\n"
          "  }\n"
          "  if (seg == 0) cxj_pre_rev(A, k, as);\n"
          "  else for (int j = 0; j < " + nsi + "; ++j) CXJ_ADJ_OUT(j) = as[j];\n"
@@ -654,9 +720,11 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
          "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n"
          "#pragma unroll 1\n"
          "  for (long long step = t0; step < t1; ++step) {\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j) = s[j];\n"
-         "    if (step + 1 < t1) {\n"
-         "      cxj_step_fwd(A, k, step, s, sn, false);\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + rs + " + j) = s[j];\n" +
+         // with trig rows the last iteration runs too: its reverse reads its pairs
+         (trig ? std::string("    {\n") : std::string("    if (step + 1 < t1) {\n")) +
+         "      cxj_step_fwd(A, k, step, s, sn, false, A.st_out, " +
+         (trig ? "(step - t0) * " + rs + " + " + ns1 + ", true" : std::string("0, false")) + ");\n"
          "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
          "    }\n"
          "  }\n"
@@ -676,18 +744,41 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
   o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
        "  double s[" + ns + "], sn[" + ns + "];\n"
        "  cxj_pre_fwd(A, k, s, true);\n";
-  if (mode == 2) o += "  double* ck = const_cast<double*>(A.st_in);\n";
-  o += "#pragma unroll 1\n"
-       "  for (long long step = 0; step < " + R + "; ++step) {\n";
-  if (mode == 2)
-    o += "    if (step % " + Ks + " == 0)\n"
-         "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n"
-         "    if (A.st_out && step >= " + std::to_string((S - 1) * K) + ")\n"
-         "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + std::to_string((S - 1) * K) +
-         ") * " + nsi + " + j) = s[j];\n";
-  o += "    cxj_step_fwd(A, k, step, s, sn, true);\n"
-       "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-       "  }\n";
+  // one loop over [from, to): the checkpoint every K iterations (mode 2),
+  // with states the iteration states of the last segment and, with trig,
+  // their trig rows
+  const std::string last0 = std::to_string((S - 1) * K);
+  auto loop = [&](const std::string& from, const std::string& to, bool states, bool trig_rows) {
+    o += "#pragma unroll 1\n"
+         "  for (long long step = " + from + "; step < " + to + "; ++step) {\n";
+    if (mode == 2)
+      o += "    if (step % " + Ks + " == 0)\n"
+           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n";
+    if (states)
+      o += "    if (A.st_out && step >= " + last0 + ")\n"
+           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + last0 + ") * " + rs +
+           " + j) = s[j];\n";
+    o += "    cxj_step_fwd(A, k, step, s, sn, true, " +
+         (trig_rows ? "A.st_out, (step - " + last0 + ") * " + rs + " + " + ns1 + ", true" : std::string("nullptr, 0, false")) +
+         ");\n"
+         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+         "  }\n";
+  };
+  if (mode == 2) {
+    o += "  double* ck = const_cast<double*>(A.st_in);\n";
+    if (trig) {  // the iterations before the last segment without a branch on the trig stores
+      loop("0", last0, false, false);
+      o += "  if (A.st_out) {\n";
+      loop(last0, R, true, true);
+      o += "  } else {\n";
+      loop(last0, R, false, false);
+      o += "  }\n";
+    } else {
+      loop("0", R, true, false);
+    }
+  } else {
+    loop("0", R, false, false);
+  }
   if (mode == 2)
     o += "  for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, " + Ss + " * " + nsi + " + j) = s[j];\n";
   o += "  cxj_epi_fwd(A, k, s, true);\n"
@@ -705,6 +796,7 @@ RollTape& roll_state(cxh_tape_s* t) {
   RollTape& r = *t->roll;
   if (!r.analysed) {
     analyse(t, r.plan);
+    if (r.plan.ok && trig_cache_enabled()) r.plan.trig_keys = trig_keys(t, r.plan);
     r.use = r.plan.ok && roll_enabled();
     r.analysed = true;
   }
@@ -748,6 +840,17 @@ void roll_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
   }
 }
 
+void roll_prepare_eval(cxh_tape_s* t) { roll_kernel(t, 0, 0); }
+
+void roll_adopt(cxh_tape_s* t, cxh_tape_s& from) {
+  if (!from.roll) return;
+  if (!t->roll || !t->roll->analysed) {
+    t->roll = from.roll;
+    return;
+  }
+  for (const auto& kv : from.roll->fn) t->roll->fn.emplace(kv.first, kv.second);
+}
+
 int roll_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s) {
   CxjArgs A{};
   A.arg = arg;
@@ -764,7 +867,7 @@ int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTab
              const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s) {
   RollTape& r = roll_state(t);
   const RollPlan& L = r.plan;
-  const int64_t K = roll_K(L, every), S = (L.R + K - 1) / K, ns = std::max<int64_t>(L.ns, 1);
+  const int64_t K = roll_K(L, every), S = (L.R + K - 1) / K, ns = std::max<int64_t>(L.ns, 1), rs = L.step_rows();
   const int64_t lds = round_up(n, 64);
   CxjArgs A{};
   A.arg = arg;
@@ -777,19 +880,19 @@ int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTab
     launch(roll_kernel(t, 2, K), n, t->block, A, s);
     return 0;
   }
-  // checkpoints (unless given), the iteration states of two segments, the
-  // state adjoint twice
+  // checkpoints (unless given), the iteration states (and trig rows) of two
+  // segments, the state adjoint twice
   const int64_t ck_rows = mode == 0 ? (S + 1) * ns : 0;
-  r.scratch.reserve(static_cast<size_t>(ck_rows + 2 * K * ns + 2 * ns) * lds * sizeof(double));
+  r.scratch.reserve(static_cast<size_t>(ck_rows + 2 * K * rs + 2 * ns) * lds * sizeof(double));
   double* base = r.scratch.as<double>();
   double* ck = mode == 0 ? base : ckpt_ext;
   double* states = base + ck_rows * lds;
-  double* adjb = states + 2 * K * ns * lds;
+  double* adjb = states + 2 * K * rs * lds;
   long long first = S;  // seg = S: recompute of the last segment only
   if (mode == 0) {
     A.res = res;
     A.st_in = ck;
-    A.st_out = states + ((S - 1) % 2) * K * ns * lds;
+    A.st_out = states + ((S - 1) % 2) * K * rs * lds;
     launch(roll_kernel(t, 2, K), n, t->block, A, s);
     first = S - 1;
   }
@@ -801,10 +904,10 @@ int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTab
   CxjArgs B = A;
   const unsigned grid = static_cast<unsigned>((n + t->block - 1) / t->block);
   for (long long seg = first; seg >= 0; --seg) {
-    A.st_out = states + (seg % 2) * K * ns * lds;
+    A.st_out = states + (seg % 2) * K * rs * lds;
     A.adj_in = adjb + ((seg + 1) % 2) * ns * lds;
     A.adj_out = adjb + (seg % 2) * ns * lds;
-    B.st_out = states + ((seg + 1) % 2) * K * ns * lds;  // the buffer of seg - 1
+    B.st_out = states + ((seg + 1) % 2) * K * rs * lds;  // the buffer of seg - 1
     void* params[] = {&A, &B, &seg};
     hip_check(hipModuleLaunchKernel(fp, grid, 2, 1, t->block, 1, 1, 0, s, params, nullptr), "cxj_roll_pair launch");
   }
@@ -820,11 +923,13 @@ extern "C" {
 int cxh_tape_roll_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info) {
   CXH_TRY
   if (!t || !info) return cxh_fail("null argument");
+  jit_settle(t);
   const RollTape& r = roll_state(t);
   const RollPlan& L = r.plan;
   const int64_t K = L.ok ? roll_K(L, checkpoint_every) : 0;
-  const int64_t v[8] = {L.ok ? 1 : 0, L.a, L.P, L.R, L.ns, K, K ? (L.R + K - 1) / K : 0, r.use ? 1 : 0};
-  for (int i = 0; i < 8; ++i) info[i] = v[i];
+  const int64_t v[9] = {L.ok ? 1 : 0, L.a, L.P, L.R, L.ns, K, K ? (L.R + K - 1) / K : 0, r.use ? 1 : 0,
+                        L.ok ? L.step_rows() : 0};
+  for (int i = 0; i < 9; ++i) info[i] = v[i];
   return 0;
   CXH_CATCH
 }
@@ -834,6 +939,7 @@ int cxh_tape_roll_source(cxh_tape t, int64_t checkpoint_every, int32_t mode, cha
   CXH_TRY
   if (!t) return cxh_fail("null tape");
   if (mode != 0 && mode != 2 && mode != 4) return cxh_fail("cxh_tape_roll_source: mode must be 0, 2 or 4");
+  jit_settle(t);
   const RollPlan& L = roll_state(t).plan;
   if (!L.ok) return cxh_fail("cxh_tape_roll_source: the tape has no loop structure");
   const std::string src = gen_roll(t, L, mode, mode == 0 ? 0 : roll_K(L, checkpoint_every));

@@ -68,6 +68,7 @@ void validate_tape(const std::vector<Instr>& a, int64_t ws, const std::vector<in
 
 struct JitTape;
 struct RollTape;
+struct JitAsync;
 
 }  // namespace sxh
 
@@ -94,6 +95,7 @@ struct cxh_tape_s {
   int engine = 0;  // CXH_ENGINE_INTERP or CXH_ENGINE_JIT
   std::shared_ptr<sxh::JitTape> jit;  // specialised kernels (jit.hip)
   std::shared_ptr<sxh::RollTape> roll;  // the tape rerolled into a loop, when it repeats (jit_roll.hip)
+  std::shared_ptr<sxh::JitAsync> async;  // compiles on a worker thread (cxh_tape_jit_async), else null
 };
 
 namespace sxh {
@@ -107,6 +109,18 @@ const Instr* device_tape(cxh_tape_s* t);
 // Evaluation and fused forward + adjoint over explicit I/O tables (any
 // per-block strides: s_inst = 0 broadcasts a block to every instance).
 int map_eval_io(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);
+
+// Asynchronous compile of the specialised kernels (jit.hip).  jit_ready
+// says whether a call may take the specialised path for (every, fwd, rev)
+// now: with cxh_tape_jit_async on, the first call that needs kernels not yet
+// built starts a worker thread compiling them on a copy of the program and
+// is told no (the caller runs the interpreter, which gives the same bits);
+// later calls take them once the worker is done.  every = -1 names
+// forward-only evaluation.  jit_settle waits for a running worker and takes
+// over what it built: every other path that touches the specialised state
+// calls it first.
+bool jit_ready(cxh_tape_s* t, int64_t every, bool fwd, bool rev);
+void jit_settle(cxh_tape_s* t);
 
 // Per instruction (bit k of word k / 32), whether its adjoint is
 // structurally nonzero in the reverse sweep CasADi builds symbolically
@@ -129,6 +143,9 @@ int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every);
 bool roll_active(cxh_tape_s* t);
 int64_t roll_ckpt_rows(cxh_tape_s* t, int64_t every);
 void roll_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev);
+void roll_prepare_eval(cxh_tape_s* t);
+// Takes over the loop kernels another tape of the same program built.
+void roll_adopt(cxh_tape_s* t, cxh_tape_s& from);
 int roll_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res, hipStream_t s);
 int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
              const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);

@@ -765,6 +765,7 @@ int cxh_tape_instructions(cxh_tape t, void* instr, int64_t cap, int64_t* n_instr
 }
 
 int cxh_tape_destroy(cxh_tape t) {
+  if (t) jit_settle(t);  // a worker still compiling for the tape is waited for
   delete t;
   return 0;
 }
@@ -858,8 +859,20 @@ int cxh_tape_tune(cxh_tape t, int64_t n, const double* const* arg, double* const
       (void)hipEventDestroy(e1);
     }
   } ev;
-  const Cfg keep{t->block, t->w_lds, t->ipw, t->ipt};
-  Cfg pick = keep;
+  // the tape's settings come back unless a fastest one is chosen
+  struct Restore {
+    cxh_tape_s* t;
+    Cfg c;
+    bool on = true;
+    ~Restore() {
+      if (!on) return;
+      t->block = c.block;
+      t->w_lds = c.w_lds;
+      t->ipw = c.ipw;
+      t->ipt = c.ipt;
+    }
+  } restore{t, {t->block, t->w_lds, t->ipw, t->ipt}};
+  Cfg pick = restore.c;
   float best_t = -1.0f;
   std::string last_err;
   for (const Cfg& c : cfgs) {
@@ -887,11 +900,12 @@ int cxh_tape_tune(cxh_tape t, int64_t n, const double* const* arg, double* const
       pick = c;
     }
   }
+  if (best_t < 0) return cxh_fail("cxh_tape_tune: no setting could be launched: " + last_err);
+  restore.on = false;
   t->block = pick.block;
   t->w_lds = pick.w_lds;
   t->ipw = pick.ipw;
   t->ipt = pick.ipt;
-  if (best_t < 0) return cxh_fail("cxh_tape_tune: no setting could be launched: " + last_err);
   if (best) {
     best[0] = pick.block;
     best[1] = pick.w_lds;
@@ -909,7 +923,7 @@ namespace sxh {
 
 int map_eval_io(cxh_tape t, int64_t n, const IOTable& d_arg, const IOTable& d_res, hipStream_t s) {
   if (n == 0 || t->alg.empty()) return 0;
-  if (t->engine == CXH_ENGINE_JIT) return jit_map_eval(t, n, d_arg, d_res, s);
+  if (t->engine == CXH_ENGINE_JIT && jit_ready(t, -1, true, false)) return jit_map_eval(t, n, d_arg, d_res, s);
   const GridShape g = grid_shape(t, n);
   const size_t lds = static_cast<size_t>(t->ws) * g.cols * sizeof(double);
   const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
@@ -1021,7 +1035,13 @@ std::vector<uint32_t> adjoint_live(const std::vector<Instr>& alg, int64_t ws) {
 int adj_io(cxh_tape t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
            const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s) {
   if (n == 0 || t->alg.empty()) return 0;
-  if (t->engine == CXH_ENGINE_JIT) return jit_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
+  if (t->engine == CXH_ENGINE_JIT) {
+    // the split sweep (modes 1 and 2) keeps one engine's checkpoint layout
+    // across its two calls, so it never changes engine half way
+    if (mode != 0) jit_settle(t);
+    if (mode != 0 || jit_ready(t, every, true, true))
+      return jit_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
+  }
   if (!t->d_live.ptr) {
     const std::vector<uint32_t> bits = adjoint_live(t->alg, t->ws);
     t->d_live.upload(bits.data(), bits.size() * sizeof(uint32_t));
@@ -1085,6 +1105,7 @@ int cxh_ckpt_size(cxh_tape t, int64_t n, int64_t checkpoint_every, int32_t block
   if (!t || !n_doubles) return cxh_fail("null argument");
   (void)block;
   if (t->engine == CXH_ENGINE_JIT) {
+    jit_settle(t);
     *n_doubles = t->alg.empty() ? 0 : jit_ckpt_rows(t, checkpoint_every) * round_up(n, 64);
     return 0;
   }

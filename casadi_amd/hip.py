@@ -183,10 +183,12 @@ class HipTape:
         prelude (instructions before the loop), period (body length),
         iterations, state (values crossing an iteration), K (iterations per
         checkpoint segment), segments, enabled (the tape runs as the loop: CXH_JIT_ROLL
-        when the tape was first analysed, and a loop was found)."""
-        info = (L.I64 * 8)()
+        when the tape was first analysed, and a loop was found), step_rows
+        (rows per iteration in the reverse's iteration-state buffers: the
+        state, then the sin / cos rows the recompute stores)."""
+        info = (L.I64 * 9)()
         L.check(L.cxh_tape_roll_plan(self._h, int(checkpoint_every), info), "cxh_tape_roll_plan")
-        keys = ("ok", "prelude", "period", "iterations", "state", "K", "segments", "enabled")
+        keys = ("ok", "prelude", "period", "iterations", "state", "K", "segments", "enabled", "step_rows")
         return {k: int(v) for k, v in zip(keys, info)}
 
     def roll_source(self, checkpoint_every=0, mode=0):
@@ -227,6 +229,17 @@ class HipTape:
             self.block = int(block)
         if ipw != -1:
             self.ipw = int(ipw)
+
+    def jit_async(self, on=True):
+        """Compile the specialised kernels on a worker thread (cxh_tape_jit_async):
+        calls made before they are ready run on the interpreter, same bits."""
+        L.check(L.cxh_tape_jit_async(self._h, int(bool(on))), "cxh_tape_jit_async")
+
+    def jit_pending(self):
+        """Whether a worker thread is still compiling kernels for this tape."""
+        rc = L.cxh_tape_jit_pending(self._h)
+        L.check(rc, "cxh_tape_jit_pending")
+        return rc == 1
 
     def ipt(self, ipt):
         """Instances per thread of the specialised forward kernels (1/2/4/8;

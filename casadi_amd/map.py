@@ -4,7 +4,9 @@
 casadi/core/map.cpp:141-157 and OmpMap::eval, casadi/core/map.cpp:340-386);
 ``hip`` runs the gfx950 kernels: by default the straight-line kernels
 specialised from the tape (opts {"engine": "jit"}), or the tape interpreter
-(opts {"engine": "interp"}), which needs no compilation.  Inputs follow Map's signature:
+(opts {"engine": "interp"}), which needs no compilation; with opts
+{"jit_async": True} the kernels compile on a worker thread and the calls made
+meanwhile run on the interpreter.  Inputs follow Map's signature:
 input i is ``repmat(sparsity_in(i), 1, n)`` -- for a column-vector input an
 ``(nrow, n)`` array -- and a non-repeated input is broadcast to all instances.
 """
@@ -46,6 +48,10 @@ class MapFunction:
                 # CasADi's option: NaN / Inf in an output raises; here checked
                 # per instance on the GPU (cxh_tape_regularity_check)
                 self._tape.regularity_check(self.n)
+            if self.opts.get("jit_async", False):
+                # calls before the specialised kernels are built run on the
+                # interpreter (cxh_tape_jit_async)
+                self._tape.jit_async(True)
         return self._tape
 
     def _aos(self, i, a):

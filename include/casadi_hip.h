@@ -159,6 +159,20 @@ int cxh_tape_engine(cxh_tape t, int32_t engine);
 /* Compile (ahead of the first launch) the specialised kernels for a
  * checkpoint interval: forward segments and/or reverse segments.            */
 int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, int32_t reverse);
+/* Asynchronous compile (on = 1) for a tape on the specialised engine: a
+ * cxh_map_eval or fused cxh_map_fwd_adj call whose kernels are not built yet
+ * starts a worker thread that builds them on a copy of the program, and runs
+ * on the interpreter meanwhile (the two engines give the same bits); the
+ * first call after the worker is done takes the specialised kernels.  Other
+ * calls on the tape (the split checkpointed sweep, cxh_ckpt_size, the
+ * introspection calls, cxh_tape_destroy) wait for a running worker first.
+ * A build that fails is left to the specialised path, which then raises its
+ * error.  on = 0 waits for a running worker and compiles in the calling
+ * thread again.  cxh_tape_jit_pending: 1 while a worker runs, else 0.
+ * Replaces nothing in the reference: CasADi's Map evaluates at once; this
+ * hides the first call's compile of map(n, "hip").                         */
+int cxh_tape_jit_async(cxh_tape t, int32_t on);
+int cxh_tape_jit_pending(cxh_tape t);
 /* Introspection: generated HIP source of one segment kernel into buf
  * (NUL-terminated, truncated to cap), its full length in *len and the number
  * of segments.  reverse: 0 the forward kernel, 1 the whole recompute +
@@ -178,10 +192,13 @@ int cxh_tape_jit_plan(cxh_tape t, int64_t checkpoint_every, int64_t* info, int64
 /* Introspection: the loop structure of a tape whose middle repeats one body
  * (a recurrence unrolled into the graph: rollouts, shooting problems), which
  * the specialised engine runs as a loop instead of straight-line segments.
- * info[8] = {found (1/0), prelude length, body length, iterations, state
+ * info[9] = {found (1/0), prelude length, body length, iterations, state
  * entries crossing an iteration, iterations per checkpoint segment for
  * checkpoint_every, checkpoint segments, whether the tape runs as the loop
- * (CXH_JIT_ROLL when the tape was first analysed, and a loop was found)}.
+ * (CXH_JIT_ROLL when the tape was first analysed, and a loop was found),
+ * rows per iteration in the reverse sweep's iteration-state buffers (the
+ * state, then a sin and a cos row per sincos operand of the body, which the
+ * recompute stores and the reverse reads; CXH_JIT_TRIG_CACHE=0: none)}.
  * Reference: the graph order of SXFunction::init
  *   (casadi/core/sx_function.cpp:476-841) is what makes such tapes periodic;
  *   evaluation as in SXFunction::eval (casadi/core/sx_function.cpp:72-127).   */

@@ -117,7 +117,7 @@ def _launch_sequence(f, every, n, seed_val, mode):
         # the driver's buffers, each at its exact size (one allocation there;
         # separate here, so that a spill from one into the next is caught)
         ck = emu.buffer((S + 1) * ns * lds)
-        states = [emu.buffer(K * ns * lds) for _ in range(2)]
+        states = [emu.buffer(K * P["step_rows"] * lds) for _ in range(2)]
         adjb = [emu.buffer(ns * lds) for _ in range(2)]
         A.res = _table(yb, n)
         A.st_in = _ptr(ck)
@@ -164,6 +164,30 @@ def test_rerolled_kernels_in_bounds_race_free_and_exact(case, mode):
         assert np.array_equal(y, r.T)
     if mode == "eval":
         return
+    _, gr = O.adj_map(f, n, [a.T for a in ins], [s.T for s in seeds])
+    for g, r in zip(gs, gr):
+        assert np.array_equal(g, r.T)
+
+
+def test_trig_rows_of_the_loop_body(monkeypatch):
+    """The cart-pole step takes sin and cos of the angle once per RK4 stage:
+    four pairs, stored beside the 5 state values of every iteration.  A body
+    that takes only sin of some operand gets none (its forward would not
+    share the sincos), nor does any body with CXH_JIT_TRIG_CACHE=0."""
+    from casadi_amd import workloads as W
+    f = W.cartpole_rollout(6)
+    assert _roll(f).roll_plan(138)["step_rows"] == 5 + 2 * 4
+    assert _roll(parametric_rollout(24)).roll_plan()["step_rows"] == 3
+    monkeypatch.setenv("CXH_JIT_TRIG_CACHE", "0")
+    assert _roll(f).roll_plan(138)["step_rows"] == 5
+
+
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_rerolled_kernels_without_trig_rows(monkeypatch, mode):
+    monkeypatch.setenv("CXH_JIT_TRIG_CACHE", "0")
+    f, every, n = _cases()["rollout6"]
+    errors, ys, gs, ins, seeds = _launch_sequence(f, every, n, 20261210 + n, mode)
+    assert errors == 0
     _, gr = O.adj_map(f, n, [a.T for a in ins], [s.T for s in seeds])
     for g, r in zip(gs, gr):
         assert np.array_equal(g, r.T)
