@@ -439,6 +439,7 @@ def other_config_figures(timed):
     cfg, _ = B.tune(reps=3)
     ms_g = timed(lambda: B.launch(), 50, 5)
     ms_d = timed(lambda: B.launch(graph=False), 50, 5)
+    ms_b = timed(lambda: B.launch(graph=None), 50, 5)
     tapes = [(HipTape(f), HipTape(f.reverse(1))) for f in fs]
 
     def plain():
@@ -446,8 +447,8 @@ def other_config_figures(timed):
             tf.eval(A, m)
             tr.eval(A + [None] * len(S) + S, m)
     ms_p = timed(plain, 10, 1)
-    out["batch_77000"] = {"instances": sum(counts), "ms_graph": ms_g, "ms_direct": ms_d, "first_call_s": c_s,
-                          "instances_per_s": sum(counts) / (min(ms_g, ms_d) * 1e-3), "tuned": cfg,
+    out["batch_77000"] = {"instances": sum(counts), "ms": ms_b, "ms_graph": ms_g, "ms_direct": ms_d,
+                          "first_call_s": c_s, "instances_per_s": sum(counts) / (ms_b * 1e-3), "tuned": cfg,
                           "plain_interpreter_ms": ms_p}
     # config 5: batched KKT solves
     n = 8192

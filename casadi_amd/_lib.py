@@ -130,6 +130,8 @@ cxh_batch_bind = _opt("cxh_batch_bind", I32, P, I32, PP, PP, PP, PP, I32, I64)
 cxh_batch_config = _opt("cxh_batch_config", I32, P, I32, I32, I32)
 cxh_batch_tune = _opt("cxh_batch_tune", I32, P, P, I32, C.POINTER(I32), C.POINTER(D))
 cxh_batch_launch = _opt("cxh_batch_launch", I32, P, P)
+cxh_batch_launch_best = _opt("cxh_batch_launch_best", I32, P, P)
+cxh_batch_prefers_graph = _opt("cxh_batch_prefers_graph", I32, P)
 cxh_batch_launch_direct = _opt("cxh_batch_launch_direct", I32, P, P)
 cxh_batch_info = _opt("cxh_batch_info", I32, P, C.POINTER(I32), PI64)
 cxh_batch_destroy = _opt("cxh_batch_destroy", I32, P)

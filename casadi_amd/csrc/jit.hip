@@ -953,6 +953,18 @@ void jit_settle(cxh_tape_s* t) {
   if (t->async && t->async->job) jit_finish(t);
 }
 
+bool jit_async_pause(cxh_tape_s* t) {
+  if (!t->async || !t->async->on) return false;
+  jit_settle(t);
+  t->async->on = false;
+  return true;
+}
+
+void jit_async_set(cxh_tape_s* t, bool on) {
+  if (on && !t->async) t->async = std::make_shared<JitAsync>();
+  if (t->async) t->async->on = on;
+}
+
 bool jit_ready(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
   if (!t->async || !t->async->on) return true;
   JitAsync& A = *t->async;
@@ -2703,6 +2715,7 @@ struct cxh_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool dirty = true;
+  int prefer_graph = -1;  // from the tuner: 1 the graph replay was faster, 0 the direct launch, -1 not tuned
   ~cxh_batch_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -2750,6 +2763,20 @@ static void batch_launch_direct(cxh_batch_s* b, hipStream_t s) {
   void* params[] = {&a, &tl, &nt};
   hip_check(hipModuleLaunchKernel(b->fn, static_cast<unsigned>(g), 1, 1, b->block, 1, 1, 0, s, params, nullptr),
             "cxj_batch launch");
+}
+
+// The launch captured into a graph (again after a change of launch shape).
+static void batch_graph(cxh_batch_s* b) {
+  if (!b->dirty) return;
+  if (b->exec) hip_check(hipGraphExecDestroy(b->exec), "hipGraphExecDestroy");
+  if (b->graph) hip_check(hipGraphDestroy(b->graph), "hipGraphDestroy");
+  b->exec = nullptr;
+  b->graph = nullptr;
+  hip_check(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  batch_launch_direct(b, b->cap);
+  hip_check(hipStreamEndCapture(b->cap, &b->graph), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  b->dirty = false;
 }
 
 static void batch_check_bound(const cxh_batch_s* b) {
@@ -2980,6 +3007,7 @@ int cxh_batch_config(cxh_batch b, int32_t block, int32_t grid_per_cu, int32_t or
     if (order < 0 || order > 2) return cxh_fail("order must be 0, 1 or 2");
     b->order = order;
   }
+  b->prefer_graph = -1;  // a tuned preference holds for the tuned shape only
   build_tiles(b);
   return 0;
   CXH_CATCH
@@ -3021,12 +3049,30 @@ int cxh_batch_tune(cxh_batch b, void* stream, int32_t reps, int32_t* best, doubl
         }
       }
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   b->block = bb;
   b->grid_mult = bg;
   b->order = bo;
   build_tiles(b);
+  // the winning shape replayed from its graph against launched directly
+  {
+    batch_graph(b);
+    float t_dir = 0, t_graph = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 0) batch_launch_direct(b, b->cap);
+      else hip_check(hipGraphLaunch(b->exec, b->cap), "hipGraphLaunch");  // warm-up
+      hip_check(hipEventRecord(e0, b->cap), "hipEventRecord");
+      for (int r = 0; r < reps; ++r) {
+        if (pass == 0) batch_launch_direct(b, b->cap);
+        else hip_check(hipGraphLaunch(b->exec, b->cap), "hipGraphLaunch");
+      }
+      hip_check(hipEventRecord(e1, b->cap), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      hip_check(hipEventElapsedTime(pass == 0 ? &t_dir : &t_graph, e0, e1), "hipEventElapsedTime");
+    }
+    b->prefer_graph = t_graph <= t_dir ? 1 : 0;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (best) {
     best[0] = bb;
     best[1] = bg;
@@ -3041,20 +3087,29 @@ int cxh_batch_launch(cxh_batch b, void* stream) {
   CXH_TRY
   if (!b) return cxh_fail("null batch");
   batch_check_bound(b);
-  if (b->dirty) {
-    if (b->exec) hip_check(hipGraphExecDestroy(b->exec), "hipGraphExecDestroy");
-    if (b->graph) hip_check(hipGraphDestroy(b->graph), "hipGraphDestroy");
-    b->exec = nullptr;
-    b->graph = nullptr;
-    hip_check(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    batch_launch_direct(b, b->cap);
-    hip_check(hipStreamEndCapture(b->cap, &b->graph), "hipStreamEndCapture");
-    hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    b->dirty = false;
-  }
+  batch_graph(b);
   hip_check(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
   return 0;
   CXH_CATCH
+}
+
+int cxh_batch_launch_best(cxh_batch b, void* stream) {
+  CXH_TRY
+  if (!b) return cxh_fail("null batch");
+  batch_check_bound(b);
+  if (b->prefer_graph == 0) {
+    batch_launch_direct(b, static_cast<hipStream_t>(stream));
+    return 0;
+  }
+  batch_graph(b);
+  hip_check(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+  return 0;
+  CXH_CATCH
+}
+
+int cxh_batch_prefers_graph(cxh_batch b) {
+  if (!b) return cxh_fail("null batch");
+  return b->prefer_graph;
 }
 
 int cxh_batch_launch_direct(cxh_batch b, void* stream) {

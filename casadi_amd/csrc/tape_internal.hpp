@@ -121,6 +121,10 @@ int map_eval_io(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& res
 // calls it first.
 bool jit_ready(cxh_tape_s* t, int64_t every, bool fwd, bool rev);
 void jit_settle(cxh_tape_s* t);
+// Turns asynchronous compile off (after settling) and says whether it was on;
+// jit_async_set(t, on) puts it back.
+bool jit_async_pause(cxh_tape_s* t);
+void jit_async_set(cxh_tape_s* t, bool on);
 
 // Per instruction (bit k of word k / 32), whether its adjoint is
 // structurally nonzero in the reverse sweep CasADi builds symbolically

@@ -863,15 +863,17 @@ int cxh_tape_tune(cxh_tape t, int64_t n, const double* const* arg, double* const
   struct Restore {
     cxh_tape_s* t;
     Cfg c;
+    bool async;  // compiles stay in this thread while tuning: the timings are the specialised kernels'
     bool on = true;
     ~Restore() {
+      jit_async_set(t, async);
       if (!on) return;
       t->block = c.block;
       t->w_lds = c.w_lds;
       t->ipw = c.ipw;
       t->ipt = c.ipt;
     }
-  } restore{t, {t->block, t->w_lds, t->ipw, t->ipt}};
+  } restore{t, {t->block, t->w_lds, t->ipw, t->ipt}, jit_async_pause(t)};
   Cfg pick = restore.c;
   float best_t = -1.0f;
   std::string last_err;

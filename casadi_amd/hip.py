@@ -609,14 +609,18 @@ class HipBatch:
         return dict(block=cfg[0], grid_per_cu=cfg[1], order=cfg[2], n_tiles=nt.value)
 
     def tune(self, reps=3, stream=None):
-        """Autotune the launch shape; returns (config dict, ms per launch)."""
+        """Autotune the launch shape, then graph replay against direct launch;
+        returns (config dict, ms per launch)."""
         best = (L.I32 * 3)()
         ms = L.D()
         L.check(L.cxh_batch_tune(self._h, _stream_ptr(stream), int(reps), best, C.byref(ms)), "cxh_batch_tune")
-        return dict(block=best[0], grid_per_cu=best[1], order=best[2]), ms.value
+        return dict(block=best[0], grid_per_cu=best[1], order=best[2],
+                    graph=bool(L.cxh_batch_prefers_graph(self._h) == 1)), ms.value
 
     def launch(self, stream=None, graph=True):
-        fn = L.cxh_batch_launch if graph else L.cxh_batch_launch_direct
+        """graph=True: replay the captured graph; False: launch directly;
+        None: whichever the tuner found faster."""
+        fn = L.cxh_batch_launch_best if graph is None else L.cxh_batch_launch if graph else L.cxh_batch_launch_direct
         L.check(fn(self._h, _stream_ptr(stream)), "cxh_batch_launch")
 
 

@@ -355,12 +355,20 @@ int cxh_batch_bind(cxh_batch b, int32_t fun, const double* const* arg, const dou
 int cxh_batch_config(cxh_batch b, int32_t block, int32_t grid_per_cu, int32_t order);
 /* Autotuner: times every launch shape (reps launches each, hipEvents on an
  * internal stream) and keeps the fastest; best[3] = {block, grid_per_cu,
- * order}, *best_ms its time.  Runs the batch, so buffers must be bound.      */
+ * order}, *best_ms its time.  Then times that shape replayed from its
+ * captured graph against launched directly, and remembers which was faster
+ * (cxh_batch_prefers_graph; cxh_batch_launch_best takes it).  Runs the
+ * batch, so buffers must be bound.                                          */
 int cxh_batch_tune(cxh_batch b, void* stream, int32_t reps, int32_t* best, double* best_ms);
 /* Replay the captured graph on stream (captured again after a config change). */
 int cxh_batch_launch(cxh_batch b, void* stream);
 /* The same kernel launched without the graph. */
 int cxh_batch_launch_direct(cxh_batch b, void* stream);
+/* The graph replay or the direct launch, whichever the tuner found faster
+ * (the graph before any tuning or after a cxh_batch_config change).
+ * cxh_batch_prefers_graph: 1 graph, 0 direct, -1 not tuned.                  */
+int cxh_batch_launch_best(cxh_batch b, void* stream);
+int cxh_batch_prefers_graph(cxh_batch b);
 int cxh_batch_info(cxh_batch b, int32_t* config, int64_t* n_tiles);
 int cxh_batch_destroy(cxh_batch b);
 /* Introspection: the generated source of a batch kernel over these tapes

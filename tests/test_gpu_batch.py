@@ -48,8 +48,8 @@ def test_batch_matches_single_launches_and_oracle(problem):
         bound.append((A, S) + B.bind(i, A, S))
     cfg, ms = B.tune(reps=2)
     print(f"[batch] autotuned {cfg} {ms:.4f} ms", flush=True)
-    assert cfg["block"] in (64, 128, 256) and 0 <= cfg["order"] <= 2
-    B.launch()
+    assert cfg["block"] in (64, 128, 256) and 0 <= cfg["order"] <= 2 and cfg["graph"] in (True, False)
+    B.launch(graph=None)                 # the tuner's choice of graph replay or direct launch
     torch.cuda.synchronize()
     for f, n, (A, S, outs, adjs), (args, seeds) in zip(fs, COUNTS, bound, data):
         t = HipTape(f, engine="jit")

@@ -52,6 +52,15 @@ def test_tune_keeps_the_bits(cuda, engine):
                 tape.config(block=128, w_in_lds=lds, ipw=ipw)
                 (y,) = [t.cpu().numpy() for t in tape.eval(dev)]
                 assert np.array_equal(y0, y), (ipw, lds)
+    # tuning a tape that compiles asynchronously times the specialised
+    # kernels (built in the calling thread meanwhile), same bits after
+    if engine == "jit":
+        t2 = HipTape(f, engine="jit")
+        t2.jit_async(True)
+        cfg2, _ = t2.tune(dev, reps=2)
+        assert not t2.jit_pending() and cfg2["block"] in (64, 128, 256)
+        (y,) = [t.cpu().numpy() for t in t2.eval(dev)]
+        assert np.array_equal(y0, y)
     # and the oracle's values on a sample
     m = 64
     (yo,) = O.map_eval(f, m, [x[:, :m].T, u[:, :m].T])
