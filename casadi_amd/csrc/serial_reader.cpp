@@ -225,6 +225,7 @@ class Reader {
       if (v.size() < 3) fail("compressed sparsity too short");
       const int64_t nrow = v[0], ncol = v[1];
       if (nrow < 0 || ncol < 0) fail("negative sparsity dimensions");
+      if (ncol > 0 && nrow > INT64_MAX / ncol) fail("sparsity dimensions overflow");
       if (v[2] == 1 && ncol > 0 && v.size() == 3) {
         n = nrow * ncol;  // dense, short form
       } else {
@@ -243,26 +244,52 @@ class Reader {
   }
 
   // SXElem: an SX node graph (SXNode::serialize, casadi/core/sx_node.cpp:238-274).
+  // A node's definition holds its dependencies' (definitions or references)
+  // and it is numbered after them.  Read with an explicit stack: the graph of
+  // a long unrolled recurrence is thousands of nodes deep.
   void sx_elem(const char* name) {
-    field(name);
-    shared('E', [&] {
-      const int64_t op = casadi_int("SXNode::op");
-      if (op_ndeps_binary(op)) {
-        sx_elem("UnarySX::dep0");
-        sx_elem("UnarySX::dep1");
-      } else if (op_ndeps_unary(op)) {
-        sx_elem("UnarySX::dep");
-      } else if (op == OP_PARAMETER) {
-        str("SymbolicSX::name");
-      } else if (op == OP_CONST) {
-        const char type = character("ConstantSX::type");
-        if (type == 'r') real("ConstantSX::value");
-        else if (type == 'i') integer("ConstantSX::value");
-        else if (type == 0 || std::strchr("01nfFm", type) == nullptr) fail("constant type");
+    struct Open {
+      int n_dep, next;  // dependencies of the node, the next one to read
+    };
+    std::vector<Open> open;
+    const char* fname = name;
+    for (;;) {
+      field(fname);
+      deco('E');
+      const char flag = character("Shared::flag");
+      bool complete = true;  // this element is read in full
+      if (flag == 'r') {
+        const int64_t k = casadi_int("Shared::reference");
+        if (k < 0 || k >= n_shared_) fail("reference to an object not yet defined");
+      } else if (flag == 'd') {
+        const int64_t op = casadi_int("SXNode::op");
+        if (op_ndeps_binary(op) || op_ndeps_unary(op)) {
+          open.push_back({op_ndeps_binary(op) ? 2 : 1, 0});
+          complete = false;
+        } else if (op == OP_PARAMETER) {
+          str("SymbolicSX::name");
+        } else if (op == OP_CONST) {
+          const char type = character("ConstantSX::type");
+          if (type == 'r') real("ConstantSX::value");
+          else if (type == 'i') integer("ConstantSX::value");
+          else if (type == 0 || std::strchr("01nfFm", type) == nullptr) fail("constant type");
+        } else {
+          fail("SX node op " + std::to_string(op) + " (call / output nodes are not read)");
+        }
+        if (complete) ++n_shared_;
       } else {
-        fail("SX node op " + std::to_string(op) + " (call / output nodes are not read)");
+        fail(std::string("shared-object flag '") + flag + "'");
       }
-    });
+      // close the nodes whose last dependency this was
+      while (complete && !open.empty() && open.back().next == open.back().n_dep) {
+        open.pop_back();
+        ++n_shared_;
+      }
+      if (open.empty()) return;
+      Open& o = open.back();
+      fname = o.n_dep == 1 ? "UnarySX::dep" : o.next == 0 ? "UnarySX::dep0" : "UnarySX::dep1";
+      ++o.next;
+    }
   }
   void sx_matrix() {  // Matrix<SXElem>::serialize (casadi/core/matrix_impl.hpp:3049-3052)
     sparsity("Matrix::sparsity");
@@ -350,10 +377,12 @@ class Reader {
   }
   void nested_function(const char* name) {  // a Function member: a shared object
     field(name);
+    if (++fn_depth_ > kMaxFunctionDepth) fail("Functions nested more than 64 deep");
     shared('F', [&] {
       SerialFunction f;
       function(&f);
     });
+    --fn_depth_;
   }
 
   // The SXFunction body with its bases, in the reference's order.
@@ -474,6 +503,10 @@ class Reader {
  private:
   // nonzeros of each Sparsity definition, by shared index
   std::vector<std::pair<int64_t, int64_t>> sp_nnz_;
+  // Function members inside Function members (derivative_of, options):
+  // each level recurses, so the depth is bounded
+  static constexpr int kMaxFunctionDepth = 64;
+  int fn_depth_ = 0;
   static bool op_ndeps_binary(int64_t op) { return op >= 0 && op < 1000 && op_is_binary(static_cast<int>(op)); }
   static bool op_ndeps_unary(int64_t op) {
     return op >= 0 && op < 1000 && !op_is_binary(static_cast<int>(op)) && op_ndeps(static_cast<int>(op)) == 1 &&

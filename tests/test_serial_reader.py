@@ -194,3 +194,56 @@ def test_cartpole_step_stream_gives_the_compiled_tape():
     assert _records(t) == (alg, f.sz_w())
     assert t.nnz_in == [4, 1] and t.nnz_out == [4]
     assert s.n_shared > 100      # the graph was really written (and skipped by the reader)
+
+
+def test_deep_expression_graph():
+    """A chain of 200 000 nested sin nodes (an unrolled recurrence is this
+    deep): the reader walks the graph without recursion, numbers the nodes
+    after their dependencies, and hands over the algorithm."""
+    from casadi_amd.ops import OP
+    N = 200000
+    s = S.Stream()
+    S.function_header(s, "chain", [SCALAR], [None], ["x"], ["y"], sp_refs_out=[0])
+    s.integer(1, "XFunction::serialization::version")
+    s._name("XFunction::in")
+    s._deco("V")
+    s.casadi_int(1)
+    s.sparsity(None, "Matrix::sparsity", ref=0)
+    s._name("Matrix::nonzeros")
+    s._deco("V")
+    s.casadi_int(1)
+    kx = s.shared_def("E", S.sx_symbol(s, "x"))
+    alg = [(int(OP.INPUT), 0, 0, 0)] + [(int(OP.SIN), 0, 0, 0)] * N + [(int(OP.OUTPUT), 0, 0, 0)]
+    S.sx_function_body(s, 1, alg, 0, 0)
+    s._name("XFunction::out")
+    s._deco("V")
+    s.casadi_int(1)
+    s.sparsity(None, "Matrix::sparsity", ref=0)
+    s._name("Matrix::nonzeros")
+    s._deco("V")
+    s.casadi_int(1)
+    for j in range(N):                  # sin(sin(...)): each node's dependency defined inside it
+        s._name("" if j == 0 else "UnarySX::dep")
+        s._deco("E")
+        s.char("d", "Shared::flag")
+        s.casadi_int(int(OP.SIN), "SXNode::op")
+    s.shared_ref("E", kx, "UnarySX::dep")
+    t = HipTape.deserialize(s.text())
+    assert _records(t) == (alg, 1)
+    # a reference to a node the chain has not finished defining is refused
+    s2 = S.Stream()
+    S.function_header(s2, "chain", [SCALAR], [None], ["x"], ["y"], sp_refs_out=[0])
+    s2.integer(1, "XFunction::serialization::version")
+    s2._name("XFunction::in")
+    s2._deco("V")
+    s2.casadi_int(1)
+    s2.sparsity(None, "Matrix::sparsity", ref=0)
+    s2._name("Matrix::nonzeros")
+    s2._deco("V")
+    s2.casadi_int(1)
+    s2._deco("E")
+    s2.char("d", "Shared::flag")
+    s2.casadi_int(int(OP.SIN), "SXNode::op")
+    s2.shared_ref("E", s2.n_shared, "UnarySX::dep")   # itself: not yet numbered
+    with pytest.raises(Exception, match="not yet defined"):
+        HipTape.deserialize(s2.text())
