@@ -132,6 +132,19 @@ def main():
     except RuntimeError as e:   # capture refused: report it, keep the direct figure
         ms_graph = f"capture failed: {e}"
     ms_plain = timed(plain, max(2, args.steps // 4), 1)
+    # the same two launches on the specialised engine: what fusion itself
+    # gains there (the plain path above also pays for the interpreter)
+    tape_fj, tape_rj = HipTape(f, engine="jit"), HipTape(R, engine="jit")
+    t0 = time.perf_counter()
+    tape_fj.eval([X0, U], n)
+    tape_rj.eval([X0, U, None, SEED], n)
+    torch.cuda.synchronize()
+    compile_plain_s = time.perf_counter() - t0
+
+    def plain_jit():
+        tape_fj.eval([X0, U], n)
+        return tape_rj.eval([X0, U, None, SEED], n)
+    ms_plain_jit = timed(plain_jit, max(2, args.steps // 4), 1)
     ms_fused_interp = timed(lambda: tape_i.fwd_adj([X0, U], [SEED], n, checkpoint_every=every),
                             max(2, args.steps // 4), 1)
 
@@ -286,6 +299,11 @@ def main():
         "plain": {"value": plain_value, "ms_per_step": ms_plain, "engine": "interpreter",
                   "tapes": [st_f["n_instr"], st_r["n_instr"]]},
         "speedup_vs_plain": value / plain_value,
+        "plain_specialised": {"value": n / (ms_plain_jit * 1e-3), "ms_per_step": ms_plain_jit,
+                              "first_call_s": compile_plain_s,
+                              "fusion_gain": ms_plain_jit / ms_fused,
+                              "note": "f and its reverse(1) Function as two specialised launches (the reverse "
+                                      "Function's tape does not reroll: straight-line segments)"},
         "fused_graph_replay": ({"ms_per_step": ms_graph, "value": n / (ms_graph * 1e-3),
                                 "vs_direct": ms_fused / ms_graph}
                                if isinstance(ms_graph, float) else {"error": ms_graph}),

@@ -136,12 +136,71 @@ struct JitPlan {
   int64_t snap_rows = 0;  // sub-block boundary states of one reverse kernel (max over segments)
   int ipt = 1;            // instances per thread of the forward kernels
   std::vector<hipFunction_t> fwd, rev;
-  std::vector<hipFunction_t> fwd_soa;          // forward kernels for all-SoA launches (32-bit instance offsets)
   bool trig_cache = false;                     // pass 1 stores sin/cos pairs for the recomputation
   std::vector<int> n_sub;                      // sub-blocks of each segment's reverse kernel
   std::vector<hipFunction_t> snap, back, pair;  // a reverse kernel split: first pass, the rest, and
                                                 // pair[g] = the rest of g beside the first pass of g-1
+  // Forward-only evaluation keeps every value that crosses a boundary in one
+  // home row for its whole life (a row is reused once its value is dead), so
+  // a value that passes through a segment unread costs that segment nothing.
+  // home[s][r]: the row of the value in live slot live[s][r] (boundaries
+  // 1 .. n_seg-1); fresh[s][r]: that value is defined in segment s-1, which
+  // stores it.  Built at the first evaluation (home_rows < 0 until then).
+  std::vector<std::vector<int>> home;
+  std::vector<std::vector<char>> fresh;
+  int home_rows = -1;
+  std::vector<hipFunction_t> fwd_home, fwd_home_soa;  // evaluation kernels on the home rows
 };
+
+// The home rows of a plan's boundaries: values are (slot, defining
+// instruction); at each boundary the rows of values dead there are freed
+// first, then each value new there takes a free row.
+static void home_plan(const cxh_tape_s* t, JitPlan& P) {
+  if (P.home_rows >= 0) return;
+  P.home.assign(P.n_seg + 1, {});
+  P.fresh.assign(P.n_seg + 1, {});
+  std::vector<int64_t> def(static_cast<size_t>(t->ws), -1);
+  std::map<std::pair<int, int64_t>, int> row_of;  // live value -> its row
+  std::vector<int> free_rows;
+  int rows = 0;
+  int b = 1;
+  for (int64_t k = 0; k <= static_cast<int64_t>(t->alg.size()) && b < P.n_seg; ++k) {
+    while (b < P.n_seg && P.t0[b] == k) {
+      const std::vector<int>& L = P.live[b];
+      std::map<std::pair<int, int64_t>, int> next;
+      for (int j : L) {
+        auto it = row_of.find({j, def[j]});
+        if (it != row_of.end()) next.insert(*it);
+      }
+      for (const auto& kv : row_of)
+        if (!next.count(kv.first)) free_rows.push_back(kv.second);
+      P.home[b].resize(L.size());
+      P.fresh[b].resize(L.size());
+      for (size_t r = 0; r < L.size(); ++r) {
+        const std::pair<int, int64_t> v{L[r], def[L[r]]};
+        auto it = next.find(v);
+        if (it == next.end()) {
+          int row;
+          if (!free_rows.empty()) {
+            row = free_rows.back();
+            free_rows.pop_back();
+          } else {
+            row = rows++;
+          }
+          it = next.emplace(v, row).first;
+          P.fresh[b][r] = 1;
+        }
+        P.home[b][r] = it->second;
+      }
+      row_of.swap(next);
+      ++b;
+    }
+    if (k < static_cast<int64_t>(t->alg.size()) && t->alg[k].op != OP_OUTPUT) def[t->alg[k].i0] = k;
+  }
+  P.home_rows = rows;
+  P.fwd_home.assign(P.n_seg, nullptr);
+  P.fwd_home_soa.assign(P.n_seg, nullptr);
+}
 
 // Instances per thread of the forward segment kernels (CXH_JIT_IPT, 1-8):
 // with several, the loads of the later instances are issued while the first
@@ -159,7 +218,8 @@ static void sync_ipt(const cxh_tape_s* t, JitPlan& P) {
   if (P.ipt == want) return;
   P.ipt = want;
   std::fill(P.fwd.begin(), P.fwd.end(), nullptr);
-  std::fill(P.fwd_soa.begin(), P.fwd_soa.end(), nullptr);
+  std::fill(P.fwd_home.begin(), P.fwd_home.end(), nullptr);
+  std::fill(P.fwd_home_soa.begin(), P.fwd_home_soa.end(), nullptr);
 }
 
 struct JitTape {
@@ -294,7 +354,6 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
     P.n_sub[s] = static_cast<int>(c.size()) - 1;
   }
   P.fwd.assign(P.n_seg, nullptr);
-  P.fwd_soa.assign(P.n_seg, nullptr);
   P.rev.assign(P.n_seg, nullptr);
   P.snap.assign(P.n_seg, nullptr);
   P.back.assign(P.n_seg, nullptr);
@@ -357,7 +416,7 @@ static std::string adj_name(int64_t c, int64_t ws) {
 // rest, with the last sub-block reloaded from its stored state as well.  The
 // two halves of consecutive segments then run side by side (jit_adj).
 static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, const std::string& name,
-                            bool sub_blocks = true, int phase = 0) {
+                            bool sub_blocks = true, int phase = 0, bool home = false) {
   const int t0 = P.t0[s], t1 = P.t0[s + 1];
   const int len = t1 - t0;
   const int64_t ws = t->ws;
@@ -367,12 +426,56 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
   std::vector<int64_t> cur(ws, 0);
   const std::vector<int>& lin = P.live[s];
   const std::vector<int>& lout = P.live[s + 1];
+  // Forward segments load only the live-in values they read: the others pass
+  // through to the live-out rows in copy loops (below), so that a wide live
+  // set (a reverse Function keeps every forward value its sweep needs) does
+  // not become thousands of values the compiler holds in registers.
+  std::vector<char> read_in(static_cast<size_t>(ws), rev ? 1 : 0);
+  if (!rev) {
+    std::vector<char> written(static_cast<size_t>(ws), 0);
+    for (int k = t0; k < t1; ++k) {
+      const Instr& e = t->alg[k];
+      if (e.op == OP_CONST || e.op == OP_INPUT) {
+        written[e.i0] = 1;
+        continue;
+      }
+      if (!written[e.i1]) read_in[e.i1] = 1;
+      if (op_is_binary(e.op) && e.op != OP_OUTPUT && !written[e.i2]) read_in[e.i2] = 1;
+      if (e.op != OP_OUTPUT) written[e.i0] = 1;
+    }
+  }
+  std::unordered_map<int, int> lin_row;  // forward segments: live-in slot -> its row, loaded at first read
   for (size_t r = 0; r < lin.size(); ++r) {
     const int j = lin[r];
     cur[j] = -(j + 1);
+    if (!read_in[j]) continue;
+    if (!rev && s > 0) {
+      lin_row[j] = home ? P.home[s][r] : static_cast<int>(r);
+      continue;
+    }
     o += "  const double w" + std::to_string(j) + " = ";
     o += s > 0 ? "CXJ_ST_IN(" + std::to_string(r) + ");\n" : "0.0;\n";
   }
+  // A live-in value is loaded where the segment first reads it, through a
+  // pointer made opaque after the value defined kLoadAhead definitions
+  // earlier: the load can be issued no sooner, so the compiler does not
+  // hoist every load of a wide live set to the top and hold them all.
+  constexpr size_t kLoadAhead = 32;
+  std::vector<std::string> defined;  // names of the values this segment has defined, in order
+  int n_ptr = 0;
+  auto load_live_in = [&](int64_t c) {
+    if (c >= 0) return;
+    const int j = static_cast<int>(-c - 1);
+    auto it = lin_row.find(j);
+    if (it == lin_row.end()) return;  // eager, or already loaded
+    const std::string p = "cxj_q" + std::to_string(n_ptr++);
+    o += "  CXJ_GLOBAL const double* " + p + " = (CXJ_GLOBAL const double*)A.st_in;\n";
+    if (defined.size() >= kLoadAhead)
+      o += "  CXJ_OPAQUE_AFTER(" + p + ", " + defined[defined.size() - kLoadAhead] + ");\n";
+    o += "  const double w" + std::to_string(j) + " = " + p + "[(long long)" + std::to_string(it->second) +
+         " * A.lds + k];\n";
+    lin_row.erase(it);
+  };
   auto V = [&](int64_t c) { return val_name(c, ws); };
   auto Adj = [&](int64_t c) { return adj_name(c, ws); };
   std::vector<int64_t> ox(len, 0), oy(len, 0), vc(len, 0);
@@ -411,6 +514,11 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
       const Instr& e = t->alg[k];
       const int64_t code = 4 * static_cast<int64_t>(k) + pass;
       const std::string nm = V(code);
+      if (!lin_row.empty() && e.op != OP_CONST && e.op != OP_INPUT) {
+        load_live_in(cur[e.i1]);
+        if (e.op != OP_OUTPUT && op_is_binary(e.op)) load_live_in(cur[e.i2]);
+      }
+      if (e.op != OP_OUTPUT) defined.push_back(nm);
       if ((e.op == OP_SIN || e.op == OP_COS) && (all_trig || pairs.count(cur[e.i1]))) {
         const int64_t x = cur[e.i1];
         auto it = trig.find(x);
@@ -467,11 +575,48 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
       }
     }
   };
-  if (!rev) {
+  if (!rev && home) {  // the values defined here that cross the end go to their home rows
     emit_fwd(t0, t1, 0, true);
     if (s + 1 < P.n_seg)
       for (size_t r = 0; r < lout.size(); ++r)
-        o += "  CXJ_ST_OUT(" + std::to_string(r) + ") = " + V(cur[lout[r]]) + ";\n";
+        if (P.fresh[s + 1][r])
+          o += "  CXJ_ST_OUT(" + std::to_string(P.home[s + 1][r]) + ") = " + V(cur[lout[r]]) + ";\n";
+    o += "}\n";
+    return o;
+  }
+  if (!rev) {
+    emit_fwd(t0, t1, 0, true);
+    if (s + 1 < P.n_seg) {
+      std::unordered_map<int, int> row_in;
+      for (size_t r = 0; r < lin.size(); ++r) row_in[lin[r]] = static_cast<int>(r);
+      std::vector<std::pair<int, int>> pass;  // (live-in row, live-out row) of values passed through unread
+      for (size_t r = 0; r < lout.size(); ++r) {
+        const int j = lout[r];
+        if (cur[j] == -(j + 1) && !read_in[j]) {
+          if (s > 0) pass.push_back({row_in.at(j), static_cast<int>(r)});
+          else o += "  CXJ_ST_OUT(" + std::to_string(r) + ") = 0.0;\n";
+        } else {
+          o += "  CXJ_ST_OUT(" + std::to_string(r) + ") = " + V(cur[j]) + ";\n";
+        }
+      }
+      // runs of consecutive rows as loops (one copy per iteration)
+      for (size_t i = 0; i < pass.size();) {
+        size_t m = 1;
+        while (i + m < pass.size() && pass[i + m].first == pass[i].first + static_cast<int>(m) &&
+               pass[i + m].second == pass[i].second + static_cast<int>(m))
+          ++m;
+        const std::string ri = std::to_string(pass[i].first), ro = std::to_string(pass[i].second);
+        if (m <= 2) {
+          for (size_t q = 0; q < m; ++q)
+            o += "  CXJ_ST_OUT(" + std::to_string(pass[i + q].second) + ") = CXJ_ST_IN(" +
+                 std::to_string(pass[i + q].first) + ");\n";
+        } else {
+          o += "#pragma unroll 1\n  for (int i = 0; i < " + std::to_string(m) + "; ++i) CXJ_ST_OUT(" + ro +
+               " + i) = CXJ_ST_IN(" + ri + " + i);\n";
+        }
+        i += m;
+      }
+    }
     o += "}\n";
     return o;
   }
@@ -641,11 +786,11 @@ static std::string gen_body(const cxh_tape_s* t, const JitPlan& P, int s, bool r
 // One segment kernel: one lane per instance, the arguments by value.  phase
 // (reverse kernels): 0 whole, 1 first pass (cxj_snap), 2 the rest.
 static std::string gen_segment(const cxh_tape_s* t, const JitPlan& P, int s, bool rev, int phase = 0,
-                               bool soa32 = false) {
+                               bool soa32 = false, bool home = false) {
   std::string o = soa32 ? "#define CXJ_SOA32\n" : "";
   o += kSxOpsSource;
   o += kPrologue;
-  o += gen_body(t, P, s, rev, "cxj_body", true, rev ? phase : 0);
+  o += gen_body(t, P, s, rev, "cxj_body", true, rev ? phase : 0, home && !rev);
   if (rev || P.ipt == 1) {
     o += std::string("extern \"C\" __global__ void __launch_bounds__(256) ") +
          (rev ? (phase == 1 ? "cxj_snap" : "cxj_rev") : "cxj_fwd") +
@@ -773,11 +918,12 @@ static hipFunction_t segment_kernel(cxh_tape_s* t, JitPlan& P, int s, bool rev) 
   return f;
 }
 
-// The forward kernel of segment s for launches whose blocks are all SoA.
-static hipFunction_t segment_kernel_soa(cxh_tape_s* t, JitPlan& P, int s) {
-  hipFunction_t& f = P.fwd_soa[s];
+// The evaluation kernel of segment s on the home rows (32-bit offsets when soa32).
+static hipFunction_t segment_kernel_home(cxh_tape_s* t, JitPlan& P, int s, bool soa32) {
+  home_plan(t, P);
+  hipFunction_t& f = soa32 ? P.fwd_home_soa[s] : P.fwd_home[s];
   if (!f)
-    f = jit_function_checked(gen_segment(t, P, s, false, 0, true), "cxj_fwd",
+    f = jit_function_checked(gen_segment(t, P, s, false, 0, soa32, true), "cxj_fwd",
                              "use a shorter checkpoint interval or the interpreter engine");
   return f;
 }
@@ -876,8 +1022,8 @@ static void jit_prepare_eval(cxh_tape_s* t) {
   JitPlan& P = eval_plan(t);
   sync_ipt(t, P);
   for (int g = 0; g < P.n_seg; ++g) {
-    segment_kernel(t, P, g, false);
-    segment_kernel_soa(t, P, g);
+    segment_kernel_home(t, P, g, false);
+    segment_kernel_home(t, P, g, true);
   }
 }
 
@@ -1021,14 +1167,16 @@ static JitPlan& eval_plan(cxh_tape_s* t) {
   if (t->jit && t->jit->eval_every > 0) return plan_for(t, t->jit->eval_every);
   for (int64_t every = default_every(static_cast<int64_t>(t->alg.size()));; every = (every + 1) / 2) {
     JitPlan& P = plan_for(t, every);
+    sync_ipt(t, P);
+    home_plan(t, P);
     bool ok = true;
     for (int g = 0; g < P.n_seg && ok; ++g) {
-      if (P.fwd[g]) continue;
-      hipFunction_t f = jit_function(gen_segment(t, P, g, false), "cxj_fwd");
+      if (P.fwd_home[g]) continue;
+      hipFunction_t f = jit_function(gen_segment(t, P, g, false, 0, false, true), "cxj_fwd");
       if (scratch_bytes(f) > 0) {
         ok = false;
       } else {
-        P.fwd[g] = f;
+        P.fwd_home[g] = f;
       }
     }
     if (ok) {
@@ -1043,19 +1191,21 @@ int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& re
   if (roll_active(t)) return roll_map_eval(t, n, arg, res, s);
   JitPlan& P = eval_plan(t);
   sync_ipt(t, P);
+  home_plan(t, P);
   const int64_t lds = round_up(n, 64);
   CxjArgs A{};
   A.arg = arg;
   A.n = n;
   A.lds = lds;
-  if (P.n_seg > 1) t->jit->st.reserve(2 * static_cast<size_t>(std::max(P.max_live, 1)) * lds * sizeof(double));
-  double* st = t->jit->st.as<double>();
+  // one buffer of home rows: a segment reads and writes the values' own rows
+  if (P.n_seg > 1) t->jit->st.reserve(static_cast<size_t>(std::max(P.home_rows, 1)) * lds * sizeof(double));
+  double* st = P.n_seg > 1 ? t->jit->st.as<double>() : nullptr;
   const bool soa = soa32_launch(arg, t->nnz_in.size(), res, t->nnz_out.size(), n);
   for (int g = 0; g < P.n_seg; ++g) {
-    hipFunction_t f = soa ? segment_kernel_soa(t, P, g) : segment_kernel(t, P, g, false);
+    hipFunction_t f = segment_kernel_home(t, P, g, soa);
     A.res = res;
-    A.st_in = st ? st + static_cast<int64_t>(g % 2) * P.max_live * lds : nullptr;
-    A.st_out = st ? st + static_cast<int64_t>((g + 1) % 2) * P.max_live * lds : nullptr;
+    A.st_in = st;
+    A.st_out = st;
     launch(f, (n + P.ipt - 1) / P.ipt, t->block, A, s);
   }
   return 0;
@@ -1145,7 +1295,11 @@ int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTabl
 std::string jit_source(cxh_tape_s* t, int64_t every, int seg, int rev) {
   JitPlan& P = plan_for(t, every);
   if (seg < 0 || seg >= P.n_seg) throw std::runtime_error("segment index out of range");
-  if (rev < 0 || rev > 4) throw std::runtime_error("reverse must be 0, 1, 2, 3 or 4");
+  if (rev < 0 || rev > 5) throw std::runtime_error("reverse must be 0 .. 5");
+  if (rev == 5) {  // forward-only evaluation on the home rows
+    home_plan(t, P);
+    return gen_segment(t, P, seg, false, 0, false, true);
+  }
   if (rev == 4) {
     if (seg < 1) throw std::runtime_error("a pair kernel pairs segment g >= 1 with g-1");
     return pair_source(t, P, seg);
@@ -1161,6 +1315,8 @@ void jit_plan_info(cxh_tape_s* t, int64_t every, int64_t* info, int64_t* ck_off)
   info[3] = P.max_live;
   info[4] = split_plan(P) ? 1 : 0;
   info[5] = P.ipt;
+  home_plan(t, P);
+  info[6] = P.home_rows;
   if (ck_off)
     for (int s = 0; s <= P.n_seg; ++s) ck_off[s] = P.ck_off[s];
 }

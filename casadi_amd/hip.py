@@ -204,13 +204,14 @@ class HipTape:
 
     def jit_plan(self, checkpoint_every=0):
         """The plan of a checkpoint interval as the drivers size their buffers
-        from it: dict of segments, ck_rows, snap_rows, max_live, split, ipt
-        and ck_off (first checkpoint row of each boundary)."""
-        info = (L.I64 * 6)()
+        from it: dict of segments, ck_rows, snap_rows, max_live, split, ipt,
+        home_rows (forward-only evaluation) and ck_off (first checkpoint row of
+        each boundary)."""
+        info = (L.I64 * 7)()
         L.check(L.cxh_tape_jit_plan(self._h, int(checkpoint_every), info, None), "cxh_tape_jit_plan")
         off = (L.I64 * (info[0] + 1))()
         L.check(L.cxh_tape_jit_plan(self._h, int(checkpoint_every), info, off), "cxh_tape_jit_plan")
-        keys = ("segments", "ck_rows", "snap_rows", "max_live", "split", "ipt")
+        keys = ("segments", "ck_rows", "snap_rows", "max_live", "split", "ipt", "home_rows")
         d = {k: int(v) for k, v in zip(keys, info)}
         d["ck_off"] = [int(v) for v in off]
         return d

@@ -177,14 +177,17 @@ int cxh_tape_jit_pending(cxh_tape t);
  * (NUL-terminated, truncated to cap), its full length in *len and the number
  * of segments.  reverse: 0 the forward kernel, 1 the whole recompute +
  * reverse-sweep kernel, 2 and 3 its two halves when split (2: the first pass
- * storing the sub-block boundary states, 3: the rest).                      */
+ * storing the sub-block boundary states, 3: the rest), 5 the kernel of
+ * forward-only evaluation (each value crossing a boundary in its home row). */
 int cxh_tape_jit_source(cxh_tape t, int64_t checkpoint_every, int32_t segment, int32_t reverse,
                         char* buf, int64_t cap, int64_t* len, int32_t* n_segments);
 /* Introspection: the plan of a checkpoint interval as the drivers size their
- * buffers from it.  info[6] = {segments, checkpoint rows, sub-block state rows
+ * buffers from it.  info[7] = {segments, checkpoint rows, sub-block state rows
  * of one reverse kernel, largest live set at a segment boundary, whether the
  * reverse sweep runs split (1) or one kernel per segment (0), instances per
- * thread of the forward kernels}; ck_off (may be NULL; segments + 1 entries):
+ * thread of the forward kernels, home rows of forward-only evaluation (one
+ * buffer that every evaluation segment reads and writes)}; ck_off (may be
+ * NULL; segments + 1 entries):
  * first checkpoint row of each boundary.  reverse = 4 in cxh_tape_jit_source
  * gives the source of the launch pairing segment `segment`'s reverse half
  * with the first pass of the segment before it.                            */

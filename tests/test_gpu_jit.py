@@ -280,3 +280,31 @@ def test_jit_settings_keep_the_bits(ca, cuda, monkeypatch, setting):
     go = np.concatenate([ox, ou], axis=1)
     scale = np.maximum(np.max(np.abs(go), axis=1, keepdims=True), 1.0)
     assert np.median(np.max(np.abs(g - go) / scale, axis=1)) <= TOL
+
+
+def test_jit_wide_live_set_reverse_function(ca, cuda):
+    """A reverse Function (reverse(1) of a 40-step rollout: its forward half's
+    values stay live through its reverse half, thousands at every segment
+    boundary) on the specialised forward kernels: live-in values loaded where
+    read, the rest copied through in loops.  Same bits as the interpreter; the
+    oracle on a 10-step one at 1e-13."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    for steps, n in ((40, 2000), (10, 300)):
+        f = W.cartpole_rollout(steps)
+        R = f.reverse(1)
+        rng = np.random.default_rng(seed_of(41) + steps)
+        x0 = rng.uniform(-1, 1, (4, n))
+        us = rng.uniform(-10, 10, (steps, n))
+        sd = rng.standard_normal((1, n))
+        print(f"[case] reverse Function of the {steps}-step rollout ({R.n_instructions()} instructions) n={n}",
+              flush=True)
+        args = _dev(x0, us) + [None] + _dev(sd)
+        yi = [t.cpu().numpy() for t in HipTape(R).eval(args, n)]
+        yj = [t.cpu().numpy() for t in HipTape(R, engine="jit").eval(args, n)]
+        for a, b in zip(yi, yj):
+            assert np.array_equal(a, b)
+        if steps == 10:
+            ref = O.map_eval(R, n, [x0.T, us.T, None, sd.T])
+            for a, r in zip(yj, ref):
+                assert O.rel_err(a, r.T) <= TOL

@@ -56,9 +56,15 @@ def test_segment_state_is_the_live_set(ca, monkeypatch):
     for s in range(nseg):
         src, ns = t.jit_source(every, s, reverse=False)
         assert ns == nseg
-        loads = sorted(int(m) for m in re.findall(r"const double w(\d+) = CXJ_ST_IN", src))
-        stores = len(re.findall(r"CXJ_ST_OUT\(\d+\) =", src))
-        assert set(loads) == (live[s * every] if s > 0 else set())
+        # a live-in value is loaded where the segment reads it, or copied to
+        # its live-out row unread (one by one or in a loop over a run of rows)
+        loads = {int(m) for m in re.findall(r"const double w(\d+) = (?:CXJ_ST_IN\(|cxj_q\d+\[)", src)}
+        direct = re.findall(r"CXJ_ST_OUT\(\d+\) = CXJ_ST_IN\(\d+\);", src)
+        runs = [int(m) for m in re.findall(r"for \(int i = 0; i < (\d+); \+\+i\) CXJ_ST_OUT\(\d+ \+ i\) = CXJ_ST_IN", src)]
+        passed = len(direct) + sum(runs)
+        stores = len(re.findall(r"CXJ_ST_OUT\(\d+\) =", src)) + sum(runs)
+        lin = live[s * every] if s > 0 else set()
+        assert loads <= lin and len(loads) + passed == len(lin)
         assert stores == (len(live[min((s + 1) * every, ni)]) if s + 1 < nseg else 0)
         rsrc, _ = t.jit_source(every, s, reverse=True)
         assert len(re.findall(r"CXJ_ADJ_OUT\(\d+\) =", rsrc)) == (len(live[s * every]) if s > 0 else 0)

@@ -294,3 +294,48 @@ def test_tangent_kernels_in_bounds_and_race_free(monkeypatch, settings):
     H, _ = ca.hessian(Lag, w)
     (ref,) = O.map_eval(ca.Function("h", [w, lam], [H]), n, [wv.T, lv.T])
     assert O.rel_err(jb.reshape(nnz_jac, n), ref.T) <= 1e-13
+
+
+@pytest.mark.parametrize("steps,every,n", [(4, 150, 130), (6, 701, 70)])
+def test_evaluation_home_rows_in_bounds_and_exact(monkeypatch, steps, every, n):
+    """Forward-only evaluation on the home rows: every value that crosses a
+    segment boundary keeps one row of one buffer for its whole life, rows of
+    dead values are reused, values passed through a segment unread are not
+    touched.  The reverse Function of a rollout (its forward values stay live
+    through its reverse half: wide live sets, many pass-throughs), launched
+    segment by segment as the driver does: in bounds, race-free, and the
+    oracle's outputs bit for bit."""
+    monkeypatch.setenv("CXH_JIT_ROLL", "0")
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(steps)
+    R = f.reverse(1)
+    tape = HipTape(R, engine="jit")
+    P = tape.jit_plan(every)
+    S = P["segments"]
+    assert S > 2 and P["ipt"] == 1
+    assert 0 < P["home_rows"] <= P["max_live"]      # rows reused: never more than the widest live set
+    srcs = [tape.jit_source(every, g, 5)[0] for g in range(S)]
+    rng = np.random.default_rng(20261211 + n)
+    ins = [rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (steps, n)), None, rng.standard_normal((1, n))]
+    with tempfile.TemporaryDirectory() as d:
+        emu = Emu(srcs, d)
+        lds = _round64(n)
+        bufs = []
+        for a in ins:
+            if a is None:
+                bufs.append(None)
+                continue
+            b = emu.buffer(a.size)
+            b[:] = a.ravel()
+            bufs.append(b)
+        outs = [emu.buffer(R.nnz_out(j) * n) for j in range(R.n_out())]
+        home = emu.buffer(P["home_rows"] * lds)
+        A = CxjArgs()
+        A.arg, A.res, A.n, A.lds = _table(bufs, n), _table(outs, n), n, lds
+        A.st_in = A.st_out = _ptr(home)
+        errors = sum(emu.launch(g, A, n) for g in range(S))
+        assert errors == 0
+        ref = O.map_eval(R, n, [None if a is None else a.T for a in ins])
+        for y, r in zip(outs, ref):
+            assert np.array_equal(y.reshape(-1, n), r.T)
