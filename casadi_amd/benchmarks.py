@@ -10,6 +10,7 @@ c4     heterogeneous batch: 40000 cart-pole + 12000 quadrotor + 25000 arm,
        forward + adjoint, one hipGraph launch (autotuned) vs plain launches
 c5     batched KKT solve, n = 274, 8192 instances, ldl and qr
 c3macc the 100-step rollout cost + gradient as a mapaccum of one step, 32768
+c3interp the 100-step rollout cost + gradient on the interpreter's fused sweep, 32768
 mapsum reduced map: cart-pole step over 1M states, shared control, summed
 sparsity Jacobian patterns from dependency sweeps, host against GPU
 
@@ -155,6 +156,21 @@ def rollout_mapaccum(n=32768, N=100, K=10):
     _out(k="rollout_mapaccum_eval", n=n, N=N, ms=ms)
 
 
+def rollout_interp(n=32768, N=100, K=10):
+    """The rollout cost + gradient on the tape interpreter's fused sweep
+    (tape_adj_kernel), checkpoints every K steps: three calls."""
+    from . import workloads as W
+    from .hip import HipTape
+    f = W.cartpole_rollout(N)
+    rng = np.random.default_rng(20261204)
+    X0, U = _dev(rng.uniform(-1, 1, (4, n))), _dev(rng.uniform(-10, 10, (N, n)))
+    S = _dev(np.ones((1, n)))
+    t = HipTape(f)
+    every = (f.n_instructions() + K - 1) // K
+    ms = _timed(lambda: t.fwd_adj([X0, U], [S], n, checkpoint_every=every), 2)
+    _out(k="rollout_interp_fwd_adj", n=n, N=N, K=K, ms=ms, inst_per_s=n / ms * 1e3)
+
+
 def kkt(n=8192):
     from . import workloads as W
     from .linsol import Linsol
@@ -250,6 +266,8 @@ def main(argv):
         kkt()
     if "c3macc" in which:
         rollout_mapaccum()
+    if "c3interp" in which:
+        rollout_interp()
     if "mapsum" in which:
         mapsum()
     if "sparsity" in which:
