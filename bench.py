@@ -38,7 +38,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=N_BATCH)
     p.add_argument("--cpu-sample", type=int, default=16384, help="instances timed on the CPU oracle")
     p.add_argument("--no-cpu", action="store_true")
@@ -113,6 +113,18 @@ def main():
         return e0.elapsed_time(e1) / steps  # ms per step
 
     ms_fused = timed(fused, args.steps, args.warmup)
+
+    # the same steps timed one by one (events around each): the median step
+    def per_step(fn, steps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return sorted(a.elapsed_time(b) for a, b in ev)
+    steps_ms = per_step(fused, args.steps)
+    ms_median = steps_ms[len(steps_ms) // 2]
 
     # the same fused call captured once in a HIP graph and replayed (the
     # library launches on the capturing stream and allocates nothing after
@@ -283,6 +295,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_fused,
+        "ms_per_step_median": ms_median,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -440,8 +453,11 @@ def other_config_figures(timed):
     hi = HipJacobian(G, symmetric=True, first_block=True)
     (_, hi_v), _ = first(lambda: hi(args, n))
     ms_i = timed(lambda: hi(args, n), 2, 0)
+    bpi = (w.nnz() + lam.nnz() + hj.nnz_jac) * 8   # primal and multipliers in, Hessian nonzeros out
+    gbs = bpi * n / (ms * 1e-3) / 1e9
     out["hessian_16384"] = {"n": n, "ms": ms, "instances_per_s": n / (ms * 1e-3), "first_call_s": c_s,
                             "colours": hj.plan["ndir"], "nnz": hj.nnz_jac, "interpreter_ms": ms_i,
+                            "bytes_per_instance": bpi, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
                             "max_abs_diff": float((hs - hi_v).abs().max())}
     # config 4: heterogeneous batch
     fs = [W.cartpole_step(), W.quadrotor_step(), W.arm_dynamics()]
@@ -465,8 +481,12 @@ def other_config_figures(timed):
             tf.eval(A, m)
             tr.eval(A + [None] * len(S) + S, m)
     ms_p = timed(plain, 10, 1)
+    nbytes = sum(m * 16 * (sum(f.nnz_in(j) for j in range(f.n_in())) + sum(f.nnz_out(j) for j in range(f.n_out())))
+                 for f, m in zip(fs, counts))   # inputs + seeds in, outputs + sensitivities out
+    gbs = nbytes / (ms_b * 1e-3) / 1e9
     out["batch_77000"] = {"instances": sum(counts), "ms": ms_b, "ms_graph": ms_g, "ms_direct": ms_d,
                           "first_call_s": c_s, "instances_per_s": sum(counts) / (ms_b * 1e-3), "tuned": cfg,
+                          "bytes": nbytes, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
                           "plain_interpreter_ms": ms_p}
     # config 5: batched KKT solves
     n = 8192
@@ -481,7 +501,10 @@ def other_config_figures(timed):
         Lg = Linsol("kkt", solver, sp, {"engine": "interp"})
         (xg, _), _ = first(lambda: Lg.solve_batch(At, Bt))
         ms_g = timed(lambda: Lg.solve_batch(At, Bt), 3, 0)
+        bpi = (sp.nnz() + 2 * sp.size1()) * 8   # A's nonzeros and b in, x out
+        gbs = bpi * n / (ms * 1e-3) / 1e9
         out[f"kkt_{solver}_8192"] = {"n": n, "ms": ms, "instances_per_s": n / (ms * 1e-3), "first_call_s": c_s,
+                                     "bytes_per_instance": bpi, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
                                      "generic_ms": ms_g, "max_abs_diff": float((xj - xg).abs().max())}
     return out
 
