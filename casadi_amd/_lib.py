@@ -64,6 +64,7 @@ cxh_tape_config = _fn("cxh_tape_config", I32, P, I32, I32, I32)
 cxh_tape_ipt = _fn("cxh_tape_ipt", I32, P, I32)
 cxh_tape_jit_async = _fn("cxh_tape_jit_async", I32, P, I32)
 cxh_tape_jit_pending = _fn("cxh_tape_jit_pending", I32, P)
+cxh_tape_jit_fallback = _fn("cxh_tape_jit_fallback", I32, P, C.c_char_p, I64)
 cxh_tape_tune = _fn("cxh_tape_tune", I32, P, I64, PP, PP, I32, I64, I32, P, C.POINTER(I32), C.POINTER(C.c_double))
 cxh_map_eval = _fn("cxh_map_eval", I32, P, I64, PP, PP, I32, I64, P)
 cxh_map_eval_host = _fn("cxh_map_eval_host", I32, P, I64, PP, PP)

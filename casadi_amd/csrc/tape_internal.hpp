@@ -96,6 +96,12 @@ struct cxh_tape_s {
   std::shared_ptr<sxh::JitTape> jit;  // specialised kernels (jit.hip)
   std::shared_ptr<sxh::RollTape> roll;  // the tape rerolled into a loop, when it repeats (jit_roll.hip)
   std::shared_ptr<sxh::JitAsync> async;  // compiles on a worker thread (cxh_tape_jit_async), else null
+  // Specialised kernels that would need scratch are refused; the calls they
+  // would serve take the interpreter instead (same bits): evaluation, and
+  // the adjoint at these checkpoint intervals (cxh_tape_jit_fallback).
+  bool eval_interp = false;
+  std::vector<int64_t> adj_interp;
+  std::string fallback_why;
 };
 
 namespace sxh {
@@ -142,6 +148,7 @@ int jit_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& re
 int jit_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTable& aseed, const IOTable& res,
             const IOTable& asens, int64_t every, double* ckpt_ext, hipStream_t s);
 int64_t jit_ckpt_rows(cxh_tape_s* t, int64_t every);
+void jit_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev);
 
 // The same over a tape whose middle repeats, as one loop body (jit_roll.hip).
 bool roll_active(cxh_tape_s* t);

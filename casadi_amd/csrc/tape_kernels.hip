@@ -784,6 +784,16 @@ int cxh_tape_config(cxh_tape t, int32_t block, int32_t w_in_lds, int32_t ipw) {
   return 0;
 }
 
+int cxh_tape_jit_fallback(cxh_tape t, char* why, int64_t cap) {
+  if (!t) return cxh_fail("null tape");
+  if (why && cap > 0) {
+    const size_t m = std::min<size_t>(t->fallback_why.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(why, t->fallback_why.data(), m);
+    why[m] = '\0';
+  }
+  return (t->eval_interp ? 1 : 0) | (t->adj_interp.empty() ? 0 : 2);
+}
+
 int cxh_tape_ipt(cxh_tape t, int32_t ipt) {
   if (!t) return cxh_fail("null tape");
   if (ipt != 0 && ipt != 1 && ipt != 2 && ipt != 4 && ipt != 8) return cxh_fail("ipt must be 0, 1, 2, 4 or 8");
@@ -923,9 +933,40 @@ int cxh_tape_tune(cxh_tape t, int64_t n, const double* const* arg, double* const
 
 namespace sxh {
 
+// A refusal of a specialised kernel that would need scratch (the message of
+// jit_function_checked and of the segment-length search).
+static bool spill_refusal(const std::exception& e) {
+  return std::string(e.what()).find("spills") != std::string::npos;
+}
+
+// Whether the specialised adjoint kernels at this checkpoint interval can be
+// built (they are, in the calling thread); if not, the tape's adjoint at this
+// interval runs on the interpreter from now on.  Decided before any launch,
+// so that a split sweep keeps one engine's checkpoint layout.
+static bool jit_adj_usable(cxh_tape_s* t, int64_t every) {
+  if (std::find(t->adj_interp.begin(), t->adj_interp.end(), every) != t->adj_interp.end()) return false;
+  try {
+    jit_prepare(t, every, true, true);
+    return true;
+  } catch (const std::runtime_error& e) {
+    if (!spill_refusal(e)) throw;
+    t->adj_interp.push_back(every);
+    t->fallback_why = e.what();
+    return false;
+  }
+}
+
 int map_eval_io(cxh_tape t, int64_t n, const IOTable& d_arg, const IOTable& d_res, hipStream_t s) {
   if (n == 0 || t->alg.empty()) return 0;
-  if (t->engine == CXH_ENGINE_JIT && jit_ready(t, -1, true, false)) return jit_map_eval(t, n, d_arg, d_res, s);
+  if (t->engine == CXH_ENGINE_JIT && !t->eval_interp && jit_ready(t, -1, true, false)) {
+    try {
+      return jit_map_eval(t, n, d_arg, d_res, s);
+    } catch (const std::runtime_error& e) {  // no segment length without scratch: the interpreter, same bits
+      if (!spill_refusal(e)) throw;
+      t->eval_interp = true;
+      t->fallback_why = e.what();
+    }
+  }
   const GridShape g = grid_shape(t, n);
   const size_t lds = static_cast<size_t>(t->ws) * g.cols * sizeof(double);
   const bool use_lds = t->w_lds != 0 && lds <= static_cast<size_t>(device_lds_bytes());
@@ -1041,7 +1082,7 @@ int adj_io(cxh_tape t, int mode, int64_t n, const IOTable& arg, const IOTable& a
     // the split sweep (modes 1 and 2) keeps one engine's checkpoint layout
     // across its two calls, so it never changes engine half way
     if (mode != 0) jit_settle(t);
-    if (mode != 0 || jit_ready(t, every, true, true))
+    if ((mode != 0 || jit_ready(t, every, true, true)) && jit_adj_usable(t, every))
       return jit_adj(t, mode, n, arg, aseed, res, asens, every, ckpt_ext, s);
   }
   if (!t->d_live.ptr) {
@@ -1108,8 +1149,18 @@ int cxh_ckpt_size(cxh_tape t, int64_t n, int64_t checkpoint_every, int32_t block
   (void)block;
   if (t->engine == CXH_ENGINE_JIT) {
     jit_settle(t);
-    *n_doubles = t->alg.empty() ? 0 : jit_ckpt_rows(t, checkpoint_every) * round_up(n, 64);
-    return 0;
+    if (t->alg.empty()) {
+      *n_doubles = 0;
+      return 0;
+    }
+    // which engine the split sweep will run on is settled by building its
+    // kernels (with a device present; without one the size is the plan's)
+    int n_dev = 0;
+    const bool device = hipGetDeviceCount(&n_dev) == hipSuccess && n_dev > 0;
+    if (!device || jit_adj_usable(t, checkpoint_every)) {
+      *n_doubles = jit_ckpt_rows(t, checkpoint_every) * round_up(n, 64);
+      return 0;
+    }
   }
   int seg_len, n_seg;
   segments(t, checkpoint_every, seg_len, n_seg);

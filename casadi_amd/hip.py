@@ -242,6 +242,14 @@ class HipTape:
         L.check(rc, "cxh_tape_jit_pending")
         return rc == 1
 
+    def jit_fallback(self):
+        """Which calls of this specialised tape run on the interpreter because a
+        kernel they need would spill to scratch (cxh_tape_jit_fallback)."""
+        buf = C.create_string_buffer(512)
+        rc = L.cxh_tape_jit_fallback(self._h, buf, 512)
+        L.check(rc, "cxh_tape_jit_fallback")
+        return {"eval": bool(rc & 1), "adjoint": bool(rc & 2), "why": buf.value.decode()}
+
     def ipt(self, ipt):
         """Instances per thread of the specialised forward kernels (1/2/4/8;
         0: the CXH_JIT_IPT setting)."""

@@ -173,6 +173,15 @@ int cxh_tape_jit_prepare(cxh_tape t, int64_t checkpoint_every, int32_t forward, 
  * hides the first call's compile of map(n, "hip").                         */
 int cxh_tape_jit_async(cxh_tape t, int32_t on);
 int cxh_tape_jit_pending(cxh_tape t);
+/* A specialised kernel that would need scratch is not launched.  When that
+ * refusal meets a tape's evaluation (no segment length avoids scratch) or its
+ * adjoint at some checkpoint interval (decided before anything is launched,
+ * and the same for cxh_ckpt_size, so a split sweep keeps one layout), those
+ * calls run on the interpreter from then on, with the same bits.  Returns
+ * bit 1 if evaluation, bit 2 if the adjoint at some interval has fallen back,
+ * and the refusal's message in why (NUL-terminated, truncated to cap; may be
+ * NULL).  Replaces nothing in the reference.                                  */
+int cxh_tape_jit_fallback(cxh_tape t, char* why, int64_t cap);
 /* Introspection: generated HIP source of one segment kernel into buf
  * (NUL-terminated, truncated to cap), its full length in *len and the number
  * of segments.  reverse: 0 the forward kernel, 1 the whole recompute +

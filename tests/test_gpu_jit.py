@@ -308,3 +308,42 @@ def test_jit_wide_live_set_reverse_function(ca, cuda):
             ref = O.map_eval(R, n, [x0.T, us.T, None, sd.T])
             for a, r in zip(yj, ref):
                 assert O.rel_err(a, r.T) <= TOL
+
+
+def test_jit_wide_live_set_fused_adjoint(ca, cuda):
+    """The fused forward + adjoint sweep of that reverse Function (a
+    second-order sweep).  Its reverse kernels would need scratch (every value
+    of the forward half is live across them), so the specialised tape refuses
+    them before launching anything and runs this adjoint on the interpreter:
+    the same bits, and the tape says so."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipTape
+    f = W.cartpole_rollout(10)
+    R = f.reverse(1)
+    n = 300
+    rng = np.random.default_rng(seed_of(42))
+    args = _dev(rng.uniform(-1, 1, (4, n)), rng.uniform(-10, 10, (10, n))) + [None] + \
+        _dev(rng.standard_normal((1, n)))
+    seeds = _dev(rng.standard_normal((4, n)), rng.standard_normal((10, n)))
+    every = 400
+    print(f"[case] fused adjoint of the reverse Function ({R.n_instructions()} instructions) every={every}",
+          flush=True)
+    oi, gi = HipTape(R).fwd_adj(args, seeds, n, checkpoint_every=every)
+    tj = HipTape(R, engine="jit")
+    oj, gj = tj.fwd_adj(args, seeds, n, checkpoint_every=every)
+    for a, b in zip(oi + gi, oj + gj):
+        if a is None:
+            assert b is None
+            continue
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    fb = tj.jit_fallback()
+    print(f"[step] fallback {fb}", flush=True)
+    assert not fb["eval"]
+    if fb["adjoint"]:
+        assert "spills" in fb["why"]
+    # the split sweep sizes and uses one layout, whichever engine runs it
+    outs, ck = tj.fwd_ckpt(args, n, every)
+    g2 = tj.rev_ckpt(args, seeds, n, every, ck)
+    for a, b in zip(gi, g2):
+        if a is not None:
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
