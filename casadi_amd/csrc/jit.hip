@@ -98,6 +98,10 @@ struct CxjArgs {
 #define CXJ_OPAQUE(p) asm volatile("" : "+v"(p))
 #define CXJ_OPAQUE_AFTER(p, x) asm volatile("" : "+v"(p) : "v"(x))
 #endif
+// whether any lane of the wavefront holds b
+#ifndef CXJ_ANY
+#define CXJ_ANY(b) (__builtin_amdgcn_ballot_w64(b) != 0)
+#endif
 #ifdef CXJ_TRIG_CALLS
 #define CXJ_SINCOS(x_, s_, c_) do { const sxh::SinCos r_ = sxh::sxh_sincos_call(x_); s_ = r_.s; c_ = r_.c; } while (0)
 #else

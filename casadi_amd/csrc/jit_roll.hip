@@ -39,10 +39,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -88,6 +90,11 @@ struct RollPlan {
   // per trig key.
   int64_t state_rows() const { return std::max<int64_t>(ns, 1); }
   int64_t step_rows() const { return state_rows() + 2 * static_cast<int64_t>(trig_keys.size()); }
+  // Loads one iteration ahead: each loop issues the next iteration's loads
+  // (its inputs; in the reverse also its state and trig rows and seeds)
+  // before it computes the current one, so that their latency overlaps the
+  // arithmetic instead of stalling each iteration's start.
+  bool prefetch = false;
   std::vector<uint32_t> live;                   // adjoint_live bits of the flat tape
   bool is_live(int64_t k) const { return (live[k >> 5] >> (k & 31)) & 1u; }
 };
@@ -105,6 +112,22 @@ namespace {
 // Whether flat tapes are rerolled (CXH_JIT_ROLL=0: never).
 bool roll_enabled() {
   const char* e = std::getenv("CXH_JIT_ROLL");
+  return !(e && e[0] == '0');
+}
+
+// Whether the loops load one iteration ahead (CXH_JIT_ROLL_PREFETCH=0:
+// never), and the most values an iteration may load for it: the loop then
+// holds two sets in registers.
+bool roll_prefetch_enabled() {
+  const char* e = std::getenv("CXH_JIT_ROLL_PREFETCH");
+  return !(e && e[0] == '0');
+}
+constexpr int64_t kMaxPrefetch = 48;
+
+// Whether the loop bodies divide by constants through sxh_div_c
+// (CXH_JIT_FAST_DIV=0: IEEE division throughout).  The bits are the same.
+bool fast_div_enabled() {
+  const char* e = std::getenv("CXH_JIT_FAST_DIV");
   return !(e && e[0] == '0');
 }
 
@@ -339,6 +362,9 @@ void analyse(const cxh_tape_s* t, RollPlan& L) {
 
 // ------------------------------------------------------------ codegen ---
 
+// Divisors whose divisions sxh_div_c takes: finite, 2^-64 <= |c| <= 2^64.
+bool fast_divisor(double c) { return std::isfinite(c) && std::fabs(c) >= 0x1p-64 && std::fabs(c) <= 0x1p+64; }
+
 // Names: v<k> the value of instruction k (body positions by their block-0
 // instruction a + j), c<e> state entry e read by the body or the epilogue;
 // adjoints a<k>, ac<e>.
@@ -354,9 +380,23 @@ struct Emitter {
   // instead; pair i sits in rows row0 + 2i (sin) and row0 + 2i + 1 (cos) of
   // `st`.
   int trig_io = 0;
+  // Iteration loads: when set, every load of the body (inputs, trig rows,
+  // seeds) is recorded here and read from in_[i] instead, the caller issuing
+  // the recorded expressions itself (cxj_load_fwd / cxj_load_rev).
+  std::vector<std::string>* pre = nullptr;
+  // Divisions by a constant of moderate magnitude through sxh_div_c (the
+  // function declares `bool& wide`); div_c counts them.
+  bool fast_div = false;
+  int64_t div_c = 0;
   char buf[320];
 
   Emitter(const cxh_tape_s* t_, const RollPlan& L_, std::string& o_, Region r) : t(t_), L(L_), o(o_), region(r) {}
+
+  std::string load(const std::string& expr) const {
+    if (!pre) return expr;
+    pre->push_back(expr);
+    return "in_[" + std::to_string(pre->size() - 1) + "]";
+  }
 
   // Operand w (1 or 2) of instruction k: its value name, adjoint name ("" for
   // a constant: no adjoint) and a key for the sin/cos pairing.
@@ -459,7 +499,8 @@ struct Emitter {
           o += "  const double v" + ks + " = " + lit(e.d) + ";\n";
           break;
         case OP_INPUT:
-          o += "  const double v" + ks + " = CXJ_IN(" + std::to_string(e.i1) + ", " + row(k, e.i2) + ");\n";
+          o += "  const double v" + ks + " = " + load("CXJ_IN(" + std::to_string(e.i1) + ", " + row(k, e.i2) + ")") +
+               ";\n";
           break;
         case OP_OUTPUT:
           if (outs) o += "  if (outs) CXJ_OUT(" + std::to_string(e.i0) + ", " + row(k, e.i2) + ", " + x + ");\n";
@@ -467,6 +508,12 @@ struct Emitter {
         default:
           if (op_is_binary(e.op)) operand(k, 2, y, ay, ky);
           else y = x;
+          if (fast_div && e.op == OP_DIV && ay.empty() && fast_divisor(t->alg[ky].d)) {
+            const double c = t->alg[ky].d;
+            o += "  const double v" + ks + " = sxh::sxh_div_c(" + x + ", " + lit(c) + ", " + lit(1.0 / c) + ", wide);\n";
+            ++div_c;
+            break;
+          }
           std::snprintf(buf, sizeof buf, "  const double v%s = sxh::op_fun_full(%d, %s, %s);\n", ks.c_str(), e.op,
                         x.c_str(), y.c_str());
           o += buf;
@@ -484,8 +531,8 @@ struct Emitter {
       const std::string id = std::to_string(k), i = std::to_string(2 * (pos - L.trig_keys.begin()));
       it = trig.emplace(key, std::make_pair("ts" + id, "tc" + id)).first;
       if (trig_io == 2) {
-        o += "  const double ts" + id + " = CXJ_SNAP(st, row0 + " + i + ");\n  const double tc" + id +
-             " = CXJ_SNAP(st, row0 + " + i + " + 1);\n";
+        o += "  const double ts" + id + " = " + load("CXJ_SNAP(st, row0 + " + i + ")") + ";\n  const double tc" + id +
+             " = " + load("CXJ_SNAP(st, row0 + " + i + " + 1)") + ";\n";
       } else {
         o += "  double ts" + id + ", tc" + id + ";\n  CXJ_SINCOS(" + xname + ", ts" + id + ", tc" + id + ");\n";
         o += "  if (store_trig) {\n    CXJ_SNAP(st, row0 + " + i + ") = ts" + id + ";\n    CXJ_SNAP(st, row0 + " + i +
@@ -519,7 +566,7 @@ struct Emitter {
           break;
         case OP_OUTPUT:
           operand(k, 1, x, ax, kx);
-          acc(ax, "CXJ_SEED(" + std::to_string(e.i0) + ", " + row(k, e.i2) + ")");
+          acc(ax, load("CXJ_SEED(" + std::to_string(e.i0) + ", " + row(k, e.i2) + ")"));
           break;
         case OP_INPUT:
           o += "  CXJ_SENS(" + std::to_string(e.i1) + ", " + row(k, e.i2) + ", " + (decl.count(an) ? an : "0.0") +
@@ -582,7 +629,19 @@ std::string load_state(const RollPlan& L, const char* arr) {
   return o;
 }
 
-std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev) {
+// A load function: in_[i] = the i-th recorded load expression of an iteration.
+std::string load_function(const char* name, const char* params, const std::string& head,
+                          const std::vector<std::string>& loads) {
+  std::string o = std::string("__device__ __forceinline__ void ") + name + "(const CxjArgs& A, const long long k, " +
+                  params + ", double* in_) {\n" + head;
+  for (size_t i = 0; i < loads.size(); ++i) o += "  in_[" + std::to_string(i) + "] = " + loads[i] + ";\n";
+  return o + "}\n";
+}
+
+// The prelude, iteration and epilogue functions.  The iteration functions
+// read what they load from in_ (cxj_load_fwd / cxj_load_rev fill it); *nf and
+// *nr receive how many values each loads.
+std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int64_t* nf, int64_t* nr) {
   const int64_t a = L.a, P = L.P, epi0 = a + L.R * P, n = L.n;
   std::string o;
   {  // prelude forward: outputs, the state before the loop
@@ -595,18 +654,48 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev) {
            ";\n";
     o += "}\n";
   }
+  // The iteration functions come in two variants when the body divides by
+  // constants: `_f` divides through sxh_div_c and reports in `wide` a
+  // dividend outside its exact range, `_x` divides as IEEE 754 does; the
+  // wrapper the loops call runs `_f` and, when any lane of the wavefront
+  // reported one, the whole iteration again with `_x` (the iteration reads
+  // only its arguments and loads, and the rerun rewrites whatever it stored).
+  // Without such divisions the one variant is the function itself.
+  const bool fast = fast_div_enabled();
   {  // one iteration forward
-    o += "__device__ __forceinline__ void cxj_step_fwd(const CxjArgs& A, const long long k, const long long step, "
-         "const double* s, double* sn, const bool outs, double* st, const long long row0, const bool store_trig) {\n";
-    o += load_state(L, "s");
-    Emitter E(t, L, o, Emitter::BODY);
-    E.trig_io = L.trig_keys.empty() ? 0 : 1;
-    E.declare_constants(a, a + P);
-    E.forward(a, a + P, true, false);
-    for (int64_t e = 0; e < L.ns; ++e)
-      o += "  sn[" + std::to_string(e) + "] = " +
-           (L.st_pos[e] >= 0 ? "v" + std::to_string(a + L.st_pos[e]) : "c" + std::to_string(e)) + ";\n";
-    o += "}\n";
+    const std::string params = "const CxjArgs& A, const long long k, const long long step, const double* s, double* sn, "
+                               "const bool outs, double* st, const long long row0, const bool store_trig, "
+                               "const double* in_";
+    std::vector<std::string> loads;
+    auto body = [&](const std::string& name, bool fast_div) {
+      std::string f = "__device__ __forceinline__ void " + name + "(" + params + (fast_div ? ", bool& wide" : "") +
+                      ") {\n" + load_state(L, "s");
+      loads.clear();
+      Emitter E(t, L, f, Emitter::BODY);
+      E.trig_io = L.trig_keys.empty() ? 0 : 1;
+      E.pre = &loads;
+      E.fast_div = fast_div;
+      E.declare_constants(a, a + P);
+      E.forward(a, a + P, true, false);
+      for (int64_t e = 0; e < L.ns; ++e)
+        f += "  sn[" + std::to_string(e) + "] = " +
+             (L.st_pos[e] >= 0 ? "v" + std::to_string(a + L.st_pos[e]) : "c" + std::to_string(e)) + ";\n";
+      return std::make_pair(f + "}\n", E.div_c);
+    };
+    const auto fv = body("cxj_step_fwd_f", true);
+    if (fast && fv.second > 0) {
+      o += fv.first + body("cxj_step_fwd_x", false).first;
+      o += "__device__ __forceinline__ void cxj_step_fwd(" + params +
+           ") {\n"
+           "  bool wide = false;\n"
+           "  cxj_step_fwd_f(A, k, step, s, sn, outs, st, row0, store_trig, in_, wide);\n"
+           "  if (CXJ_ANY(wide)) cxj_step_fwd_x(A, k, step, s, sn, outs, st, row0, store_trig, in_);\n"
+           "}\n";
+    } else {
+      o += body("cxj_step_fwd", false).first;
+    }
+    o += load_function("cxj_load_fwd", "const long long step", "", loads);
+    *nf = static_cast<int64_t>(loads.size());
   }
   {  // epilogue forward: outputs from the state after the loop
     o += "__device__ __forceinline__ void cxj_epi_fwd(const CxjArgs& A, const long long k, const double* s, const "
@@ -617,6 +706,7 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev) {
     E.forward(epi0, n, true, false);
     o += "}\n";
   }
+  *nr = 0;
   if (!rev) return o;
   {  // epilogue reverse: the adjoint of the state after the loop
     o += "__device__ __forceinline__ void cxj_epi_rev(const CxjArgs& A, const long long k, const double* s, double* "
@@ -633,28 +723,59 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev) {
     o += "}\n";
   }
   {  // one iteration reverse: as holds the adjoint of the state the iteration
-     // hands on, on return that of the state it received
-    o += "__device__ __forceinline__ void cxj_step_rev(const CxjArgs& A, const long long k, const long long step, "
-         "const double* s, double* as, const double* st, const long long row0) {\n";
-    o += load_state(L, "s");
-    Emitter E(t, L, o, Emitter::BODY);
-    E.trig_io = L.trig_keys.empty() ? 0 : 2;
-    E.declare_constants(a, a + P);
-    E.forward(a, a + P, false, true);
-    for (int64_t e = 0; e < L.ns; ++e) {  // adjoints handed over, where they are structurally nonzero
-      const std::string es = std::to_string(e);
-      const int64_t def = L.st_pos[e] >= 0 ? a + L.st_pos[e] : L.st_init[e];
-      if (!L.is_live(def)) continue;
-      const std::string nm = L.st_pos[e] >= 0 ? "a" + std::to_string(def) : "ac" + es;
-      o += "  double " + nm + " = as[" + es + "];\n";
-      E.decl.insert(nm);
+     // hands on, on return that of the state it received; its stored state
+     // (rows srow + e of st) comes first in in_
+    const std::string params = "const CxjArgs& A, const long long k, const long long step, double* as, const double* in_";
+    std::vector<std::string> loads;
+    auto body = [&](const std::string& name, bool fast_div) {
+      std::string f = "__device__ __forceinline__ void " + name + "(" + params + (fast_div ? ", bool& wide" : "") +
+                      ") {\n" + load_state(L, "in_");
+      loads.clear();
+      for (int64_t e = 0; e < L.ns; ++e) loads.push_back("CXJ_SNAP(st, srow + " + std::to_string(e) + ")");
+      Emitter E(t, L, f, Emitter::BODY);
+      E.trig_io = L.trig_keys.empty() ? 0 : 2;
+      E.pre = &loads;
+      E.fast_div = fast_div;
+      E.declare_constants(a, a + P);
+      E.forward(a, a + P, false, true);
+      for (int64_t e = 0; e < L.ns; ++e) {  // adjoints handed over, where they are structurally nonzero
+        const std::string es = std::to_string(e);
+        const int64_t def = L.st_pos[e] >= 0 ? a + L.st_pos[e] : L.st_init[e];
+        if (!L.is_live(def)) continue;
+        const std::string nm = L.st_pos[e] >= 0 ? "a" + std::to_string(def) : "ac" + es;
+        f += "  double " + nm + " = as[" + es + "];\n";
+        E.decl.insert(nm);
+      }
+      E.reverse(a, a + P);
+      for (int64_t e = 0; e < L.ns; ++e) {
+        const std::string an = "ac" + std::to_string(e);
+        f += "  as[" + std::to_string(e) + "] = " + (E.decl.count(an) ? an : "0.0") + ";\n";
+      }
+      return std::make_pair(f + "}\n", E.div_c);
+    };
+    const auto fv = body("cxj_step_rev_f", true);
+    if (fast && fv.second > 0) {
+      const std::string ns = std::to_string(std::max<int64_t>(L.ns, 1)), nsi = std::to_string(L.ns);
+      o += fv.first + body("cxj_step_rev_x", false).first;
+      o += "__device__ __forceinline__ void cxj_step_rev(" + params +
+           ") {\n"
+           "  double as0[" + ns + "];\n"
+           "  for (int j = 0; j < " + nsi + "; ++j) as0[j] = as[j];\n"
+           "  bool wide = false;\n"
+           "  cxj_step_rev_f(A, k, step, as, in_, wide);\n"
+           "  if (CXJ_ANY(wide)) {\n"
+           "    for (int j = 0; j < " + nsi + "; ++j) as[j] = as0[j];\n"
+           "    cxj_step_rev_x(A, k, step, as, in_);\n"
+           "  }\n"
+           "}\n";
+    } else {
+      o += body("cxj_step_rev", false).first;
     }
-    E.reverse(a, a + P);
-    for (int64_t e = 0; e < L.ns; ++e) {
-      const std::string an = "ac" + std::to_string(e);
-      o += "  as[" + std::to_string(e) + "] = " + (E.decl.count(an) ? an : "0.0") + ";\n";
-    }
-    o += "}\n";
+    // the iteration's trig pairs sit after its state, from row srow + state_rows
+    o += load_function("cxj_load_rev", "const long long step, const double* st, const long long srow",
+                       "  const long long row0 = srow + " + std::to_string(L.state_rows()) + ";\n  (void)row0;\n",
+                       loads);
+    *nr = static_cast<int64_t>(loads.size());
   }
   {  // prelude reverse: input sensitivities from the adjoint of the state before the loop
     o += "__device__ __forceinline__ void cxj_pre_rev(const CxjArgs& A, const long long k, const double* as) {\n";
@@ -689,45 +810,74 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
   if (K <= 0) K = L.R;  // evaluation: no checkpoints
   std::string o = kSxOpsSource;
   o += kPrologue;
-  o += gen_functions(t, L, mode == 4);
+  int64_t nf = 0, nr = 0;
+  o += gen_functions(t, L, mode == 4, &nf, &nr);
   const int64_t S = (L.R + K - 1) / K;
   const std::string ns = std::to_string(std::max<int64_t>(L.ns, 1)), nsi = std::to_string(L.ns);
   const std::string R = std::to_string(L.R), Ks = std::to_string(K), Ss = std::to_string(S);
   // iteration-state buffers: rs rows per iteration, the trig pairs from row ns1
   const std::string rs = std::to_string(L.step_rows()), ns1 = std::to_string(L.state_rows());
   const bool trig = !L.trig_keys.empty();
+  const bool pf = L.prefetch && nf <= kMaxPrefetch && nr <= kMaxPrefetch;
+  const std::string nfa = std::to_string(std::max<int64_t>(nf, 1)), nra = std::to_string(std::max<int64_t>(nr, 1));
+  // A loop over `step` from `first` to `last` (inclusive; down: decreasing)
+  // running body(buffer), text that uses `step` and reads its iteration's
+  // loads from the buffer that load(step expression, buffer) fills.  With
+  // prefetch the first iteration's loads are issued before the loop and each
+  // iteration issues the next one's into the other of two buffers before its
+  // arithmetic, the next step clamped to `last` so that every load stays
+  // within the loop's rows.  A trip holds two iterations, so the buffers
+  // alternate without copies (a copy would wait for the loads in flight).
+  // Without prefetch each iteration loads into b0 at its start.
+  using Text = std::function<std::string(const std::string&, const std::string&)>;
+  auto iterate = [&](bool down, const std::string& first, const std::string& last, const Text& load,
+                     const std::function<std::string(const std::string&)>& body) {
+    if (!pf)
+      return "#pragma unroll 1\n  for (long long step = " + first + "; " + (down ? "step >= " : "step <= ") + last +
+             "; " + (down ? "--step" : "++step") + ") {\n    " + load("step", "b0") + ";\n" + body("b0") + "  }\n";
+    const std::string sg = down ? " - " : " + ", more = down ? " > " : " < ";
+    auto next = [&](const std::string& i) { return "(" + i + more + last + " ? " + i + sg + "1 : " + last + ")"; };
+    return "  {\n  long long i = " + first + ";\n  " + load("i", "b0") + ";\n#pragma unroll 1\n  for (;; i " +
+           (down ? "-" : "+") + "= 2) {\n    " + load(next("i"), "b1") + ";\n    {\n    const long long step = i;\n" + body("b0") +
+           "    }\n    if (i == " + last + ") break;\n    " + load(next("(i" + sg + "1)"), "b0") +
+           ";\n    {\n    const long long step = i" + sg + "1;\n" + body("b1") + "    }\n    if (i" + sg + "1 == " +
+           last + ") break;\n  }\n  }\n";
+  };
+  const Text load_fwd = [](const std::string& x, const std::string& b) {
+    return "cxj_load_fwd(A, k, " + x + ", " + b + ")";
+  };
+  const Text load_rev = [&](const std::string& x, const std::string& b) {
+    return "cxj_load_rev(A, k, " + x + ", A.st_out, (" + x + " - t0) * " + rs + ", " + b + ")";
+  };
   if (mode == 4) {
     o += "__device__ __forceinline__ void cxj_back(const CxjArgs& A, const long long k, const long long seg) {\n"
-         "  double s[" + ns + "], as[" + ns + "];\n"
+         "  double s[" + ns + "], as[" + ns + "], b0[" + nra + "], b1[" + nra + "];\n"
          "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + R + ") ? t0 + " + Ks + " : " + R + ";\n"
          "  if (seg == " + Ss + " - 1) {\n"
          "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, " + Ss + " * " + nsi + " + j);\n"
          "    cxj_epi_rev(A, k, s, as);\n"
          "  } else {\n"
          "    for (int j = 0; j < " + nsi + "; ++j) as[j] = CXJ_ADJ_IN(j);\n"
-         "  }\n"
-         "#pragma unroll 1\n"
-         "  for (long long step = t1 - 1; step >= t0; --step) {\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + rs + " + j);\n"
-         "    cxj_step_rev(A, k, step, s, as, A.st_out, (step - t0) * " + rs + " + " + ns1 + ");\n"
-         "  }\n"
+         "  }\n" +
+         iterate(true, "t1 - 1", "t0", load_rev,
+                 [](const std::string& b) { return "    cxj_step_rev(A, k, step, as, " + b + ");\n"; }) +
          "  if (seg == 0) cxj_pre_rev(A, k, as);\n"
          "  else for (int j = 0; j < " + nsi + "; ++j) CXJ_ADJ_OUT(j) = as[j];\n"
          "}\n"
          "__device__ __forceinline__ void cxj_recompute(const CxjArgs& A, const long long k, const long long seg) {\n"
-         "  double s[" + ns + "], sn[" + ns + "];\n"
+         "  double s[" + ns + "], sn[" + ns + "], b0[" + nfa + "], b1[" + nfa + "];\n"
          "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + R + ") ? t0 + " + Ks + " : " + R + ";\n"
-         "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n"
-         "#pragma unroll 1\n"
-         "  for (long long step = t0; step < t1; ++step) {\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + rs + " + j) = s[j];\n" +
-         // with trig rows the last iteration runs too: its reverse reads its pairs
-         (trig ? std::string("    {\n") : std::string("    if (step + 1 < t1) {\n")) +
-         "      cxj_step_fwd(A, k, step, s, sn, false, A.st_out, " +
-         (trig ? "(step - t0) * " + rs + " + " + ns1 + ", true" : std::string("0, false")) + ");\n"
-         "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-         "    }\n"
-         "  }\n"
+         "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n" +
+         iterate(false, "t0", "t1 - 1", load_fwd, [&](const std::string& b) {
+           return "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + rs + " + j) = s[j];\n" +
+                  // with trig rows the last iteration runs too: its reverse reads its pairs
+                  (trig ? std::string("    {\n") : std::string("    if (step + 1 < t1) {\n")) +
+                  "      cxj_step_fwd(A, k, step, s, sn, false, A.st_out, " +
+                  (trig ? "(step - t0) * " + rs + " + " + ns1 + ", true" : std::string("0, false")) + ", " + b +
+                  ");\n"
+                  "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+                  "    }\n";
+         }) +
          "}\n"
          "extern \"C\" __global__ void __launch_bounds__(256) cxj_roll_pair(const CxjArgs A, const CxjArgs B, "
          "const long long seg) {\n"
@@ -742,32 +892,33 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
     return finish_source(std::move(o));
   }
   o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
-       "  double s[" + ns + "], sn[" + ns + "];\n"
+       "  double s[" + ns + "], sn[" + ns + "], b0[" + nfa + "], b1[" + nfa + "];\n"
        "  cxj_pre_fwd(A, k, s, true);\n";
-  // one loop over [from, to): the checkpoint every K iterations (mode 2),
-  // with states the iteration states of the last segment and, with trig,
-  // their trig rows
+  // one loop over [from, to) (not empty): the checkpoint every K iterations
+  // (mode 2), with states the iteration states of the last segment and, with
+  // trig, their trig rows
   const std::string last0 = std::to_string((S - 1) * K);
   auto loop = [&](const std::string& from, const std::string& to, bool states, bool trig_rows) {
-    o += "#pragma unroll 1\n"
-         "  for (long long step = " + from + "; step < " + to + "; ++step) {\n";
-    if (mode == 2)
-      o += "    if (step % " + Ks + " == 0)\n"
-           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n";
-    if (states)
-      o += "    if (A.st_out && step >= " + last0 + ")\n"
-           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + last0 + ") * " + rs +
-           " + j) = s[j];\n";
-    o += "    cxj_step_fwd(A, k, step, s, sn, true, " +
-         (trig_rows ? "A.st_out, (step - " + last0 + ") * " + rs + " + " + ns1 + ", true" : std::string("nullptr, 0, false")) +
-         ");\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-         "  }\n";
+    o += iterate(false, from, to + " - 1", load_fwd, [&](const std::string& b) {
+      std::string x;
+      if (mode == 2)
+        x += "    if (step % " + Ks + " == 0)\n"
+             "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n";
+      if (states)
+        x += "    if (A.st_out && step >= " + last0 + ")\n"
+             "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + last0 + ") * " + rs +
+             " + j) = s[j];\n";
+      return x + "    cxj_step_fwd(A, k, step, s, sn, true, " +
+             (trig_rows ? "A.st_out, (step - " + last0 + ") * " + rs + " + " + ns1 + ", true"
+                        : std::string("nullptr, 0, false")) +
+             ", " + b + ");\n"
+             "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n";
+    });
   };
   if (mode == 2) {
     o += "  double* ck = const_cast<double*>(A.st_in);\n";
     if (trig) {  // the iterations before the last segment without a branch on the trig stores
-      loop("0", last0, false, false);
+      if (S > 1) loop("0", last0, false, false);
       o += "  if (A.st_out) {\n";
       loop(last0, R, true, true);
       o += "  } else {\n";
@@ -797,6 +948,7 @@ RollTape& roll_state(cxh_tape_s* t) {
   if (!r.analysed) {
     analyse(t, r.plan);
     if (r.plan.ok && trig_cache_enabled()) r.plan.trig_keys = trig_keys(t, r.plan);
+    r.plan.prefetch = roll_prefetch_enabled();
     r.use = r.plan.ok && roll_enabled();
     r.analysed = true;
   }

@@ -171,6 +171,52 @@ SXH_HD double sxh_erfinv(double x) {
   return y;
 }
 
+// x / y for 2^-900 <= |x| <= 2^900 and 2^-64 <= |y| <= 2^64, given yi = the
+// correctly rounded 1 / y: the correctly rounded quotient, by two corrections
+// of q = x * yi with exact FMA remainders (x - y q is representable in that
+// range).  q is within 1.5 ulp of x / y; after the first correction it is
+// within half an ulp plus ~2^-51 ulp (faithful), and then q + (x - y q) yi
+// rounded to nearest is the correctly rounded x / y (Markstein's theorem: a
+// faithful quotient and a reciprocal within half an ulp; P. Markstein,
+// "Computation of elementary functions on the IBM RISC System/6000
+// processor", IBM J. Res. Dev. 34(1), 1990).
+SXH_HD double sxh_div_markstein(double x, double y, double yi) {
+  double q = x * yi;
+  q = fma(fma(-q, y, x), yi, q);
+  return fma(fma(-q, y, x), yi, q);
+}
+
+// x / c for a constant c with 2^-64 <= |c| <= 2^64 and ci = RN(1 / c), on
+// the device: sxh_div_markstein on the magnitudes, then v_div_fixup, which
+// gives the sign and the IEEE results of zero, infinite and NaN dividends.
+// That is x / c bit for bit except for finite nonzero |x| outside
+// [2^-900, 2^900] (exponent from frexp outside [-899, 901]), where the
+// remainders may round; such a dividend sets `wide`, and the caller then
+// evaluates again with IEEE division.  Short dependent chain (multiply, four
+// FMAs, fixup) and no branch: the kernels that use it are latency-bound
+// single wavefronts whose scheduler needs long straight-line blocks.
+#if defined(__HIPCC_RTC__) || defined(__HIP_DEVICE_COMPILE__)
+SXH_HD double sxh_div_c(double x, double c, double ci, bool& wide) {
+  wide |= static_cast<unsigned>(__builtin_amdgcn_frexp_exp(x) + 899) > 1800u;
+  return __builtin_amdgcn_div_fixup(sxh_div_markstein(fabs(x), fabs(c), fabs(ci)), c, x);
+}
+#else
+// The same on the host (the kernels' host emulation): frexp's exponent is 0
+// for zero, infinity and NaN as v_frexp_exp_i32_f64 gives it, and the fixup
+// follows v_div_fixup_f64 for a finite nonzero divisor c and a quotient q >= 0.
+SXH_HD double sxh_div_c(double x, double c, double ci, bool& wide) {
+  int e = 0;
+  if (std::isfinite(x)) std::frexp(x, &e);
+  wide |= static_cast<unsigned>(e + 899) > 1800u;
+  const double q = sxh_div_markstein(std::fabs(x), std::fabs(c), std::fabs(ci));
+  const bool neg = std::signbit(x) != std::signbit(c);
+  if (std::isnan(x)) return x + x;
+  if (std::isinf(x)) return neg ? -HUGE_VAL : HUGE_VAL;
+  if (x == 0) return neg ? -0.0 : 0.0;
+  return neg ? -q : q;
+}
+#endif
+
 // sin(x) and cos(x) from one argument reduction on the device (the values
 // equal sin(x) and cos(x) computed separately); two calls on the host.
 SXH_HD void sxh_sincos(double x, double* s, double* c) {

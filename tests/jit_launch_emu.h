@@ -129,6 +129,8 @@ struct Ptr {
 #define CXJ_AFTER(p, x) ((void)0)
 #define CXJ_OPAQUE(p) ((void)0)
 #define CXJ_OPAQUE_AFTER(p, x) ((void)0)
+// lanes run one at a time: a wavefront-wide vote is the lane's own value
+#define CXJ_ANY(b) (b)
 
 extern "C" {
 // Register a device buffer [p, p + bytes) before the launches that use it.

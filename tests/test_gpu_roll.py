@@ -13,7 +13,7 @@ import pytest
 
 import oracle_ref as O
 from conftest import seed_of
-from test_jit_roll import parametric_rollout, second_order
+from test_jit_roll import division_inputs, division_rollout, parametric_rollout, second_order
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-13
@@ -172,3 +172,33 @@ def test_fused_call_captured_in_a_graph(cuda):
         (Je,), (gxe, gue) = t.fwd_adj([X0.clone(), U.clone()], [S], n, checkpoint_every=every)
         torch.cuda.synchronize()
         assert torch.equal(Jg, Je) and torch.equal(gxg, gxe) and torch.equal(gug, gue), rep
+
+
+def test_fast_constant_divisions_special_dividends(cuda, monkeypatch):
+    """The rerolled loop bodies divide by constants through sxh_div_c (the
+    folded reciprocal and two FMA corrections, v_div_fixup for zeros,
+    infinities and NaN) and re-run an iteration with IEEE division when a
+    lane's dividend leaves the exact range.  Dividends that misround or
+    overflow to NaN without that re-run, zeros, signed zeros, subnormals,
+    infinities and NaN, from the start or at one step, in some wavefronts
+    and not in others: the same bits as the straight-line kernels and the
+    interpreter (IEEE division), NaN where NaN, for values, adjoints, the
+    split form and evaluation."""
+    from casadi_amd.hip import HipTape
+    n_steps, n, every = 40, 1000, 9 * 4
+    f = division_rollout(n_steps)
+    rng = np.random.default_rng(seed_of(2) + 91)
+    ins = division_inputs(n_steps, n, rng)
+    seeds = [rng.standard_normal((f.nnz_out(0), n))]
+    print(f"[case] fast constant divisions n={n} every={every}", flush=True)
+    assert "sxh::sxh_div_c(" in HipTape(f, engine="jit").roll_source(every, 4)
+    rolled = _sweeps(f, ins, seeds, every, "1", monkeypatch)
+    flat = _sweeps(f, ins, seeds, every, "0", monkeypatch)
+
+    def same(a, b):
+        return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+    assert all(same(a, b) for a, b in zip(rolled, flat))
+    outs, g = HipTape(f).fwd_adj(_dev(*ins), _dev(*seeds), checkpoint_every=every)
+    for a, b in zip(rolled[:1 + f.n_in()], _np(outs) + _np(g)):
+        assert same(a, b)
+    assert np.isnan(rolled[0]).any() and np.isinf(rolled[0]).any()

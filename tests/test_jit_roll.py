@@ -44,6 +44,45 @@ def parametric_rollout(n_steps):
     return ca.Function("prollout", [x0, p, us], [ca.vertcat(*outs), ca.sumsqr(x)])
 
 
+def division_rollout(n_steps):
+    """x_{t+1} = x_t / 1.1 + u_t, y_{t+1} = sin(y_t / -3) + x_t / 0.3: every
+    iteration divides by constants, which the loop kernels do through
+    sxh_div_c (exact re-run of an iteration with an out-of-range dividend)."""
+    import casadi_amd as ca
+    x0, us = ca.SX.sym("x0", 2), ca.SX.sym("u", n_steps)
+    x, y, outs = x0[0], x0[1], []
+    for t in range(n_steps):
+        x, y = x / 1.1 + us[t], ca.sin(y / -3.0) + x / 0.3
+        outs += [x, y]
+    return ca.Function("drollout", [x0, us], [ca.vertcat(*outs)])
+
+
+def division_inputs(n_steps, n, rng):
+    """Inputs of division_rollout for n instances: uniform values, with lanes
+    whose dividends are zero, signed zero, subnormal, out of the fast range
+    (2^950, 2^-1000), infinite or NaN from the start, and lanes where one
+    control u_t is such a value, so that only some iterations (and on the
+    GPU some wavefronts) take the exact re-run."""
+    x0 = rng.uniform(-1, 1, (2, n))
+    u = rng.uniform(-1, 1, (n_steps, n))
+    # with u = 0 the first three misround under the fast sequence alone
+    # (subnormal quotients of 1.1 and 0.3; 1.7e308 / 0.3 overflows to NaN
+    # there): only the re-run gives the IEEE bits
+    special = [float.fromhex("0x0.3f20dab6def8dp-1022"), float.fromhex("0x0.452d93f3099fdp-1022"), 1.7e308,
+               0.0, -0.0, 5e-324, -2.5e-310, 2.0 ** 950, -(2.0 ** -1000), np.inf, -np.inf, np.nan]
+    lane = 3
+    for i, v in enumerate(special):
+        x0[:, lane] = v
+        u[:, lane] = 0.0 if i < 3 or v != v or abs(v) == np.inf else v
+        lane += 7
+    for i, v in enumerate(special):
+        u[(5 * i + 2) % n_steps, lane] = v
+        lane += 3
+    x0[:, 192:256] = 0.0          # a wavefront of zeros: exact without the re-run
+    u[:, 192:256] = 0.0
+    return [x0, u]
+
+
 def second_order(n_steps):
     """x_{t+1} = sin(x_t) x_{t-1} + u_t: each step reads two steps back."""
     import casadi_amd as ca
@@ -85,7 +124,7 @@ def _buffers(emu, f, n, rng, arrays):
     return out
 
 
-def _launch_sequence(f, every, n, seed_val, mode):
+def _launch_sequence(f, every, n, seed_val, mode, inputs=None):
     """The rerolled kernels launched as the driver launches them, on the
     emulator.  mode "eval": the forward chain; "fused": checkpointing forward
     then the pairs from segment S-1; "split": forward storing the checkpoints
@@ -97,7 +136,7 @@ def _launch_sequence(f, every, n, seed_val, mode):
     assert P["ok"]
     K, S, ns = P["K"], P["segments"], max(P["state"], 1)
     rng = np.random.default_rng(seed_val)
-    ins = [rng.uniform(-1, 1, (f.nnz_in(i), n)) for i in range(f.n_in())]
+    ins = [rng.uniform(-1, 1, (f.nnz_in(i), n)) for i in range(f.n_in())] if inputs is None else inputs
     seeds = [rng.standard_normal((f.nnz_out(j), n)) for j in range(f.n_out())]
     srcs = [tape.roll_source(every, 0), tape.roll_source(every, 2), tape.roll_source(every, 4)]
     lds = _round64(n)
@@ -198,3 +237,38 @@ def test_parametric_structure_passes_the_parameter_through():
     P = _roll(f).roll_plan()
     # two states carried, the parameter passed through every iteration
     assert P["ok"] and P["state"] == 3 and P["iterations"] >= 20
+
+
+def _same(a, b):
+    """Equal, or NaN where NaN (the sign of a zero adjoint is left open)."""
+    return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+@pytest.mark.parametrize("mode", ["eval", "fused", "split"])
+def test_fast_constant_divisions_exact_on_special_dividends(mode):
+    """The loop body divides by 1.1, -3 and 0.3 through sxh_div_c; zeros,
+    infinities and NaN come out of its fixup, subnormal and out-of-range
+    dividends re-run the iteration with IEEE division.  Executed on the host
+    (each lane its own vote), in bounds, and bit for bit the oracle's."""
+    n_steps, n = 40, 256
+    f = division_rollout(n_steps)
+    src = _roll(f).roll_source(0, 4)
+    assert "sxh::sxh_div_c(" in src and "cxj_step_rev_x" in src
+    ins = division_inputs(n_steps, n, np.random.default_rng(5))
+    errors, ys, gs, ins, seeds = _launch_sequence(f, 9 * 4, n, 11, mode, inputs=ins)
+    assert errors == 0
+    yr = O.map_eval(f, n, [a.T for a in ins])
+    for y, r in zip(ys, yr):
+        assert _same(y, r.T)
+    if mode == "eval":
+        return
+    _, gr = O.adj_map(f, n, [a.T for a in ins], [s.T for s in seeds])
+    for g, r in zip(gs, gr):
+        assert _same(g, r.T)
+
+
+def test_fast_division_switch(monkeypatch):
+    """CXH_JIT_FAST_DIV=0: IEEE division in the loop bodies, one variant."""
+    monkeypatch.setenv("CXH_JIT_FAST_DIV", "0")
+    src = _roll(division_rollout(40)).roll_source(0, 4)
+    assert "sxh_div_c(" not in src.split("namespace sxh")[-1].split("cxj_pre_fwd")[-1] and "_x(" not in src
