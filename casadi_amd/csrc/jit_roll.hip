@@ -640,8 +640,9 @@ std::string load_function(const char* name, const char* params, const std::strin
 
 // The prelude, iteration and epilogue functions.  The iteration functions
 // read what they load from in_ (cxj_load_fwd / cxj_load_rev fill it); *nf and
-// *nr receive how many values each loads.
-std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int64_t* nf, int64_t* nr) {
+// *nr receive how many values each loads, *fast whether they come in the two
+// variants below.
+std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int64_t* nf, int64_t* nr, bool* fast) {
   const int64_t a = L.a, P = L.P, epi0 = a + L.R * P, n = L.n;
   std::string o;
   {  // prelude forward: outputs, the state before the loop
@@ -654,14 +655,12 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int6
            ";\n";
     o += "}\n";
   }
-  // The iteration functions come in two variants when the body divides by
-  // constants: `_f` divides through sxh_div_c and reports in `wide` a
-  // dividend outside its exact range, `_x` divides as IEEE 754 does; the
-  // wrapper the loops call runs `_f` and, when any lane of the wavefront
-  // reported one, the whole iteration again with `_x` (the iteration reads
-  // only its arguments and loads, and the rerun rewrites whatever it stored).
-  // Without such divisions the one variant is the function itself.
-  const bool fast = fast_div_enabled();
+  // The iteration functions: `_x` divides as IEEE 754 does; when the body
+  // divides by constants, `_f` divides through sxh_div_c and reports in
+  // `wide` a dividend outside its exact range (the loops then run again
+  // with `_x`, see gen_roll).
+  *fast = false;
+  const bool fast_ok = fast_div_enabled();
   {  // one iteration forward
     const std::string params = "const CxjArgs& A, const long long k, const long long step, const double* s, double* sn, "
                                "const bool outs, double* st, const long long row0, const bool store_trig, "
@@ -683,17 +682,11 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int6
       return std::make_pair(f + "}\n", E.div_c);
     };
     const auto fv = body("cxj_step_fwd_f", true);
-    if (fast && fv.second > 0) {
-      o += fv.first + body("cxj_step_fwd_x", false).first;
-      o += "__device__ __forceinline__ void cxj_step_fwd(" + params +
-           ") {\n"
-           "  bool wide = false;\n"
-           "  cxj_step_fwd_f(A, k, step, s, sn, outs, st, row0, store_trig, in_, wide);\n"
-           "  if (CXJ_ANY(wide)) cxj_step_fwd_x(A, k, step, s, sn, outs, st, row0, store_trig, in_);\n"
-           "}\n";
-    } else {
-      o += body("cxj_step_fwd", false).first;
+    if (fast_ok && fv.second > 0) {
+      o += fv.first;
+      *fast = true;
     }
+    o += body("cxj_step_fwd_x", false).first;
     o += load_function("cxj_load_fwd", "const long long step", "", loads);
     *nf = static_cast<int64_t>(loads.size());
   }
@@ -754,23 +747,8 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int6
       return std::make_pair(f + "}\n", E.div_c);
     };
     const auto fv = body("cxj_step_rev_f", true);
-    if (fast && fv.second > 0) {
-      const std::string ns = std::to_string(std::max<int64_t>(L.ns, 1)), nsi = std::to_string(L.ns);
-      o += fv.first + body("cxj_step_rev_x", false).first;
-      o += "__device__ __forceinline__ void cxj_step_rev(" + params +
-           ") {\n"
-           "  double as0[" + ns + "];\n"
-           "  for (int j = 0; j < " + nsi + "; ++j) as0[j] = as[j];\n"
-           "  bool wide = false;\n"
-           "  cxj_step_rev_f(A, k, step, as, in_, wide);\n"
-           "  if (CXJ_ANY(wide)) {\n"
-           "    for (int j = 0; j < " + nsi + "; ++j) as[j] = as0[j];\n"
-           "    cxj_step_rev_x(A, k, step, as, in_);\n"
-           "  }\n"
-           "}\n";
-    } else {
-      o += body("cxj_step_rev", false).first;
-    }
+    if (*fast) o += fv.first;  // the same divisions as the forward iteration
+    o += body("cxj_step_rev_x", false).first;
     // the iteration's trig pairs sit after its state, from row srow + state_rows
     o += load_function("cxj_load_rev", "const long long step, const double* st, const long long srow",
                        "  const long long row0 = srow + " + std::to_string(L.state_rows()) + ";\n  (void)row0;\n",
@@ -811,7 +789,8 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
   std::string o = kSxOpsSource;
   o += kPrologue;
   int64_t nf = 0, nr = 0;
-  o += gen_functions(t, L, mode == 4, &nf, &nr);
+  bool fastv = false;
+  o += gen_functions(t, L, mode == 4, &nf, &nr, &fastv);
   const int64_t S = (L.R + K - 1) / K;
   const std::string ns = std::to_string(std::max<int64_t>(L.ns, 1)), nsi = std::to_string(L.ns);
   const std::string R = std::to_string(L.R), Ks = std::to_string(K), Ss = std::to_string(S);
@@ -830,16 +809,17 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
   // alternate without copies (a copy would wait for the loads in flight).
   // Without prefetch each iteration loads into b0 at its start.
   using Text = std::function<std::string(const std::string&, const std::string&)>;
-  auto iterate = [&](bool down, const std::string& first, const std::string& last, const Text& load,
-                     const std::function<std::string(const std::string&)>& body) {
-    if (!pf)
+  using Body = std::function<std::string(const std::string&)>;
+  auto iterate = [&](bool down, const std::string& first, const std::string& last, const Text& load, const Body& body,
+                     bool prefetch) {
+    if (!prefetch)
       return "#pragma unroll 1\n  for (long long step = " + first + "; " + (down ? "step >= " : "step <= ") + last +
              "; " + (down ? "--step" : "++step") + ") {\n    " + load("step", "b0") + ";\n" + body("b0") + "  }\n";
     const std::string sg = down ? " - " : " + ", more = down ? " > " : " < ";
     auto next = [&](const std::string& i) { return "(" + i + more + last + " ? " + i + sg + "1 : " + last + ")"; };
     return "  {\n  long long i = " + first + ";\n  " + load("i", "b0") + ";\n#pragma unroll 1\n  for (;; i " +
-           (down ? "-" : "+") + "= 2) {\n    " + load(next("i"), "b1") + ";\n    {\n    const long long step = i;\n" + body("b0") +
-           "    }\n    if (i == " + last + ") break;\n    " + load(next("(i" + sg + "1)"), "b0") +
+           (down ? "-" : "+") + "= 2) {\n    " + load(next("i"), "b1") + ";\n    {\n    const long long step = i;\n" +
+           body("b0") + "    }\n    if (i == " + last + ") break;\n    " + load(next("(i" + sg + "1)"), "b0") +
            ";\n    {\n    const long long step = i" + sg + "1;\n" + body("b1") + "    }\n    if (i" + sg + "1 == " +
            last + ") break;\n  }\n  }\n";
   };
@@ -849,18 +829,39 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
   const Text load_rev = [&](const std::string& x, const std::string& b) {
     return "cxj_load_rev(A, k, " + x + ", A.st_out, (" + x + " - t0) * " + rs + ", " + b + ")";
   };
+  // The iteration function of a pass and its trailing argument: the fast
+  // variant, flagging into `wide`, or the IEEE one.
+  auto step_fn = [&](const char* base, bool exact) { return std::string(base) + (fastv && !exact ? "_f" : "_x"); };
+  auto flag = [&](bool exact) { return fastv && !exact ? std::string(", wide") : std::string(); };
+  // loops(exact) once; with fast variants, the fast pass (prefetching, no
+  // branch in the loop body) and, when any lane of the wavefront reported a
+  // dividend outside sxh_div_c's exact range, `restart` and the loops again
+  // with IEEE division: each pass writes every row and output it stores.
+  auto passes = [&](const std::function<std::string(bool)>& loops, const std::string& restart) {
+    if (!fastv) return loops(true);
+    return "  bool wide = false;\n" + loops(false) + "  if (CXJ_ANY(wide)) {\n" + restart + loops(true) + "  }\n";
+  };
   if (mode == 4) {
     o += "__device__ __forceinline__ void cxj_back(const CxjArgs& A, const long long k, const long long seg) {\n"
-         "  double s[" + ns + "], as[" + ns + "], b0[" + nra + "], b1[" + nra + "];\n"
+         "  double s[" + ns + "], as[" + ns + "], as0[" + ns + "], b0[" + nra + "], b1[" + nra + "];\n"
          "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + R + ") ? t0 + " + Ks + " : " + R + ";\n"
          "  if (seg == " + Ss + " - 1) {\n"
          "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, " + Ss + " * " + nsi + " + j);\n"
          "    cxj_epi_rev(A, k, s, as);\n"
          "  } else {\n"
          "    for (int j = 0; j < " + nsi + "; ++j) as[j] = CXJ_ADJ_IN(j);\n"
-         "  }\n" +
-         iterate(true, "t1 - 1", "t0", load_rev,
-                 [](const std::string& b) { return "    cxj_step_rev(A, k, step, as, " + b + ");\n"; }) +
+         "  }\n"
+         "  for (int j = 0; j < " + nsi + "; ++j) as0[j] = as[j];\n" +
+         passes(
+             [&](bool exact) {
+               return iterate(true, "t1 - 1", "t0", load_rev,
+                              [&](const std::string& b) {
+                                return "    " + step_fn("cxj_step_rev", exact) + "(A, k, step, as, " + b +
+                                       flag(exact) + ");\n";
+                              },
+                              pf && !exact);
+             },
+             "    for (int j = 0; j < " + nsi + "; ++j) as[j] = as0[j];\n") +
          "  if (seg == 0) cxj_pre_rev(A, k, as);\n"
          "  else for (int j = 0; j < " + nsi + "; ++j) CXJ_ADJ_OUT(j) = as[j];\n"
          "}\n"
@@ -868,16 +869,21 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
          "  double s[" + ns + "], sn[" + ns + "], b0[" + nfa + "], b1[" + nfa + "];\n"
          "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + R + ") ? t0 + " + Ks + " : " + R + ";\n"
          "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n" +
-         iterate(false, "t0", "t1 - 1", load_fwd, [&](const std::string& b) {
-           return "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + rs + " + j) = s[j];\n" +
-                  // with trig rows the last iteration runs too: its reverse reads its pairs
-                  (trig ? std::string("    {\n") : std::string("    if (step + 1 < t1) {\n")) +
-                  "      cxj_step_fwd(A, k, step, s, sn, false, A.st_out, " +
-                  (trig ? "(step - t0) * " + rs + " + " + ns1 + ", true" : std::string("0, false")) + ", " + b +
-                  ");\n"
-                  "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-                  "    }\n";
-         }) +
+         passes(
+             [&](bool exact) {
+               return iterate(false, "t0", "t1 - 1", load_fwd, [&](const std::string& b) {
+                 return "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + rs +
+                        " + j) = s[j];\n" +
+                        // with trig rows the last iteration runs too: its reverse reads its pairs
+                        (trig ? std::string("    {\n") : std::string("    if (step + 1 < t1) {\n")) + "      " +
+                        step_fn("cxj_step_fwd", exact) + "(A, k, step, s, sn, false, A.st_out, " +
+                        (trig ? "(step - t0) * " + rs + " + " + ns1 + ", true" : std::string("0, false")) + ", " + b +
+                        flag(exact) + ");\n"
+                        "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+                        "    }\n";
+               }, pf && !exact);
+             },
+             "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n") +
          "}\n"
          "extern \"C\" __global__ void __launch_bounds__(256) cxj_roll_pair(const CxjArgs A, const CxjArgs B, "
          "const long long seg) {\n"
@@ -892,14 +898,16 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
     return finish_source(std::move(o));
   }
   o += "__device__ __forceinline__ void cxj_body(const CxjArgs& A, const long long k) {\n"
-       "  double s[" + ns + "], sn[" + ns + "], b0[" + nfa + "], b1[" + nfa + "];\n"
-       "  cxj_pre_fwd(A, k, s, true);\n";
+       "  double s[" + ns + "], s0[" + ns + "], sn[" + ns + "], b0[" + nfa + "], b1[" + nfa + "];\n"
+       "  cxj_pre_fwd(A, k, s, true);\n"
+       "  for (int j = 0; j < " + nsi + "; ++j) s0[j] = s[j];\n";
+  if (mode == 2) o += "  double* ck = const_cast<double*>(A.st_in);\n";
   // one loop over [from, to) (not empty): the checkpoint every K iterations
   // (mode 2), with states the iteration states of the last segment and, with
   // trig, their trig rows
   const std::string last0 = std::to_string((S - 1) * K);
-  auto loop = [&](const std::string& from, const std::string& to, bool states, bool trig_rows) {
-    o += iterate(false, from, to + " - 1", load_fwd, [&](const std::string& b) {
+  auto loop = [&](bool exact, const std::string& from, const std::string& to, bool states, bool trig_rows) {
+    return iterate(false, from, to + " - 1", load_fwd, [&](const std::string& b) {
       std::string x;
       if (mode == 2)
         x += "    if (step % " + Ks + " == 0)\n"
@@ -908,28 +916,22 @@ std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K
         x += "    if (A.st_out && step >= " + last0 + ")\n"
              "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + last0 + ") * " + rs +
              " + j) = s[j];\n";
-      return x + "    cxj_step_fwd(A, k, step, s, sn, true, " +
+      return x + "    " + step_fn("cxj_step_fwd", exact) + "(A, k, step, s, sn, true, " +
              (trig_rows ? "A.st_out, (step - " + last0 + ") * " + rs + " + " + ns1 + ", true"
                         : std::string("nullptr, 0, false")) +
-             ", " + b + ");\n"
+             ", " + b + flag(exact) + ");\n"
              "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n";
-    });
+    }, pf && !exact);
   };
-  if (mode == 2) {
-    o += "  double* ck = const_cast<double*>(A.st_in);\n";
-    if (trig) {  // the iterations before the last segment without a branch on the trig stores
-      if (S > 1) loop("0", last0, false, false);
-      o += "  if (A.st_out) {\n";
-      loop(last0, R, true, true);
-      o += "  } else {\n";
-      loop(last0, R, false, false);
-      o += "  }\n";
-    } else {
-      loop("0", R, true, false);
-    }
-  } else {
-    loop("0", R, false, false);
-  }
+  o += passes(
+      [&](bool exact) {
+        if (mode != 2) return loop(exact, "0", R, false, false);
+        if (!trig) return loop(exact, "0", R, true, false);
+        // the iterations before the last segment without a branch on the trig stores
+        return (S > 1 ? loop(exact, "0", last0, false, false) : std::string()) + "  if (A.st_out) {\n" +
+               loop(exact, last0, R, true, true) + "  } else {\n" + loop(exact, last0, R, false, false) + "  }\n";
+      },
+      "    for (int j = 0; j < " + nsi + "; ++j) s[j] = s0[j];\n");
   if (mode == 2)
     o += "  for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, " + Ss + " * " + nsi + " + j) = s[j];\n";
   o += "  cxj_epi_fwd(A, k, s, true);\n"

@@ -268,7 +268,10 @@ def test_fast_constant_divisions_exact_on_special_dividends(mode):
 
 
 def test_fast_division_switch(monkeypatch):
-    """CXH_JIT_FAST_DIV=0: IEEE division in the loop bodies, one variant."""
+    """CXH_JIT_FAST_DIV=0: IEEE division in the loop bodies, only the `_x`
+    iteration functions and no re-run."""
+    src = _roll(division_rollout(40)).roll_source(0, 4)
+    assert "sxh::sxh_div_c(" in src and "cxj_step_rev_f(" in src and "CXJ_ANY(wide)" in src
     monkeypatch.setenv("CXH_JIT_FAST_DIV", "0")
     src = _roll(division_rollout(40)).roll_source(0, 4)
-    assert "sxh_div_c(" not in src.split("namespace sxh")[-1].split("cxj_pre_fwd")[-1] and "_x(" not in src
+    assert "sxh::sxh_div_c(" not in src and "_f(" not in src and "CXJ_ANY" not in src.split("cxj_pre_fwd")[-1]
