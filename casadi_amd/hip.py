@@ -22,9 +22,17 @@ def _torch():
 
 
 def _stream_ptr(stream):
+    """The HIP stream to launch on: the given torch stream's handle, else the
+    current stream of the current device.  The raw handle query avoids
+    building a Stream object per call (about 3 us of the host side of a
+    small launch); torch builds without it take the object's handle."""
+    if stream is not None:
+        return stream.cuda_stream
     torch = _torch()
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return s.cuda_stream
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return raw(torch.cuda.current_device())
+    return torch.cuda.current_stream().cuda_stream
 
 
 def _ptrs(tensors):
@@ -32,22 +40,30 @@ def _ptrs(tensors):
 
 
 def _check_soa(tensors, nnz, n, what):
+    """The leading dimension of SoA blocks (nnz[i], >= n) after checking type,
+    shape and strides (each tensor attribute read once: this runs on every
+    call's host path)."""
     torch = _torch()
+    Tensor, f64 = torch.Tensor, torch.float64
     ld = None
     for i, t in enumerate(tensors):
         if t is None:
             continue
-        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        if not isinstance(t, Tensor) or t.dtype is not f64 or not t.is_cuda:
             raise TypeError(f"{what}[{i}] must be a float64 CUDA tensor")
-        if t.dim() != 2 or t.shape[0] != nnz[i] or t.shape[1] < n:
-            raise ValueError(f"{what}[{i}] has shape {tuple(t.shape)}, expected ({nnz[i]}, {n})")
-        if nnz[i] > 1 and t.shape[1] > 1 and t.stride(1) != 1:   # one instance: its stride is never used
-            raise ValueError(f"{what}[{i}] must have unit stride along instances")
-        s = t.stride(0) if nnz[i] > 1 else n
-        if ld is None:
-            ld = s
-        elif nnz[i] > 1 and s != ld:
-            raise ValueError(f"{what}: all blocks must share the leading dimension")
+        sh, k = t.shape, nnz[i]
+        if len(sh) != 2 or sh[0] != k or sh[1] < n:
+            raise ValueError(f"{what}[{i}] has shape {tuple(sh)}, expected ({k}, {n})")
+        if k > 1:
+            st = t.stride()
+            if sh[1] > 1 and st[1] != 1:   # one instance: its stride is never used
+                raise ValueError(f"{what}[{i}] must have unit stride along instances")
+            if ld is None:
+                ld = st[0]
+            elif st[0] != ld:
+                raise ValueError(f"{what}: all blocks must share the leading dimension")
+        elif ld is None:
+            ld = n
     return ld if ld is not None else n
 
 
