@@ -42,6 +42,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -367,6 +368,11 @@ static JitPlan& plan_for(cxh_tape_s* t, int64_t every) {
 }
 
 // ------------------------------------------------------------ codegen ---
+
+bool fast_div_enabled() {
+  const char* e = std::getenv("CXH_JIT_FAST_DIV");
+  return !(e && e[0] == '0');
+}
 
 std::string lit(double d) {
   if (std::isnan(d)) return "__builtin_nan(\"\")";
@@ -2528,11 +2534,24 @@ static MaccPlan macc_plan(const cxh_tape_s* t, int64_t N, int n_accum) {
   return M;
 }
 
+// `const double v<k> = v<x> / c` through sxh_div_c (flagging into `wide`)
+// when y is a tape constant the fast division takes; empty otherwise.
+static std::string macc_fast_div(const cxh_tape_s* t, int k, const Instr& e, int64_t x, int64_t y) {
+  if (e.op != OP_DIV || t->alg[y].op != OP_CONST || !fast_divisor(t->alg[y].d)) return std::string();
+  const double c = t->alg[y].d;
+  return "  const double v" + std::to_string(k) + " = sxh::sxh_div_c(v" + std::to_string(x) + ", " + lit(c) + ", " +
+         lit(1.0 / c) + ", wide);\n";
+}
+
 // Forward step body: state s -> sn, per-step inputs/outputs at column block
-// `step`.  Outputs are written when `outs` is true.
-static std::string gen_macc_step_fwd(const cxh_tape_s* t, const MaccPlan& M) {
-  std::string o = "__device__ __forceinline__ void cxj_step_fwd(const CxjArgs& A, const long long k, const long long "
-                  "step, const double* s, double* sn, const bool outs) {\n";
+// `step`.  Outputs are written when `outs` is true.  fast: the variant
+// `cxj_step_fwd_f` dividing by constants through sxh_div_c (*ndiv counts
+// them), else `cxj_step_fwd_x` with IEEE division.
+static std::string gen_macc_step_fwd(const cxh_tape_s* t, const MaccPlan& M, bool fast, int* ndiv = nullptr) {
+  std::string o = std::string("__device__ __forceinline__ void cxj_step_fwd") + (fast ? "_f" : "_x") +
+                  "(const CxjArgs& A, const long long k, const long long step, const double* s, double* sn, "
+                  "const bool outs" + (fast ? ", bool& wide" : "") + ") {\n";
+  if (ndiv) *ndiv = 0;
   std::vector<int64_t> cur(t->ws, 0);
   char buf[320];
   const std::unordered_set<int64_t> pairs = step_trig_pairs(t);
@@ -2569,9 +2588,15 @@ static std::string gen_macc_step_fwd(const cxh_tape_s* t, const MaccPlan& M) {
         break;
       default: {
         const int64_t x = cur[e.i1], y = op_is_binary(e.op) ? cur[e.i2] : x;
-        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, v%lld, v%lld);\n", k, e.op,
-                      static_cast<long long>(x), static_cast<long long>(y));
-        o += buf;
+        const std::string fd = fast ? macc_fast_div(t, k, e, x, y) : std::string();
+        if (!fd.empty()) {
+          o += fd;
+          if (ndiv) ++*ndiv;
+        } else {
+          std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, v%lld, v%lld);\n", k, e.op,
+                        static_cast<long long>(x), static_cast<long long>(y));
+          o += buf;
+        }
         cur[e.i0] = k;
       }
     }
@@ -2584,10 +2609,11 @@ static std::string gen_macc_step_fwd(const cxh_tape_s* t, const MaccPlan& M) {
 // the adjoint of the step's output state on entry and of its input state on
 // exit; seeds of step t's outputs and sensitivities of its inputs are at
 // column block `step`.
-static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
+static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M, bool fast) {
   const int ni = static_cast<int>(t->alg.size());
-  std::string o = "__device__ __forceinline__ void cxj_step_rev(const CxjArgs& A, const long long k, const long long "
-                  "step, const double* s, double* as) {\n";
+  std::string o = std::string("__device__ __forceinline__ void cxj_step_rev") + (fast ? "_f" : "_x") +
+                  "(const CxjArgs& A, const long long k, const long long step, const double* s, double* as" +
+                  (fast ? ", bool& wide" : "") + ") {\n";
   std::vector<int64_t> cur(t->ws, 0), ox(ni, 0), oy(ni, 0);
   char buf[320];
   std::unordered_map<int64_t, std::pair<std::string, std::string>> trig;
@@ -2620,9 +2646,13 @@ static std::string gen_macc_step_rev(const cxh_tape_s* t, const MaccPlan& M) {
       default:
         ox[k] = cur[e.i1];
         oy[k] = op_is_binary(e.op) ? cur[e.i2] : ox[k];
-        std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, v%lld, v%lld);\n", k, e.op,
-                      static_cast<long long>(ox[k]), static_cast<long long>(oy[k]));
-        o += buf;
+        if (fast && !macc_fast_div(t, k, e, ox[k], oy[k]).empty()) {
+          o += macc_fast_div(t, k, e, ox[k], oy[k]);
+        } else {
+          std::snprintf(buf, sizeof buf, "  const double v%d = sxh::op_fun_full(%d, v%lld, v%lld);\n", k, e.op,
+                        static_cast<long long>(ox[k]), static_cast<long long>(oy[k]));
+          o += buf;
+        }
         cur[e.i0] = k;
     }
   }
@@ -2705,8 +2735,27 @@ static const char* const kMaccKernel[] = {"cxj_macc_fwd", "cxj_macc_adj", "cxj_m
 static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, int64_t K) {
   std::string o = kSxOpsSource;
   o += kPrologue;
-  o += gen_macc_step_fwd(t, M);
-  if (mode == 1 || mode == 3 || mode == 4) o += gen_macc_step_rev(t, M);
+  // Modes 0, 2 and 4 (evaluation, checkpointing forward, paired reverse)
+  // divide by tape constants through sxh_div_c in a fast pass and run a loop
+  // again with IEEE division when a lane of the wavefront reported a dividend
+  // outside its exact range (each pass rewrites every row and output it
+  // stores).  The one-launch sweeps (1, 3) divide as IEEE 754 does.
+  int nd = 0;
+  const std::string fwd_f = (mode == 0 || mode == 2 || mode == 4) && fast_div_enabled()
+                                ? gen_macc_step_fwd(t, M, true, &nd) : std::string();
+  const bool fastv = nd > 0;
+  if (fastv) o += fwd_f;
+  o += gen_macc_step_fwd(t, M, false);
+  if (mode == 1 || mode == 3 || mode == 4) {
+    if (fastv) o += gen_macc_step_rev(t, M, true);
+    o += gen_macc_step_rev(t, M, false);
+  }
+  auto fn = [&](const char* base, bool exact) { return std::string(base) + (fastv && !exact ? "_f" : "_x"); };
+  auto flag = [&](bool exact) { return fastv && !exact ? std::string(", wide") : std::string(); };
+  auto passes = [&](const std::function<std::string(bool)>& loops, const std::string& restart) {
+    if (!fastv) return loops(true);
+    return "  bool wide = false;\n" + loops(false) + "  if (CXJ_ANY(wide)) {\n" + restart + loops(true) + "  }\n";
+  };
   const std::string ns = std::to_string(std::max<int64_t>(M.ns, 1)), nsi = std::to_string(M.ns);
   const std::string N = std::to_string(M.N), Ks = std::to_string(K);
   const int64_t S = (M.N + K - 1) / K;
@@ -2723,17 +2772,23 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
     // the recompute of a segment from its checkpoint (A.st_in) storing its
     // step states.  Same arithmetic, in the same order, as mode 1.
     o += "__device__ __forceinline__ void cxj_back(const CxjArgs& A, const long long k, const long long seg) {\n"
-         "  double s[" + ns + "], as[" + ns + "];\n"
+         "  double s[" + ns + "], as[" + ns + "], as0[" + ns + "];\n"
          "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
          "  if (seg == " + std::to_string(S) + " - 1)\n"
          "    for (int j = 0; j < " + nsi + "; ++j) as[j] = 0.0;\n"
          "  else\n"
          "    for (int j = 0; j < " + nsi + "; ++j) as[j] = CXJ_ADJ_IN(j);\n"
-         "#pragma unroll 1\n"
-         "  for (long long step = t1 - 1; step >= t0; --step) {\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j);\n"
-         "    cxj_step_rev(A, k, step, s, as);\n"
-         "  }\n"
+         "  for (int j = 0; j < " + nsi + "; ++j) as0[j] = as[j];\n" +
+         passes(
+             [&](bool x) {
+               return "#pragma unroll 1\n"
+                      "  for (long long step = t1 - 1; step >= t0; --step) {\n"
+                      "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + nsi +
+                      " + j);\n"
+                      "    " + fn("cxj_step_rev", x) + "(A, k, step, s, as" + flag(x) + ");\n"
+                      "  }\n";
+             },
+             "    for (int j = 0; j < " + nsi + "; ++j) as[j] = as0[j];\n") +
          "  if (seg == 0) {\n" + store_sens + "  } else {\n"
          "    for (int j = 0; j < " + nsi + "; ++j) CXJ_ADJ_OUT(j) = as[j];\n"
          "  }\n"
@@ -2741,15 +2796,20 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
          "__device__ __forceinline__ void cxj_recompute(const CxjArgs& A, const long long k, const long long seg) {\n"
          "  double s[" + ns + "], sn[" + ns + "];\n"
          "  const long long t0 = seg * " + Ks + ", t1 = (t0 + " + Ks + " < " + N + ") ? t0 + " + Ks + " : " + N + ";\n"
-         "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n"
-         "#pragma unroll 1\n"
-         "  for (long long step = t0; step < t1; ++step) {\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j) = s[j];\n"
-         "    if (step + 1 < t1) {\n"
-         "      cxj_step_fwd(A, k, step, s, sn, false);\n"
-         "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-         "    }\n"
-         "  }\n"
+         "  for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n" +
+         passes(
+             [&](bool x) {
+               return "#pragma unroll 1\n"
+                      "  for (long long step = t0; step < t1; ++step) {\n"
+                      "    for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + nsi +
+                      " + j) = s[j];\n"
+                      "    if (step + 1 < t1) {\n"
+                      "      " + fn("cxj_step_fwd", x) + "(A, k, step, s, sn, false" + flag(x) + ");\n"
+                      "      for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+                      "    }\n"
+                      "  }\n";
+             },
+             "    for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_in, seg * " + nsi + " + j);\n") +
          "}\n"
          "extern \"C\" __global__ void __launch_bounds__(256) cxj_macc_pair(const CxjArgs A, const CxjArgs B, "
          "const long long seg) {\n"
@@ -2767,19 +2827,23 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
        "  double s[" + ns + "], sn[" + ns + "], as[" + ns + "];\n"
        "  double* ck = const_cast<double*>(A.st_in);\n";
   if (mode != 3) {  // forward chain
-    o += load_x0 +
-         "#pragma unroll 1\n"
-         "  for (long long step = 0; step < " + N + "; ++step) {\n";
-    if (mode != 0)
-      o += "    if (step % " + Ks + " == 0)\n"
-           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " + nsi + " + j) = s[j];\n";
-    if (mode == 2)  // with A.st_out, also the step states of the last segment (what its recompute would store)
-      o += "    if (A.st_out && step >= " + std::to_string((S - 1) * K) + ")\n"
-           "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " + std::to_string((S - 1) * K) +
-           ") * " + nsi + " + j) = s[j];\n";
-    o += "    cxj_step_fwd(A, k, step, s, sn, true);\n"
-         "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
-         "  }\n";
+    o += load_x0 + passes(
+                       [&](bool x) {
+                         std::string l = "#pragma unroll 1\n"
+                                         "  for (long long step = 0; step < " + N + "; ++step) {\n";
+                         if (mode != 0)
+                           l += "    if (step % " + Ks + " == 0)\n"
+                                "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(ck, (step / " + Ks + ") * " +
+                                nsi + " + j) = s[j];\n";
+                         if (mode == 2)  // with A.st_out, also the step states of the last segment (what its recompute would store)
+                           l += "    if (A.st_out && step >= " + std::to_string((S - 1) * K) + ")\n"
+                                "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - " +
+                                std::to_string((S - 1) * K) + ") * " + nsi + " + j) = s[j];\n";
+                         return l + "    " + fn("cxj_step_fwd", x) + "(A, k, step, s, sn, true" + flag(x) + ");\n"
+                                    "    for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
+                                    "  }\n";
+                       },
+                       load_x0);
   }
   if (mode == 1 || mode == 3) {  // reverse over the segments
     o += "  for (int j = 0; j < " + nsi + "; ++j) as[j] = 0.0;\n"
@@ -2791,14 +2855,14 @@ static std::string gen_macc(const cxh_tape_s* t, const MaccPlan& M, int mode, in
          "    for (long long step = t0; step < t1; ++step) {\n"
          "      for (int j = 0; j < " + nsi + "; ++j) CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j) = s[j];\n"
          "      if (step + 1 < t1) {\n"
-         "        cxj_step_fwd(A, k, step, s, sn, false);\n"
+         "        cxj_step_fwd_x(A, k, step, s, sn, false);\n"
          "        for (int j = 0; j < " + nsi + "; ++j) s[j] = sn[j];\n"
          "      }\n"
          "    }\n"
          "#pragma unroll 1\n"
          "    for (long long step = t1 - 1; step >= t0; --step) {\n"
          "      for (int j = 0; j < " + nsi + "; ++j) s[j] = CXJ_SNAP(A.st_out, (step - t0) * " + nsi + " + j);\n"
-         "      cxj_step_rev(A, k, step, s, as);\n"
+         "      cxj_step_rev_x(A, k, step, s, as);\n"
          "    }\n"
          "  }\n" + store_sens;
   }

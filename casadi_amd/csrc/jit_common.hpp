@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cmath>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
@@ -49,6 +50,12 @@ std::unordered_set<int64_t> step_trig_pairs(const cxh_tape_s* t);
 bool emit_trig(std::string& o, const Instr& e, int64_t k, int64_t x, const std::string& xname, bool all_trig,
                const std::unordered_set<int64_t>& pairs,
                std::unordered_map<int64_t, std::pair<std::string, std::string>>& trig);
+
+// Divisions by a tape constant in the loop kernels: whether they go
+// through sxh_div_c (CXH_JIT_FAST_DIV, default on; the same bits either way),
+// and the divisors it takes (finite, 2^-64 <= |c| <= 2^64).
+bool fast_div_enabled();
+inline bool fast_divisor(double c) { return std::isfinite(c) && std::fabs(c) >= 0x1p-64 && std::fabs(c) <= 0x1p+64; }
 
 // hiprtc compile (cached by source) and load; the checked form refuses a
 // kernel that needs scratch, naming the remedy.

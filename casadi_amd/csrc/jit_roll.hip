@@ -124,12 +124,6 @@ bool roll_prefetch_enabled() {
 }
 constexpr int64_t kMaxPrefetch = 48;
 
-// Whether the loop bodies divide by constants through sxh_div_c
-// (CXH_JIT_FAST_DIV=0: IEEE division throughout).  The bits are the same.
-bool fast_div_enabled() {
-  const char* e = std::getenv("CXH_JIT_FAST_DIV");
-  return !(e && e[0] == '0');
-}
 
 void ssa_operands(const cxh_tape_s* t, std::vector<int64_t>& d1, std::vector<int64_t>& d2) {
   const int64_t n = static_cast<int64_t>(t->alg.size());
@@ -361,9 +355,6 @@ void analyse(const cxh_tape_s* t, RollPlan& L) {
 }
 
 // ------------------------------------------------------------ codegen ---
-
-// Divisors whose divisions sxh_div_c takes: finite, 2^-64 <= |c| <= 2^64.
-bool fast_divisor(double c) { return std::isfinite(c) && std::fabs(c) >= 0x1p-64 && std::fabs(c) <= 0x1p+64; }
 
 // Names: v<k> the value of instruction k (body positions by their block-0
 // instruction a + j), c<e> state entry e read by the body or the epilogue;

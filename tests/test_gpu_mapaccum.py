@@ -120,3 +120,42 @@ def test_mapaccum_split_reverse_is_bit_identical(cuda, monkeypatch):
         (o,), (g0, gU) = m.fwd_adj(_dev(X0, U), _dev(S), n, checkpoint_steps=K)
         got[split] = [x.cpu().numpy() for x in (o, g0, gU)]
     assert all(np.array_equal(a, b) for a, b in zip(got["1"], got["0"]))
+
+
+def test_mapaccum_fast_divisions_on_special_dividends(cuda, monkeypatch):
+    """The split sweep's kernels divide the step's constants (1.1) through
+    sxh_div_c and re-run a loop with IEEE division when a lane's dividend
+    leaves the exact range; the fused kernel (CXH_JIT_SPLIT=0) divides as
+    IEEE 754 does throughout.  Lanes whose controls or states make the
+    dividends subnormal (5e-320), huge (1e300), infinite, NaN or zero, among
+    ordinary lanes: the same bits, NaN where NaN, with and without the fast
+    division (CXH_JIT_FAST_DIV=0)."""
+    from casadi_amd import workloads as W
+    from casadi_amd.hip import HipMapAccum
+    N, n, K = 12, 640, 4
+    rng = np.random.default_rng(seed_of(2) + 17)
+    X0 = np.vstack([rng.uniform(-1, 1, (4, n)), np.zeros((1, n))])
+    U = rng.uniform(-3, 3, (N, n))
+    for lane, u in ((64, 5e-320), (65, -5e-320), (130, 1e300), (200, np.inf), (260, np.nan)):
+        X0[:4, lane] = 0.0
+        U[:, lane] = u
+    X0[:, 320:384] = 0.0            # a wavefront of zeros: exact without the re-run
+    U[:, 320:384] = 0.0
+    X0[1, 400] = np.nan
+    U[5, 450] = 5e-320              # one step of one lane
+    S = rng.standard_normal((5 * N, n))
+
+    def same(a, b):
+        return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+    got = {}
+    for name, split, fast in (("split", "1", "1"), ("fused", "0", "1"), ("split_ieee", "1", "0")):
+        monkeypatch.setenv("CXH_JIT_SPLIT", split)
+        monkeypatch.setenv("CXH_JIT_FAST_DIV", fast)
+        m = HipMapAccum(W.cartpole_cost_step(), N)
+        print(f"[case] mapaccum special dividends {name} N={N} K={K} n={n}", flush=True)
+        (o,), (g0, gU) = m.fwd_adj(_dev(X0, U), _dev(S), n, checkpoint_steps=K)
+        (e,) = m.eval(_dev(X0, U), n)
+        got[name] = [x.cpu().numpy() for x in (o, g0, gU, e)]
+    for other in ("fused", "split_ieee"):
+        assert all(same(a, b) for a, b in zip(got["split"], got[other])), other
+    assert np.isnan(got["split"][0]).any() and np.isinf(got["split"][0]).any()

@@ -74,7 +74,7 @@ def _compile(sources, workdir, groups=None):
         loop = ("  for (long long k = 0; k < n; ++k) cxj_body(*A, k);\n" if groups is None else
                 f"  for (int g = 0; g < {groups}; ++g)\n    for (long long k = 0; k < n; ++k) cxj_body(*A, k, g);\n")
         code = ("#define __device__\n#define __forceinline__ inline\n#define CXJ_KEEP(x) ((void)0)\n#define CXJ_KEEP_PTR(x) ((void)0)\n#define CXJ_AFTER(p, x) ((void)0)\n"
-                "#define CXJ_GLOBAL\n#define CXJ_OPAQUE(p) ((void)0)\n#define CXJ_OPAQUE_AFTER(p, x) ((void)0)\n" + src[:cut] +
+                "#define CXJ_GLOBAL\n#define CXJ_OPAQUE(p) ((void)0)\n#define CXJ_OPAQUE_AFTER(p, x) ((void)0)\n#define CXJ_ANY(b) (b)\n" + src[:cut] +
                 f'extern "C" void run_{i}(const CxjArgs* A, long long n) {{\n' + loop + "}\n")
         path = os.path.join(workdir, f"seg{i}.cpp")
         with open(path, "w") as fh:
@@ -284,7 +284,7 @@ def test_mapaccum_loop_kernels_run_on_host(K):
     cut = src.index('extern "C" __global__')
     code = ("#define __device__\n#define __forceinline__ inline\n#define CXJ_KEEP(x) ((void)0)\n"
             "#define CXJ_KEEP_PTR(x) ((void)0)\n#define CXJ_AFTER(p, x) ((void)0)\n#define CXJ_GLOBAL\n"
-            "#define CXJ_OPAQUE(p) ((void)0)\n#define CXJ_OPAQUE_AFTER(p, x) ((void)0)\n" + src[:cut] +
+            "#define CXJ_OPAQUE(p) ((void)0)\n#define CXJ_OPAQUE_AFTER(p, x) ((void)0)\n#define CXJ_ANY(b) (b)\n" + src[:cut] +
             "extern \"C\" void run_pair(const CxjArgs* A, const CxjArgs* B, long long n, long long seg) {\n"
             f"  if (seg < {S}) for (long long k = 0; k < n; ++k) cxj_back(*A, k, seg);\n"
             "  if (seg >= 1) for (long long k = 0; k < n; ++k) cxj_recompute(*B, k, seg - 1);\n}\n")
