@@ -633,7 +633,8 @@ std::string load_function(const char* name, const char* params, const std::strin
 // read what they load from in_ (cxj_load_fwd / cxj_load_rev fill it); *nf and
 // *nr receive how many values each loads, *fast whether they come in the two
 // variants below.
-std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int64_t* nf, int64_t* nr, bool* fast) {
+std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int64_t* nf, int64_t* nr, bool* fast,
+                          bool allow_fast) {
   const int64_t a = L.a, P = L.P, epi0 = a + L.R * P, n = L.n;
   std::string o;
   {  // prelude forward: outputs, the state before the loop
@@ -651,7 +652,7 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int6
   // `wide` a dividend outside its exact range (the loops then run again
   // with `_x`, see gen_roll).
   *fast = false;
-  const bool fast_ok = fast_div_enabled();
+  const bool fast_ok = allow_fast && fast_div_enabled();
   {  // one iteration forward
     const std::string params = "const CxjArgs& A, const long long k, const long long step, const double* s, double* sn, "
                                "const bool outs, double* st, const long long row0, const bool store_trig, "
@@ -775,20 +776,23 @@ std::string gen_functions(const cxh_tape_s* t, const RollPlan& L, bool rev, int6
 // recompute of the segment before it.
 const char* const kRollKernel[] = {"cxj_roll_fwd", "", "cxj_roll_ckpt", "", "cxj_roll_pair"};
 
-std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K) {
+// lean: the variant without the one-iteration-ahead loads and the fast
+// constant divisions (fewer registers: more wavefronts per SIMD for large
+// batches); the same bits.
+std::string gen_roll(const cxh_tape_s* t, const RollPlan& L, int mode, int64_t K, bool lean = false) {
   if (K <= 0) K = L.R;  // evaluation: no checkpoints
   std::string o = kSxOpsSource;
   o += kPrologue;
   int64_t nf = 0, nr = 0;
   bool fastv = false;
-  o += gen_functions(t, L, mode == 4, &nf, &nr, &fastv);
+  o += gen_functions(t, L, mode == 4, &nf, &nr, &fastv, !lean);
   const int64_t S = (L.R + K - 1) / K;
   const std::string ns = std::to_string(std::max<int64_t>(L.ns, 1)), nsi = std::to_string(L.ns);
   const std::string R = std::to_string(L.R), Ks = std::to_string(K), Ss = std::to_string(S);
   // iteration-state buffers: rs rows per iteration, the trig pairs from row ns1
   const std::string rs = std::to_string(L.step_rows()), ns1 = std::to_string(L.state_rows());
   const bool trig = !L.trig_keys.empty();
-  const bool pf = L.prefetch && nf <= kMaxPrefetch && nr <= kMaxPrefetch;
+  const bool pf = !lean && L.prefetch && nf <= kMaxPrefetch && nr <= kMaxPrefetch;
   const std::string nfa = std::to_string(std::max<int64_t>(nf, 1)), nra = std::to_string(std::max<int64_t>(nr, 1));
   // A loop over `step` from `first` to `last` (inclusive; down: decreasing)
   // running body(buffer), text that uses `step` and reads its iteration's
@@ -955,13 +959,33 @@ int64_t roll_K(const RollPlan& L, int64_t every) {
   return std::max<int64_t>(1, std::min<int64_t>(L.R, e / L.P));
 }
 
-hipFunction_t roll_kernel(cxh_tape_s* t, int mode, int64_t K) {
+hipFunction_t roll_kernel(cxh_tape_s* t, int mode, int64_t K, bool lean = false) {
   RollTape& r = roll_state(t);
-  hipFunction_t& f = r.fn[mode * 1000003 + K];
+  hipFunction_t& f = r.fn[(mode * 2 + (lean ? 1 : 0)) * 1000003 + K];
   if (!f)
-    f = jit_function_checked(gen_roll(t, r.plan, mode, K), kRollKernel[mode],
+    f = jit_function_checked(gen_roll(t, r.plan, mode, K, lean), kRollKernel[mode],
                              "set CXH_JIT_ROLL=0 for the straight-line segment kernels or use the interpreter engine");
   return f;
+}
+
+// Whether a batch of n instances runs the lean kernels.  The full ones hold
+// two iterations' loads and both division variants in registers (up to 190
+// VGPRs for the cart-pole pair kernel, against about 64 lean): the best
+// choice while a launch has at most one or two wavefronts per SIMD, and
+// 2-3% slower on the MI355X once it has four or more (262 144 instances and
+// up), where the lean kernels keep more wavefronts resident.  The limit is
+// 128 instances per SIMD (131 072 on 1024 SIMDs); CXH_JIT_ROLL_FAT_MAX sets it
+// in instances.
+bool roll_lean(int64_t n) {
+  if (const char* e = std::getenv("CXH_JIT_ROLL_FAT_MAX")) return n > std::atoll(e);
+  static const int64_t fat_max = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    return static_cast<int64_t>(cus) * 4 * 128;
+  }();
+  return n > fat_max;
 }
 
 }  // namespace
@@ -1002,7 +1026,7 @@ int roll_map_eval(cxh_tape_s* t, int64_t n, const IOTable& arg, const IOTable& r
   A.res = res;
   A.n = n;
   A.lds = round_up(n, 64);
-  launch(roll_kernel(t, 0, 0), n, t->block, A, s);
+  launch(roll_kernel(t, 0, 0, roll_lean(n)), n, t->block, A, s);
   return 0;
 }
 
@@ -1022,7 +1046,7 @@ int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTab
     A.res = res;
     A.st_in = ckpt_ext;
     A.st_out = nullptr;
-    launch(roll_kernel(t, 2, K), n, t->block, A, s);
+    launch(roll_kernel(t, 2, K, roll_lean(n)), n, t->block, A, s);
     return 0;
   }
   // checkpoints (unless given), the iteration states (and trig rows) of two
@@ -1038,10 +1062,10 @@ int roll_adj(cxh_tape_s* t, int mode, int64_t n, const IOTable& arg, const IOTab
     A.res = res;
     A.st_in = ck;
     A.st_out = states + ((S - 1) % 2) * K * rs * lds;
-    launch(roll_kernel(t, 2, K), n, t->block, A, s);
+    launch(roll_kernel(t, 2, K, roll_lean(n)), n, t->block, A, s);
     first = S - 1;
   }
-  hipFunction_t fp = roll_kernel(t, 4, K);
+  hipFunction_t fp = roll_kernel(t, 4, K, roll_lean(n));
   A.res = IOTable{};
   A.aseed = aseed;
   A.asens = asens;

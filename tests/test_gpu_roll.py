@@ -202,3 +202,28 @@ def test_fast_constant_divisions_special_dividends(cuda, monkeypatch):
     for a, b in zip(rolled[:1 + f.n_in()], _np(outs) + _np(g)):
         assert same(a, b)
     assert np.isnan(rolled[0]).any() and np.isinf(rolled[0]).any()
+
+
+def test_lean_kernels_keep_the_bits(cuda, monkeypatch):
+    """Batches above 128 instances per SIMD run the lean loop kernels (no
+    one-iteration-ahead loads, IEEE division: fewer registers, more resident
+    wavefronts); CXH_JIT_ROLL_FAT_MAX=0 forces them at any size.  Lean and
+    full kernels give the same bits: the 20-step cart-pole rollout and the
+    division rollout on special dividends, fused, split and evaluated."""
+    from casadi_amd import workloads as W
+    cases = [(W.cartpole_rollout(20), (W.cartpole_rollout(20).n_instructions() + 3) // 4, None),
+             (division_rollout(40), 9 * 4, "special")]
+    for f, every, kind in cases:
+        n = 1000
+        rng = np.random.default_rng(seed_of(2) + 55)
+        if kind == "special":
+            ins = division_inputs(40, n, rng)
+        else:
+            ins = [rng.uniform(-1, 1, (f.nnz_in(i), n)) for i in range(f.n_in())]
+        seeds = [rng.standard_normal((f.nnz_out(j), n)) for j in range(f.n_out())]
+        print(f"[case] lean kernels {f.name()} n={n}", flush=True)
+        full = _sweeps(f, ins, seeds, every, "1", monkeypatch)
+        monkeypatch.setenv("CXH_JIT_ROLL_FAT_MAX", "0")
+        lean = _sweeps(f, ins, seeds, every, "1", monkeypatch)
+        monkeypatch.delenv("CXH_JIT_ROLL_FAT_MAX")
+        assert all(bool(np.all((a == b) | (np.isnan(a) & np.isnan(b)))) for a, b in zip(full, lean))

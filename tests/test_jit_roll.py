@@ -275,3 +275,35 @@ def test_fast_division_switch(monkeypatch):
     monkeypatch.setenv("CXH_JIT_FAST_DIV", "0")
     src = _roll(division_rollout(40)).roll_source(0, 4)
     assert "sxh::sxh_div_c(" not in src and "_f(" not in src and "CXJ_ANY" not in src.split("cxj_pre_fwd")[-1]
+
+
+def _roll_default_source(f, every):
+    import os
+    saved = {k: os.environ.pop(k, None) for k in ("CXH_JIT_ROLL_PREFETCH", "CXH_JIT_FAST_DIV")}
+    try:
+        return _roll(f).roll_source(every, 4)
+    finally:
+        os.environ.update({k: v for k, v in saved.items() if v is not None})
+
+
+@pytest.mark.parametrize("case", ["rollout20", "division"])
+def test_lean_variant_in_bounds_and_exact(monkeypatch, case):
+    """The kernels large batches run (CXH_JIT_ROLL_PREFETCH=0 and
+    CXH_JIT_FAST_DIV=0 give the same source): each iteration loading at its
+    start, IEEE division, no re-run; in bounds, race-free and the oracle's
+    bits on the host, split sweep."""
+    monkeypatch.setenv("CXH_JIT_ROLL_PREFETCH", "0")
+    monkeypatch.setenv("CXH_JIT_FAST_DIV", "0")
+    if case == "division":
+        f, every, n = division_rollout(40), 9 * 4, 256
+        ins = division_inputs(40, n, np.random.default_rng(5))
+    else:
+        (f, every, n), ins = _cases()["rollout20"], None
+    src = _roll(f).roll_source(every, 4)
+    assert "sxh::sxh_div_c(" not in src and "for (;; i" not in src     # no fast pass, no prefetching loop
+    assert "for (;; i" in _roll_default_source(f, every)
+    errors, ys, gs, ins, seeds = _launch_sequence(f, every, n, 13, "split", inputs=ins)
+    assert errors == 0
+    _, gr = O.adj_map(f, n, [a.T for a in ins], [s.T for s in seeds])
+    for g, r in zip(gs, gr):
+        assert _same(g, r.T)
