@@ -1000,16 +1000,23 @@ int64_t roll_ckpt_rows(cxh_tape_s* t, int64_t every) {
   return ((L.R + K - 1) / K + 1) * std::max<int64_t>(L.ns, 1);
 }
 
+// Both variants (full and lean): the batch size picks one at launch, and an
+// asynchronous build should leave no compile to the first call of either.
 void roll_prepare(cxh_tape_s* t, int64_t every, bool fwd, bool rev) {
   const int64_t K = roll_K(roll_state(t).plan, every);
-  if (fwd) roll_kernel(t, 0, K);
-  if (rev) {
-    roll_kernel(t, 2, K);
-    roll_kernel(t, 4, K);
+  for (const bool lean : {false, true}) {
+    if (fwd) roll_kernel(t, 0, K, lean);
+    if (rev) {
+      roll_kernel(t, 2, K, lean);
+      roll_kernel(t, 4, K, lean);
+    }
   }
 }
 
-void roll_prepare_eval(cxh_tape_s* t) { roll_kernel(t, 0, 0); }
+void roll_prepare_eval(cxh_tape_s* t) {
+  roll_kernel(t, 0, 0, false);
+  roll_kernel(t, 0, 0, true);
+}
 
 void roll_adopt(cxh_tape_s* t, cxh_tape_s& from) {
   if (!from.roll) return;
