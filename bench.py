@@ -442,7 +442,8 @@ def other_config_figures(timed):
     Hessian of the 20-step quadrotor OCP Lagrangian (16384 instances), the
     heterogeneous batch (40000 cart-pole + 12000 quadrotor + 25000 arm
     forward + adjoint, autotuned launch) and the batched KKT solve (n = 274,
-    8192 instances, ldl and qr).  first_call_s includes compiling the kernels;
+    8192 instances, ldl and qr) and the 100-step rollout's cost + gradient at
+    262144 and 1048576 instances.  first_call_s includes compiling the kernels;
     ms are HIP-event times after it.  max_abs_diff compares the two engines
     (the tests hold both to the oracle)."""
     import torch
@@ -526,6 +527,22 @@ def other_config_figures(timed):
         out[f"kkt_{solver}_8192"] = {"n": n, "ms": ms, "instances_per_s": n / (ms * 1e-3), "first_call_s": c_s,
                                      "bytes_per_instance": bpi, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
                                      "generic_ms": ms_g, "max_abs_diff": float((xj - xg).abs().max())}
+    # the same rollout cost + gradient at larger batches: above 128 instances
+    # per SIMD the lean loop kernels run (no loads ahead, IEEE division: more
+    # resident wavefronts), same bits
+    f100 = W.cartpole_rollout(100)
+    every = (f100.n_instructions() + 9) // 10
+    tape = HipTape(f100, engine="jit")
+    fat_max = int(os.environ.get("CXH_JIT_ROLL_FAT_MAX", torch.cuda.get_device_properties(0).multi_processor_count * 512))
+    for n in (262144, 1048576):
+        x0, us = t(rng.uniform(-1, 1, (4, n))), t(rng.uniform(-10, 10, (100, n)))
+        seed = torch.ones((1, n), dtype=torch.float64, device=dev)
+        call = lambda: tape.fwd_adj([x0, us], [seed], checkpoint_every=every)
+        _, t1 = first(call)
+        ms = timed(call, 5, 1)
+        out[f"rollout_{n}"] = {"n": n, "ms": ms, "instances_per_s": n / (ms * 1e-3), "first_call_s": t1,
+                               "kernels": "lean" if n > fat_max else "full"}
+        del x0, us, seed
     return out
 
 
